@@ -1,0 +1,560 @@
+// Exact GPR on one GPU: the NLML + gradient evaluation behind gprx's
+// objective seam, and the predictive moments.
+//
+// One evaluation (src/gpr/exact_fit.rs:319-341, 665-729):
+//   fill      A = K(x, x) + (sigma^2 + j) I            (lower tiles)
+//   factor    L = chol(A), X = L^-1                      (recursive, MFMA)
+//   alpha     alpha = X^T (X y)                          (FP64)   or
+//             alpha refined with f64 residuals           (MIXED, src/precision/refine.rs:56-98)
+//   value     1/2 y'alpha + sum log L_ii + n/2 log 2 pi  (src/gpr/factor.rs:254-267)
+//   inverse   A^-1 = X^T X                               (lower tiles, MFMA)
+//   gradient  -1/2 <alpha alpha' - A^-1, dK/dtheta>      (fused: K recomputed from x)
+//             -1/2 sigma^2 sum_i (alpha_i^2 - A^-1_ii)   (src/gpr/exact_fit.rs:676-682)
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "model.hpp"
+
+using namespace gprx;
+
+namespace {
+
+constexpr double kLog2Pi = 1.8378770664093453;  // log(2 pi)
+
+struct ExactModel : gprx_hip_model {
+  ExactModel() : gprx_hip_model(kExact) {}
+  ~ExactModel() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  HostProgram hp;
+  int n = 0, d = 0, dp = 0, nt = 0;
+  long npad = 0, ldt = 0;
+  int precision = GPRX_FP64;
+  gprx_jitter_policy jit{0.0, 1.0, 0.0, 0};
+  std::vector<double> theta;
+  double last_jitter = 0.0;
+  int fallback = 0, refine_steps = 0;
+  bool factored = false;
+  int cur_form = GPRX_FP64;  // which buffers hold X and alpha
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double phase_ms[5] = {0, 0, 0, 0, 0};
+  DevBuf<double> x, y, ones;
+  DevBuf<DevProgram> prog;
+  HostBuf<DevProgram> hprog;
+  DevBuf<double> A64, X64, T64, K64;
+  DevBuf<float> A32, X32, T32;
+  DevBuf<double> logdiag, alpha, tv, r, dlt, p1, p2, q1, q2, gpart, partials, accs, scal;
+  DevBuf<int> info;
+  HostBuf<double> hscal, hacc;
+  HostBuf<int> hinfo;
+
+  DevPoints pts() const { return DevPoints{x.p, npad, n}; }
+
+  void alloc_fp64() {
+    if (A64.p) return;
+    A64.alloc(npad * npad);
+    X64.alloc(npad * npad);
+    T64.alloc(ldt * (npad / 2 + kTile));
+    GPRX_HIP_CHECK(hipMemsetAsync(A64.p, 0, sizeof(double) * npad * npad, stream));
+    GPRX_HIP_CHECK(hipMemsetAsync(X64.p, 0, sizeof(double) * npad * npad, stream));
+  }
+  void alloc_mixed() {
+    if (A32.p) return;
+    A32.alloc(npad * npad);
+    X32.alloc(npad * npad);
+    T32.alloc(ldt * (npad / 2 + kTile));
+    K64.alloc(npad * npad);
+    GPRX_HIP_CHECK(hipMemsetAsync(A32.p, 0, sizeof(float) * npad * npad, stream));
+    GPRX_HIP_CHECK(hipMemsetAsync(X32.p, 0, sizeof(float) * npad * npad, stream));
+    GPRX_HIP_CHECK(hipMemsetAsync(K64.p, 0, sizeof(double) * npad * npad, stream));
+  }
+
+  void upload_program(const double* th, double jitter) {
+    make_dev_program(hp, th, std::exp(th[hp.n_params]) + jitter, hprog.p);
+    GPRX_HIP_CHECK(hipMemcpyAsync(prog.p, hprog.p, sizeof(DevProgram), hipMemcpyHostToDevice, stream));
+  }
+
+  // Reads back info and the scalars; returns the info bits.
+  int readback(int nscal, bool with_acc) {
+    GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, sizeof(double) * nscal, hipMemcpyDeviceToHost, stream));
+    if (with_acc)
+      GPRX_HIP_CHECK(hipMemcpyAsync(hacc.p, accs.p, sizeof(double) * kMaxAcc, hipMemcpyDeviceToHost, stream));
+    GPRX_HIP_CHECK(hipStreamSynchronize(stream));
+    return *hinfo.p;
+  }
+
+  // FP64 form.  Returns info bits (1: factor failed, 2: non-finite kernel).
+  int run_fp64(bool want_grad) {
+    alloc_fp64();
+    hipStream_t s = stream;
+    GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
+    fill_lower<double>(dp, prog.p, pts(), A64.p, npad, nullptr, info.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
+    factor_inverse<double>(A64.p, X64.p, T64.p, npad, ldt, nt, logdiag.p, info.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
+    gemv_n<double>(X64.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
+    gemv_t<double>(X64.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
+    dot_and_sum(y.p, alpha.p, logdiag.p, n, scal.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[3], s));
+    if (want_grad) {
+      gemm_call<double>(true, false, X64.p, npad, X64.p, npad, A64.p, npad, (int)npad, (int)npad,
+                        (int)npad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
+      GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
+      contract_exact<double>(dp, prog.p, pts(), A64.p, npad, alpha.p, partials.p, s);
+      reduce_partials(partials.p, lower_tiles(npad), accs.p, s);
+      noise_trace<double>(alpha.p, A64.p, npad, n, scal.p + 2, s);
+    } else {
+      GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
+    }
+    GPRX_HIP_CHECK(hipEventRecord(ev[5], s));
+    cur_form = GPRX_FP64;
+    return readback(3, want_grad);
+  }
+
+  // MIXED form.  Returns info bits, or -1 when refinement did not converge.
+  int run_mixed(bool want_grad) {
+    alloc_mixed();
+    hipStream_t s = stream;
+    GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
+    fill_lower<float>(dp, prog.p, pts(), A32.p, npad, K64.p, info.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
+    factor_inverse<float>(A32.p, X32.p, T32.p, npad, ldt, nt, logdiag.p, info.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
+    gemv_n<float>(X32.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
+    gemv_t<float>(X32.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
+    // ||A||_inf from the f64 lower triangle (row sums of |A|).
+    gemv_n<double>(K64.p, npad, (int)npad, (int)npad, ones.p, q1.p, gpart.p, 1, 0, 1, s);
+    gemv_t<double>(K64.p, npad, (int)npad, (int)npad, ones.p, q2.p, 1, 1, 1, 0, s);
+    int bits = readback(0, false);
+    if (bits) return bits;
+    // Iterative refinement (src/precision/refine.rs:56-98): residual in f64
+    // against the f64 lower triangle, corrections through the f32 inverse.
+    const double tol = 10.0 * (double)n * 2.220446049250313e-16;
+    double prev = -1.0;
+    int streak = 0;
+    bool ok = false;
+    refine_steps = 0;
+    for (int it = 0; it < 10; ++it) {
+      gemv_n<double>(K64.p, npad, (int)npad, (int)npad, alpha.p, p1.p, gpart.p, 1, 0, 0, s);
+      gemv_t<double>(K64.p, npad, (int)npad, (int)npad, alpha.p, p2.p, 1, 1, 0, 0, s);
+      residual_norms(y.p, p1.p, p2.p, alpha.p, it == 0 ? q1.p : nullptr, q2.p, r.p, n, scal.p + 4, s);
+      readback(8, false);
+      const double rinf = hscal.p[4], winf = hscal.p[5], yinf = hscal.p[6], ainf = hscal.p[7];
+      const double denom = ainf * winf + yinf;
+      if (denom > 0.0 && rinf / denom < tol) {
+        ok = true;
+        break;
+      }
+      if (prev >= 0.0) {
+        const double ratio = prev > 0.0 ? rinf / prev : 0.0;
+        if (ratio > 0.9) {
+          if (++streak >= 2) break;
+        } else {
+          streak = 0;
+        }
+      }
+      prev = rinf;
+      gemv_n<float>(X32.p, npad, (int)npad, (int)npad, r.p, tv.p, gpart.p, 1, 0, 0, s);
+      gemv_t<float>(X32.p, npad, (int)npad, (int)npad, tv.p, dlt.p, 1, 0, 0, 0, s);
+      axpy(alpha.p, dlt.p, n, s);
+      ++refine_steps;
+    }
+    if (!ok) return -1;
+    dot_and_sum(y.p, alpha.p, logdiag.p, n, scal.p, s);
+    GPRX_HIP_CHECK(hipEventRecord(ev[3], s));
+    if (want_grad) {
+      gemm_call<float>(true, false, X32.p, npad, X32.p, npad, A32.p, npad, (int)npad, (int)npad,
+                       (int)npad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
+      GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
+      contract_exact<float>(dp, prog.p, pts(), A32.p, npad, alpha.p, partials.p, s);
+      reduce_partials(partials.p, lower_tiles(npad), accs.p, s);
+      noise_trace<float>(alpha.p, A32.p, npad, n, scal.p + 2, s);
+    } else {
+      GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
+    }
+    GPRX_HIP_CHECK(hipEventRecord(ev[5], s));
+    cur_form = GPRX_MIXED;
+    return readback(3, want_grad);
+  }
+
+  // One attempt at jitter j; fills nlml/grad on success.  Returns info bits.
+  int attempt(const double* th, double j, bool want_grad, double* nlml, double* grad) {
+    upload_program(th, j);
+    GPRX_HIP_CHECK(hipEventRecord(ev[0], stream));
+    int bits;
+    fallback = 0;
+    if (precision == GPRX_MIXED) {
+      bits = run_mixed(want_grad);
+      if (bits == -1) {
+        fallback = 1;
+        bits = run_fp64(want_grad);
+      }
+    } else {
+      bits = run_fp64(want_grad);
+    }
+    if (bits == 0) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[0], ev[5]);
+      last_eval_ms = ms;
+      for (int k = 0; k < 5; ++k) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+        phase_ms[k] = t;
+      }
+      last_dense_ms = phase_ms[1];
+    }
+    if (bits) return bits;
+    const double* sc = hscal.p;
+    *nlml = 0.5 * sc[0] + sc[1] + 0.5 * (double)n * kLog2Pi;
+    if (!std::isfinite(*nlml)) return 2;
+    if (want_grad && grad) {
+      acc_to_grad(hp, th, hacc.p, grad);
+      for (int p = 0; p < hp.n_params; ++p) grad[p] *= -0.5;
+      grad[hp.n_params] = -0.5 * std::exp(th[hp.n_params]) * sc[2];
+    }
+    return 0;
+  }
+
+  // The jitter loop of src/linalg/chol.rs:20-43 around one evaluation.
+  int evaluate(const double* th, bool want_grad, double* nlml, double* grad, gprx_error* err) {
+    double j = 0.0, next = jit.initial;
+    int remaining = jit.max_retries;
+    for (;;) {
+      const int bits = attempt(th, j, want_grad, nlml, grad);
+      if (bits == 0) {
+        last_jitter = j;
+        factored = true;
+        return GPRX_OK;
+      }
+      factored = false;
+      if (bits & 2) {
+        set_error(err, GPRX_NON_FINITE_KERNEL_VALUE, "kernel matrix holds a non-finite value");
+        return GPRX_NON_FINITE_KERNEL_VALUE;
+      }
+      if (remaining == 0 || !(next > 0.0) || next > jit.max_jitter) {
+        set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky factor failed", GPRX_STAGE_FIT, j, (size_t)n);
+        return GPRX_CHOLESKY_FAILED;
+      }
+      --remaining;
+      j = next;
+      next *= jit.multiplier;
+      if (!std::isfinite(next)) remaining = 0;
+    }
+  }
+};
+
+int validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err) {
+  if (p != (size_t)hp.n_params + 1) {
+    set_error(err, GPRX_LENGTH_MISMATCH, "parameter count differs from the program's");
+    return GPRX_LENGTH_MISMATCH;
+  }
+  for (size_t i = 0; i < p; ++i)
+    if (!std::isfinite(th[i])) {
+      set_error(err, i + 1 == p ? GPRX_INVALID_NOISE_VARIANCE : GPRX_INVALID_HYPERPARAMETER,
+                "non-finite log parameter");
+      return i + 1 == p ? GPRX_INVALID_NOISE_VARIANCE : GPRX_INVALID_HYPERPARAMETER;
+    }
+  const double noise = std::exp(th[p - 1]);
+  if (!(noise > 0.0) || !std::isfinite(noise)) {
+    set_error(err, GPRX_INVALID_NOISE_VARIANCE, "noise variance must be positive and finite");
+    return GPRX_INVALID_NOISE_VARIANCE;
+  }
+  return GPRX_OK;
+}
+
+template <typename F>
+int guarded(gprx_error* err, F&& f) {
+  try {
+    return f();
+  } catch (const DeviceError& e) {
+    set_error(err, GPRX_DEVICE_ERROR, e.msg);
+    return GPRX_DEVICE_ERROR;
+  } catch (const std::bad_alloc&) {
+    set_error(err, GPRX_DEVICE_ERROR, "host allocation failed");
+    return GPRX_DEVICE_ERROR;
+  } catch (...) {
+    set_error(err, GPRX_DEVICE_ERROR, "unexpected failure");
+    return GPRX_DEVICE_ERROR;
+  }
+}
+
+// Uploads column-major points (n x d) padded to (rows_pad x dp).
+void upload_points(DevBuf<double>& dst, const double* src, int n, int d, long rows_pad, int dp,
+                   hipStream_t s) {
+  std::vector<double> h((size_t)rows_pad * dp, 0.0);
+  for (int k = 0; k < d; ++k)
+    for (int i = 0; i < n; ++i) h[(size_t)k * rows_pad + i] = src[(size_t)k * n + i];
+  dst.alloc(h.size());
+  GPRX_HIP_CHECK(hipMemcpyAsync(dst.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+namespace gprx {
+int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err) {
+  return validate_theta(hp, th, p, err);
+}
+void exact_upload_points(DevBuf<double>& dst, const double* src, int n, int d, long rows_pad, int dp,
+                         hipStream_t s) {
+  upload_points(dst, src, n, d, rows_pad, dp, s);
+}
+}  // namespace gprx
+
+extern "C" {
+
+const char* gprx_hip_version(void) { return "gprx-hip 0.1.0 (gfx950)"; }
+
+int gprx_hip_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
+                          const gprx_kernel_program* kernel, const double* log_theta, size_t p,
+                          int32_t precision, const gprx_jitter_policy* jitter,
+                          gprx_hip_model** out, gprx_error* err) {
+  if (out) *out = nullptr;
+  return guarded(err, [&]() -> int {
+    if (!x || !y || !out || n == 0 || d == 0) {
+      set_error(err, GPRX_EMPTY_INPUT, "empty training data");
+      return GPRX_EMPTY_INPUT;
+    }
+    if (d > (size_t)kMaxDim) {
+      set_error(err, GPRX_UNSUPPORTED, "input dimension above 32");
+      return GPRX_UNSUPPORTED;
+    }
+    if (n > (size_t)1 << 20) {
+      set_error(err, GPRX_UNSUPPORTED, "exact GPR above 2^20 points");
+      return GPRX_UNSUPPORTED;
+    }
+    if (!all_finite(x, n * d) || !all_finite(y, n)) {
+      set_error(err, GPRX_NON_FINITE_INPUT, "non-finite training data");
+      return GPRX_NON_FINITE_INPUT;
+    }
+    if (precision != GPRX_FP64 && precision != GPRX_MIXED) {
+      set_error(err, GPRX_UNSUPPORTED, "unknown precision");
+      return GPRX_UNSUPPORTED;
+    }
+    auto m = std::make_unique<ExactModel>();
+    std::string why;
+    int rc = parse_program(kernel, (int)d, &m->hp, &why);
+    if (rc != GPRX_OK) {
+      set_error(err, rc, why);
+      return rc;
+    }
+    if (!log_theta) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "missing parameters");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    rc = validate_theta(m->hp, log_theta, p, err);
+    if (rc != GPRX_OK) return rc;
+    if (jitter) {
+      if (!(jitter->initial >= 0.0) || !(jitter->multiplier >= 1.0) || jitter->max_retries < 0) {
+        set_error(err, GPRX_INVALID_HYPERPARAMETER, "invalid jitter policy");
+        return GPRX_INVALID_HYPERPARAMETER;
+      }
+      m->jit = *jitter;
+    }
+    m->n = (int)n;
+    m->d = (int)d;
+    m->dp = padded_dim((int)d);
+    m->npad = (long)round_up(n, kTile);
+    m->nt = (int)(m->npad / kTile);
+    m->ldt = (long)((m->nt + 1) / 2) * kTile;
+    m->precision = precision;
+    m->theta.assign(log_theta, log_theta + p);
+    GPRX_HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    for (auto& e : m->ev) GPRX_HIP_CHECK(hipEventCreate(&e));
+    hipStream_t s = m->stream;
+    upload_points(m->x, x, (int)n, (int)d, m->npad, m->dp, s);
+    {
+      std::vector<double> yh(m->npad, 0.0), oh(m->npad, 1.0);
+      std::memcpy(yh.data(), y, sizeof(double) * n);
+      m->y.alloc(m->npad);
+      m->ones.alloc(m->npad);
+      GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
+      GPRX_HIP_CHECK(hipMemcpyAsync(m->ones.p, oh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
+      GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    m->prog.alloc(1);
+    m->hprog.alloc(1);
+    const long np = m->npad;
+    m->logdiag.alloc(np);
+    m->alpha.alloc(np);
+    m->tv.alloc(np);
+    m->r.alloc(np);
+    m->dlt.alloc(np);
+    m->p1.alloc(np);
+    m->p2.alloc(np);
+    m->q1.alloc(np);
+    m->q2.alloc(np);
+    m->gpart.alloc((size_t)((np + 255) / 256) * np);
+    m->partials.alloc((size_t)lower_tiles(np) * kMaxAcc);
+    m->accs.alloc(kMaxAcc);
+    m->scal.alloc(16);
+    m->info.alloc(1);
+    m->hscal.alloc(16);
+    m->hacc.alloc(kMaxAcc);
+    m->hinfo.alloc(1);
+    GPRX_HIP_CHECK(hipMemsetAsync(m->r.p, 0, sizeof(double) * np, s));
+    GPRX_HIP_CHECK(hipMemsetAsync(m->alpha.p, 0, sizeof(double) * np, s));
+    GPRX_HIP_CHECK(hipMemsetAsync(m->logdiag.p, 0, sizeof(double) * np, s));
+    GPRX_HIP_CHECK(hipMemsetAsync(m->scal.p, 0, sizeof(double) * 16, s));
+    if (precision == GPRX_MIXED)
+      m->alloc_mixed();
+    else
+      m->alloc_fp64();
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    *out = m.release();
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_exact_value_and_grad(gprx_hip_model* model, const double* log_theta, size_t p,
+                                  double* nlml, double* grad, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !log_theta || !nlml) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    int rc = validate_theta(m->hp, log_theta, p, err);
+    if (rc != GPRX_OK) return rc;
+    std::vector<double> g(p, 0.0);
+    double v = 0.0;
+    rc = m->evaluate(log_theta, grad != nullptr, &v, grad ? g.data() : nullptr, err);
+    if (rc != GPRX_OK) return rc;  // parameters unchanged; the factor is rebuilt on demand
+    m->theta.assign(log_theta, log_theta + p);
+    *nlml = v;
+    if (grad) std::memcpy(grad, g.data(), sizeof(double) * p);
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                           int32_t variance_kind, double* mean, double* var, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !mean) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    if (d != (size_t)m->d) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "query dimension differs from training");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    if (q == 0) return GPRX_OK;
+    if (!xs || !all_finite(xs, q * d)) {
+      set_error(err, GPRX_NON_FINITE_INPUT, "non-finite query");
+      return GPRX_NON_FINITE_INPUT;
+    }
+    if (!m->factored) {
+      double v;
+      int rc = m->evaluate(m->theta.data(), false, &v, nullptr, err);
+      if (rc != GPRX_OK) {
+        if (err) err->stage = GPRX_STAGE_PREDICT;
+        return rc;
+      }
+    } else {
+      m->upload_program(m->theta.data(), m->last_jitter);
+    }
+    hipStream_t s = m->stream;
+    const long qpad = (long)round_up(q, kTile);
+    DevBuf<double> xq, Ks, kss, ss, mu;
+    upload_points(xq, xs, (int)q, (int)d, qpad, m->dp, s);
+    DevPoints pq{xq.p, qpad, (int)q};
+    Ks.alloc((size_t)m->npad * qpad);
+    kss.alloc(qpad);
+    ss.alloc(qpad);
+    mu.alloc(qpad);
+    fill_cross<double>(m->dp, m->prog.p, m->pts(), pq, Ks.p, m->npad, s);
+    gemv_t<double>(Ks.p, m->npad, (int)m->npad, (int)qpad, m->alpha.p, mu.p, 0, 0, 0, 0, s);
+    const bool want_var = var != nullptr;
+    if (want_var) {
+      fill_diag(m->dp, m->prog.p, pq, kss.p, s);
+      if (m->cur_form == GPRX_FP64) {
+        DevBuf<double> V;
+        V.alloc((size_t)m->npad * qpad);
+        gemm_call<double>(false, false, m->X64.p, m->npad, Ks.p, m->npad, V.p, m->npad,
+                          (int)m->npad, (int)qpad, (int)m->npad, 1.0, 0.0, kKhiRow, 0,
+                          kOrderColFastRev, s);
+        col_sumsq<double>(V.p, m->npad, (int)m->npad, (int)qpad, ss.p, s);
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+      } else {
+        DevBuf<float> K32, V;
+        K32.alloc((size_t)m->npad * qpad);
+        V.alloc((size_t)m->npad * qpad);
+        fill_cross<float>(m->dp, m->prog.p, m->pts(), pq, K32.p, m->npad, s);
+        gemm_call<float>(false, false, m->X32.p, m->npad, K32.p, m->npad, V.p, m->npad,
+                         (int)m->npad, (int)qpad, (int)m->npad, 1.0, 0.0, kKhiRow, 0,
+                         kOrderColFastRev, s);
+        col_sumsq<float>(V.p, m->npad, (int)m->npad, (int)qpad, ss.p, s);
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+      }
+    }
+    GPRX_HIP_CHECK(hipMemcpyAsync(mean, mu.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    std::vector<double> hk(q), hs(q);
+    if (want_var) {
+      GPRX_HIP_CHECK(hipMemcpyAsync(hk.data(), kss.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+      GPRX_HIP_CHECK(hipMemcpyAsync(hs.data(), ss.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    }
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    if (want_var) {
+      const double noise = std::exp(m->theta[m->hp.n_params]);
+      for (size_t j = 0; j < q; ++j) {
+        double lat = hk[j] - hs[j];
+        if (lat < 0.0) lat = 0.0;  // src/gpr/shared.rs:798-806
+        var[j] = variance_kind == GPRX_VARIANCE_LATENT ? lat : lat + noise;
+      }
+    }
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_get_params(const gprx_hip_model* model, double* log_theta, size_t p) {
+  if (!model || !log_theta) return GPRX_LENGTH_MISMATCH;
+  if (model->kind != gprx_hip_model::kExact) return GPRX_UNSUPPORTED;
+  auto* m = static_cast<const ExactModel*>(model);
+  if (p != m->theta.size()) return GPRX_LENGTH_MISMATCH;
+  std::memcpy(log_theta, m->theta.data(), sizeof(double) * p);
+  return GPRX_OK;
+}
+
+double gprx_hip_last_jitter(const gprx_hip_model* model) {
+  if (!model || model->kind != gprx_hip_model::kExact) return 0.0;
+  return static_cast<const ExactModel*>(model)->last_jitter;
+}
+
+int gprx_hip_last_fallback(const gprx_hip_model* model) {
+  if (!model || model->kind != gprx_hip_model::kExact) return 0;
+  return static_cast<const ExactModel*>(model)->fallback;
+}
+
+int gprx_hip_last_refine_steps(const gprx_hip_model* model) {
+  if (!model || model->kind != gprx_hip_model::kExact) return 0;
+  return static_cast<const ExactModel*>(model)->refine_steps;
+}
+
+void gprx_hip_destroy(gprx_hip_model* model) { delete model; }
+
+double gprx_hip_last_eval_ms(const gprx_hip_model* model) { return model ? model->last_eval_ms : 0.0; }
+double gprx_hip_last_dense_ms(const gprx_hip_model* model) {
+  return model ? model->last_dense_ms : 0.0;
+}
+
+double gprx_hip_last_phase_ms(const gprx_hip_model* model, int32_t phase) {
+  if (!model || model->kind != gprx_hip_model::kExact || phase < 0 || phase > 4) return 0.0;
+  return static_cast<const ExactModel*>(model)->phase_ms[phase];
+}
+
+}  // extern "C"

@@ -1,0 +1,662 @@
+// Kernel programs: host parsing into sum-of-products form, and the HIP
+// kernels that evaluate them per matrix entry (fills) or contract a weight
+// matrix against their parameter derivatives.
+//
+// Leaf formulas (the reference's, in log parameters):
+//   Constant  k = c                           dk/dlog c = c        (src/kernel/constant.rs:12-16)
+//   RBF       k = exp(-s / (2 l^2))           dk/dlog l = k s / l^2 (src/kernel/rbf.rs:18-20,254-256)
+//   ARD-RBF   k = exp(-1/2 sum dx_d^2 / l_d^2) dk/dlog l_d = k dx_d^2 / l_d^2 (src/kernel/rbf_ard.rs:16,274-276)
+//   Periodic  k = exp(-2 sin^2(pi r / p) / l^2), r = |x - x'|
+//             dk/dlog l = k 4 sin^2 / l^2, dk/dlog p = k 4 sin cos a a / l^2, a = pi r / p
+//             (src/kernel/periodic.rs:17-23,599-637)
+// with s the squared Euclidean distance by direct differences
+// (src/kernel/simd/dist.rs:7-19).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "fill.hpp"
+
+namespace gprx {
+
+// ---------------------------------------------------------------------------
+// Host: program parsing.
+// ---------------------------------------------------------------------------
+namespace {
+struct Sop {
+  std::vector<uint32_t> terms;  // bitmasks over leaves
+};
+}  // namespace
+
+int parse_program(const gprx_kernel_program* prog, int d, HostProgram* out, std::string* why) {
+  if (!prog || !prog->ops || prog->n_ops == 0) {
+    *why = "empty kernel program";
+    return GPRX_INVALID_HYPERPARAMETER;
+  }
+  HostProgram hp;
+  hp.d = d;
+  std::vector<Sop> stack;
+  int poff = 0;
+  for (size_t k = 0; k < prog->n_ops; ++k) {
+    const int op = prog->ops[2 * k], arg = prog->ops[2 * k + 1];
+    if (op >= GPRX_OP_CONST && op <= GPRX_OP_PERIODIC) {
+      if (hp.nl >= kMaxLeaves) {
+        *why = "kernel program has more than 8 leaves";
+        return GPRX_UNSUPPORTED;
+      }
+      const int l = hp.nl++;
+      hp.param_off[l] = poff;
+      switch (op) {
+        case GPRX_OP_CONST:
+          hp.kind[l] = kLeafConst;
+          poff += 1;
+          break;
+        case GPRX_OP_RBF:
+          hp.kind[l] = kLeafRbf;
+          poff += 1;
+          break;
+        case GPRX_OP_RBF_ARD:
+          if (arg != d) {
+            *why = "RBF-ARD lengthscale count differs from the input dimension";
+            return GPRX_LENGTH_MISMATCH;
+          }
+          if (hp.ard_leaf >= 0) {
+            *why = "more than one RBF-ARD leaf in a program";
+            return GPRX_UNSUPPORTED;
+          }
+          hp.kind[l] = kLeafArd;
+          hp.ard_leaf = l;
+          poff += d;
+          break;
+        default:
+          hp.kind[l] = kLeafPeriodic;
+          poff += 2;
+          break;
+      }
+      stack.push_back(Sop{{1u << l}});
+    } else if (op == GPRX_OP_SUM || op == GPRX_OP_PROD) {
+      if (arg < 1 || (size_t)arg > stack.size()) {
+        *why = "malformed kernel program (operator arity)";
+        return GPRX_INVALID_HYPERPARAMETER;
+      }
+      std::vector<Sop> kids(stack.end() - arg, stack.end());
+      stack.resize(stack.size() - arg);
+      Sop res;
+      if (op == GPRX_OP_SUM) {
+        for (auto& kd : kids) res.terms.insert(res.terms.end(), kd.terms.begin(), kd.terms.end());
+      } else {
+        res.terms = {0u};
+        for (auto& kd : kids) {
+          std::vector<uint32_t> nx;
+          for (uint32_t a : res.terms)
+            for (uint32_t b : kd.terms) nx.push_back(a | b);
+          res.terms.swap(nx);
+          if (res.terms.size() > (size_t)kMaxTerms) break;
+        }
+      }
+      if (res.terms.size() > (size_t)kMaxTerms) {
+        *why = "kernel program expands to more than 8 product terms";
+        return GPRX_UNSUPPORTED;
+      }
+      stack.push_back(std::move(res));
+    } else {
+      *why = "unknown kernel program op";
+      return GPRX_INVALID_HYPERPARAMETER;
+    }
+  }
+  if (stack.size() != 1) {
+    *why = "malformed kernel program (stack)";
+    return GPRX_INVALID_HYPERPARAMETER;
+  }
+  hp.terms = stack[0].terms;
+  hp.nt = (int)hp.terms.size();
+  hp.n_params = poff;
+  *out = hp;
+  return GPRX_OK;
+}
+
+void make_dev_program(const HostProgram& hp, const double* th, double diag_noise, DevProgram* P) {
+  std::memset(P, 0, sizeof(*P));
+  P->nl = hp.nl;
+  P->nt = hp.nt;
+  P->d = hp.d;
+  P->ard_leaf = hp.ard_leaf;
+  for (int t = 0; t < hp.nt; ++t) P->mask[t] = hp.terms[t];
+  for (int l = 0; l < hp.nl; ++l) {
+    P->kind[l] = hp.kind[l];
+    const double* q = th + hp.param_off[l];
+    switch (hp.kind[l]) {
+      case kLeafConst:
+        P->a[l] = std::exp(q[0]);
+        break;
+      case kLeafRbf:
+        P->a[l] = std::exp(-2.0 * q[0]);
+        break;
+      case kLeafArd:
+        for (int k = 0; k < hp.d; ++k) P->ard_w[k] = std::exp(-2.0 * q[k]);
+        break;
+      default:
+        P->a[l] = M_PI * std::exp(-q[1]);
+        P->b[l] = std::exp(-2.0 * q[0]);
+        break;
+    }
+  }
+  P->noise = diag_noise;
+}
+
+void acc_to_grad(const HostProgram& hp, const double* th, const double* acc, double* grad) {
+  for (int l = 0; l < hp.nl; ++l) {
+    const int o = hp.param_off[l];
+    const double a0 = acc[kAccPerLeaf * l], a1 = acc[kAccPerLeaf * l + 1];
+    switch (hp.kind[l]) {
+      case kLeafConst:
+        grad[o] = a0;
+        break;
+      case kLeafRbf:
+        grad[o] = a0 * std::exp(-2.0 * th[o]);
+        break;
+      case kLeafArd:
+        for (int k = 0; k < hp.d; ++k)
+          grad[o + k] = acc[kMaxLeaves * kAccPerLeaf + k] * std::exp(-2.0 * th[o + k]);
+        break;
+      default: {
+        const double b4 = 4.0 * std::exp(-2.0 * th[o]);
+        grad[o] = a0 * b4;
+        grad[o + 1] = a1 * b4;
+        break;
+      }
+    }
+  }
+}
+
+void set_error(gprx_error* err, int status, const std::string& msg, int stage, double jitter,
+               size_t matrix_size) {
+  if (!err) return;
+  err->status = status;
+  err->stage = stage;
+  err->jitter = jitter;
+  err->matrix_size = matrix_size;
+  std::snprintf(err->message, sizeof(err->message), "%s", msg.c_str());
+}
+
+// ---------------------------------------------------------------------------
+// Device: one entry of the program.
+// ---------------------------------------------------------------------------
+struct Leaves {
+  double v[kMaxLeaves];
+  double f0[kMaxLeaves];
+  double f1[kMaxLeaves];
+};
+
+template <int DP, bool GRAD>
+__device__ __forceinline__ double eval_entry(const DevProgram& P, const double (&dx)[DP], Leaves& L) {
+  double s = 0.0, r2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) {
+    const double t = dx[q] * dx[q];
+    s += t;
+    r2 += t * P.ard_w[q];
+  }
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) {
+    L.v[l] = 1.0;
+    if (GRAD) {
+      L.f0[l] = 0.0;
+      L.f1[l] = 0.0;
+    }
+    if (l < P.nl) {
+      const int kd = P.kind[l];
+      if (kd == kLeafConst) {
+        L.v[l] = P.a[l];
+        if (GRAD) L.f0[l] = 1.0;
+      } else if (kd == kLeafRbf) {
+        L.v[l] = exp(-0.5 * s * P.a[l]);
+        if (GRAD) L.f0[l] = s;
+      } else if (kd == kLeafArd) {
+        L.v[l] = exp(-0.5 * r2);
+      } else {
+        const double r = sqrt(s);
+        const double al = r * P.a[l];
+        double sn, cs;
+        sincos(al, &sn, &cs);
+        L.v[l] = exp(-2.0 * sn * sn * P.b[l]);
+        if (GRAD) {
+          L.f0[l] = sn * sn;
+          L.f1[l] = sn * cs * al;
+        }
+      }
+    }
+  }
+  double k = 0.0;
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t < P.nt) {
+      double pr = 1.0;
+#pragma unroll
+      for (int l = 0; l < kMaxLeaves; ++l)
+        if ((P.mask[t] >> l) & 1u) pr *= L.v[l];
+      k += pr;
+    }
+  }
+  return k;
+}
+
+// Adds w * dK/d(leaf features) of one entry to the accumulators.
+template <int DP>
+__device__ __forceinline__ void accumulate_entry(const DevProgram& P, const double (&dx)[DP],
+                                                 const Leaves& L, double w,
+                                                 double (&acc)[kMaxLeaves][kAccPerLeaf],
+                                                 double (&accd)[DP]) {
+  double gard = 0.0;
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) {
+    if (l < P.nl) {
+      double cl = 0.0;
+#pragma unroll
+      for (int t = 0; t < kMaxTerms; ++t) {
+        if (t < P.nt && ((P.mask[t] >> l) & 1u)) {
+          double pr = 1.0;
+#pragma unroll
+          for (int l2 = 0; l2 < kMaxLeaves; ++l2)
+            if (l2 != l && ((P.mask[t] >> l2) & 1u)) pr *= L.v[l2];
+          cl += pr;
+        }
+      }
+      const double g = w * cl * L.v[l];
+      acc[l][0] += g * L.f0[l];
+      acc[l][1] += g * L.f1[l];
+      if (l == P.ard_leaf) gard = g;
+    }
+  }
+  if (P.ard_leaf >= 0) {
+#pragma unroll
+    for (int q = 0; q < DP; ++q) accd[q] += gard * dx[q] * dx[q];
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block reduction of the accumulators into partial[tile * kMaxAcc + k].
+template <int DP>
+__device__ __forceinline__ void store_partials(double (&acc)[kMaxLeaves][kAccPerLeaf],
+                                               double (&accd)[DP], double* red /* [4][kMaxAcc] */,
+                                               double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l)
+#pragma unroll
+    for (int c = 0; c < kAccPerLeaf; ++c) {
+      const double v = wave_sum(acc[l][c]);
+      if (lane == 0) red[wave * kMaxAcc + l * kAccPerLeaf + c] = v;
+    }
+#pragma unroll
+  for (int q = 0; q < kMaxDim; ++q) {
+    double v = 0.0;
+    if (q < DP) v = wave_sum(accd[q < DP ? q : 0]);
+    if (lane == 0) red[wave * kMaxAcc + kMaxLeaves * kAccPerLeaf + q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxAcc) {
+    const int k = threadIdx.x;
+    out[k] = ((red[k] + red[kMaxAcc + k]) + red[2 * kMaxAcc + k]) + red[3 * kMaxAcc + k];
+  }
+}
+
+__device__ __forceinline__ void tri_index_f(int bid, int& ti, int& tj) {
+  int t = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+  while ((t + 1) * (t + 2) / 2 <= bid) ++t;
+  while (t * (t + 1) / 2 > bid) --t;
+  ti = t;
+  tj = bid - t * (t + 1) / 2;
+}
+
+// Loads the DP coordinates of 128 points starting at row r0 into LDS.
+template <int DP>
+__device__ __forceinline__ void load_points(const double* __restrict__ p, long ld, int r0,
+                                            double (*dst)[kTile]) {
+  for (int e = threadIdx.x; e < DP * kTile; e += 256) {
+    const int q = e / kTile, i = e % kTile;
+    dst[q][i] = p[(long)q * ld + r0 + i];
+  }
+}
+
+template <int DP, typename T, bool DUAL>
+__global__ __launch_bounds__(256) void k_fill_lower(const DevProgram* __restrict__ Pg,
+                                                    const double* __restrict__ x, long ldx, int n,
+                                                    T* __restrict__ K, long ldk,
+                                                    double* __restrict__ K64, int* info) {
+  const DevProgram& P = *Pg;
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  int ti, tj;
+  tri_index_f(blockIdx.x, ti, tj);
+  const int r0 = ti * kTile, c0 = tj * kTile;
+  load_points<DP>(x, ldx, r0, xr);
+  load_points<DP>(x, ldx, c0, xc);
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  bool bad = false;
+  for (int jj = 0; jj < 64; ++jj) {
+    const int j = jh + 2 * jj;
+    const int gj = c0 + j;
+    if (gi < gj) continue;
+    double val;
+    if (gi >= n || gj >= n) {
+      val = gi == gj ? 1.0 : 0.0;
+    } else {
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      val = eval_entry<DP, false>(P, dx, L);
+      if (!isfinite(val)) bad = true;
+      if (gi == gj) val += P.noise;
+    }
+    K[gi + (long)gj * ldk] = (T)val;
+    if (DUAL) K64[gi + (long)gj * ldk] = val;
+  }
+  if (bad) atomicOr(info, 2);
+}
+
+template <int DP, typename T>
+__global__ __launch_bounds__(256) void k_fill_cross(const DevProgram* __restrict__ Pg,
+                                                    const double* __restrict__ a, long lda, int na,
+                                                    const double* __restrict__ b, long ldb, int nb,
+                                                    T* __restrict__ out, long ldo, int mt) {
+  const DevProgram& P = *Pg;
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  const int ti = blockIdx.x % mt, tj = blockIdx.x / mt;
+  const int r0 = ti * kTile, c0 = tj * kTile;
+  load_points<DP>(a, lda, r0, xr);
+  load_points<DP>(b, ldb, c0, xc);
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  for (int jj = 0; jj < 64; ++jj) {
+    const int j = jh + 2 * jj;
+    const int gj = c0 + j;
+    double val = 0.0;
+    if (gi < na && gj < nb) {
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      val = eval_entry<DP, false>(P, dx, L);
+    }
+    out[gi + (long)gj * ldo] = (T)val;
+  }
+}
+
+template <int DP>
+__global__ void k_fill_diag(const DevProgram* __restrict__ Pg, int na, double* __restrict__ out) {
+  const DevProgram& P = *Pg;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= na) return;
+  double dx[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) dx[q] = 0.0;
+  Leaves L;
+  out[i] = eval_entry<DP, false>(P, dx, L);
+}
+
+template <int DP, typename T>
+__global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __restrict__ Pg,
+                                                        const double* __restrict__ x, long ldx,
+                                                        int n, const T* __restrict__ Ainv, long lda,
+                                                        const double* __restrict__ alpha,
+                                                        double* __restrict__ partial) {
+  const DevProgram& P = *Pg;
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  __shared__ double ar[kTile], ac[kTile];
+  __shared__ double red[4 * kMaxAcc];
+  int ti, tj;
+  tri_index_f(blockIdx.x, ti, tj);
+  const int r0 = ti * kTile, c0 = tj * kTile;
+  load_points<DP>(x, ldx, r0, xr);
+  load_points<DP>(x, ldx, c0, xc);
+  if (threadIdx.x < kTile) {
+    ar[threadIdx.x] = alpha[r0 + threadIdx.x];
+    ac[threadIdx.x] = alpha[c0 + threadIdx.x];
+  }
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  const double ai = ar[i];
+  double acc[kMaxLeaves][kAccPerLeaf];
+  double accd[DP];
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) accd[q] = 0.0;
+  if (gi < n) {
+    for (int jj = 0; jj < 64; ++jj) {
+      const int j = jh + 2 * jj;
+      const int gj = c0 + j;
+      if (gj > gi || gj >= n) continue;
+      double w = ai * ac[j] - (double)Ainv[gi + (long)gj * lda];
+      if (gi != gj) w *= 2.0;
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      eval_entry<DP, true>(P, dx, L);
+      accumulate_entry<DP>(P, dx, L, w, acc, accd);
+    }
+  }
+  store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);
+}
+
+template <int DP, typename T>
+__global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __restrict__ Pg,
+                                                        const double* __restrict__ a, long lda,
+                                                        int na, const double* __restrict__ b,
+                                                        long ldb, int nb, const T* __restrict__ W,
+                                                        long ldw, int mt,
+                                                        double* __restrict__ partial) {
+  const DevProgram& P = *Pg;
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  __shared__ double red[4 * kMaxAcc];
+  const int ti = blockIdx.x % mt, tj = blockIdx.x / mt;
+  const int r0 = ti * kTile, c0 = tj * kTile;
+  load_points<DP>(a, lda, r0, xr);
+  load_points<DP>(b, ldb, c0, xc);
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  double acc[kMaxLeaves][kAccPerLeaf];
+  double accd[DP];
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) accd[q] = 0.0;
+  if (gi < na) {
+    for (int jj = 0; jj < 64; ++jj) {
+      const int j = jh + 2 * jj;
+      const int gj = c0 + j;
+      if (gj >= nb) continue;
+      const double w = (double)W[gi + (long)gj * ldw];
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      eval_entry<DP, true>(P, dx, L);
+      accumulate_entry<DP>(P, dx, L, w, acc, accd);
+    }
+  }
+  store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);
+}
+
+template <int DP>
+__global__ __launch_bounds__(256) void k_contract_diag(const DevProgram* __restrict__ Pg, int na,
+                                                       double w, double* __restrict__ partial) {
+  const DevProgram& P = *Pg;
+  __shared__ double red[4 * kMaxAcc];
+  double acc[kMaxLeaves][kAccPerLeaf];
+  double accd[DP];
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) accd[q] = 0.0;
+  // k(x, x) does not depend on x for these stationary leaves: one entry
+  // times the count.
+  if (threadIdx.x == 0) {
+    double dx[DP];
+#pragma unroll
+    for (int q = 0; q < DP; ++q) dx[q] = 0.0;
+    Leaves L;
+    eval_entry<DP, true>(P, dx, L);
+    accumulate_entry<DP>(P, dx, L, w * (double)na, acc, accd);
+  }
+  store_partials<DP>(acc, accd, red, partial);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial,
+                                                         int tiles, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += 256) acc += partial[(long)t * kMaxAcc + k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers: dispatch on the padded dimension.
+// ---------------------------------------------------------------------------
+#define GPRX_DISPATCH_DIM(dp, CALL) \
+  switch (dp) {                     \
+    case 1: {                       \
+      constexpr int DP = 1;         \
+      CALL;                         \
+    } break;                        \
+    case 2: {                       \
+      constexpr int DP = 2;         \
+      CALL;                         \
+    } break;                        \
+    case 4: {                       \
+      constexpr int DP = 4;         \
+      CALL;                         \
+    } break;                        \
+    case 8: {                       \
+      constexpr int DP = 8;         \
+      CALL;                         \
+    } break;                        \
+    case 16: {                      \
+      constexpr int DP = 16;        \
+      CALL;                         \
+    } break;                        \
+    case 32: {                      \
+      constexpr int DP = 32;        \
+      CALL;                         \
+    } break;                        \
+    default:                        \
+      throw DeviceError("unsupported padded dimension"); \
+  }
+
+template <typename T>
+void fill_lower(int dp, const DevProgram* prog, DevPoints x, T* K, long ldk, double* K64, int* info,
+                hipStream_t stream) {
+  const int tiles = lower_tiles(x.ld);
+  if (K64) {
+    GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_fill_lower<DP, T, true>), dim3(tiles), dim3(256),
+                                             0, stream, prog, x.p, x.ld, x.rows, K, ldk, K64, info));
+  } else {
+    GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_fill_lower<DP, T, false>), dim3(tiles), dim3(256),
+                                             0, stream, prog, x.p, x.ld, x.rows, K, ldk, K64, info));
+  }
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void fill_cross(int dp, const DevProgram* prog, DevPoints a, DevPoints b, T* out, long ldo,
+                hipStream_t stream) {
+  const int mt = (int)(a.ld / kTile);
+  const int tiles = cross_tiles(a.ld, b.ld);
+  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_fill_cross<DP, T>), dim3(tiles), dim3(256), 0,
+                                           stream, prog, a.p, a.ld, a.rows, b.p, b.ld, b.rows, out,
+                                           ldo, mt));
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+void fill_diag(int dp, const DevProgram* prog, DevPoints a, double* out, hipStream_t stream) {
+  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_fill_diag<DP>), dim3((a.rows + 255) / 256),
+                                           dim3(256), 0, stream, prog, a.rows, out));
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void contract_exact(int dp, const DevProgram* prog, DevPoints x, const T* Ainv, long lda,
+                    const double* alpha, double* partial, hipStream_t stream) {
+  const int tiles = lower_tiles(x.ld);
+  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_exact<DP, T>), dim3(tiles), dim3(256), 0,
+                                           stream, prog, x.p, x.ld, x.rows, Ainv, lda, alpha,
+                                           partial));
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void contract_cross(int dp, const DevProgram* prog, DevPoints a, DevPoints b, const T* W, long ldw,
+                    double* partial, hipStream_t stream) {
+  const int mt = (int)(a.ld / kTile);
+  const int tiles = cross_tiles(a.ld, b.ld);
+  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_cross<DP, T>), dim3(tiles), dim3(256), 0,
+                                           stream, prog, a.p, a.ld, a.rows, b.p, b.ld, b.rows, W,
+                                           ldw, mt, partial));
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+void contract_diag(int dp, const DevProgram* prog, DevPoints a, double w, double* partial,
+                   hipStream_t stream) {
+  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_diag<DP>), dim3(1), dim3(256), 0, stream,
+                                           prog, a.rows, w, partial));
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+void reduce_partials(const double* partial, int tiles, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(kMaxAcc), dim3(256), 0, stream, partial, tiles, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template void fill_lower<double>(int, const DevProgram*, DevPoints, double*, long, double*, int*,
+                                 hipStream_t);
+template void fill_lower<float>(int, const DevProgram*, DevPoints, float*, long, double*, int*,
+                                hipStream_t);
+template void fill_cross<double>(int, const DevProgram*, DevPoints, DevPoints, double*, long,
+                                 hipStream_t);
+template void fill_cross<float>(int, const DevProgram*, DevPoints, DevPoints, float*, long,
+                                hipStream_t);
+template void contract_exact<double>(int, const DevProgram*, DevPoints, const double*, long,
+                                     const double*, double*, hipStream_t);
+template void contract_exact<float>(int, const DevProgram*, DevPoints, const float*, long,
+                                    const double*, double*, hipStream_t);
+template void contract_cross<double>(int, const DevProgram*, DevPoints, DevPoints, const double*,
+                                     long, double*, hipStream_t);
+template void contract_cross<float>(int, const DevProgram*, DevPoints, DevPoints, const float*,
+                                    long, double*, hipStream_t);
+
+}  // namespace gprx

@@ -1,0 +1,73 @@
+// Model handles behind the C ABI.
+#pragma once
+
+#include "common.hpp"
+#include "dense.hpp"
+#include "fill.hpp"
+
+struct gprx_hip_model {
+  enum Kind { kExact = 1, kSgpr = 2, kSvgp = 3, kBatch = 4 };
+  explicit gprx_hip_model(Kind k) : kind(k) {}
+  virtual ~gprx_hip_model() = default;
+  Kind kind;
+  double last_eval_ms = 0.0;
+  double last_dense_ms = 0.0;
+};
+
+namespace gprx {
+
+// Small on-device vector helpers shared by the models (vec.hip).
+// out[0] = sum_{i<n} a_i b_i, out[1] = sum_{i<n} c_i  (one block, fixed order)
+void dot_and_sum(const double* a, const double* b, const double* c, int n, double* out,
+                 hipStream_t s);
+// out[0] = sum_{i<n} (alpha_i^2 - M_ii) over the diagonal of M (storage T)
+template <typename T>
+void noise_trace(const double* alpha, const T* M, long ld, int n, double* out, hipStream_t s);
+// r = y - (p1 + p2) for i < n (zero beyond); out[0] = max|r|, out[1] = max|w|,
+// out[2] = max|y|, out[3] = max(rowsum) of the absolute row sums in q1 + q2
+// (q1, q2 may be null: out[3] untouched).
+void residual_norms(const double* y, const double* p1, const double* p2, const double* w,
+                    const double* q1, const double* q2, double* r, int n, double* out,
+                    hipStream_t s);
+// w += d for i < n
+void axpy(double* w, const double* d, int n, hipStream_t s);
+// dst (T) = src (U) elementwise, count entries.
+template <typename T, typename U>
+void convert(T* dst, const U* src, size_t count, hipStream_t s);
+
+// Host helpers.
+bool all_finite(const double* p, size_t n);
+
+// Tile-GEMM call in one line.
+template <typename T>
+inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long ldb, T* C, long ldc,
+                      int M, int N, int K, double alpha, double beta, int krange, int lower,
+                      int order, hipStream_t s) {
+  GemmArgs<T> g{};
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.krange = krange;
+  g.lower = lower;
+  g.order = order;
+  g.batch = 1;
+  gemm<T>(ta, tb, g, s);
+}
+
+// Recursive blocked Cholesky + triangular inverse of the lower part of the
+// nt*128 square A (ld), writing X = L^-1 (lower) into X and the log of every
+// pivot into logdiag.  A's lower triangle is consumed; Tmp holds
+// ceil(nt/2)*128 x floor(nt/2)*128 (ld ldt).
+template <typename T>
+void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdiag, int* info,
+                    hipStream_t s);
+
+}  // namespace gprx

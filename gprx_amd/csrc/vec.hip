@@ -1,0 +1,188 @@
+// Small vector kernels and the recursive factor + inverse driver.
+#include "model.hpp"
+
+namespace gprx {
+
+bool all_finite(const double* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+// One 1024-thread block; fixed-order tree reductions.
+__global__ __launch_bounds__(1024) void k_dot_and_sum(const double* a, const double* b,
+                                                      const double* c, int n, double* out) {
+  __shared__ double r0[1024], r1[1024];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    s0 += a[i] * b[i];
+    s1 += c[i];
+  }
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r0[0];
+    out[1] = r1[0];
+  }
+}
+
+void dot_and_sum(const double* a, const double* b, const double* c, int n, double* out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_and_sum, dim3(1), dim3(1024), 0, s, a, b, c, n, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_noise_trace(const double* alpha, const T* M, long ld,
+                                                      int n, double* out) {
+  __shared__ double r0[1024];
+  double s0 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s0 += alpha[i] * alpha[i] - (double)M[(long)i * (ld + 1)];
+  r0[threadIdx.x] = s0;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) r0[threadIdx.x] += r0[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = r0[0];
+}
+
+template <typename T>
+void noise_trace(const double* alpha, const T* M, long ld, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_noise_trace<T>), dim3(1), dim3(1024), 0, s, alpha, M, ld, n, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+template void noise_trace<double>(const double*, const double*, long, int, double*, hipStream_t);
+template void noise_trace<float>(const double*, const float*, long, int, double*, hipStream_t);
+
+__global__ __launch_bounds__(1024) void k_residual_norms(const double* y, const double* p1,
+                                                         const double* p2, const double* w,
+                                                         const double* q1, const double* q2,
+                                                         double* r, int n, double* out) {
+  __shared__ double m0[1024], m1[1024], m2[1024], m3[1024];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const double ri = y[i] - (p1[i] + p2[i]);
+    r[i] = ri;
+    a0 = fmax(a0, fabs(ri));
+    a1 = fmax(a1, fabs(w[i]));
+    a2 = fmax(a2, fabs(y[i]));
+    if (q1) a3 = fmax(a3, q1[i] + q2[i]);
+  }
+  m0[threadIdx.x] = a0;
+  m1[threadIdx.x] = a1;
+  m2[threadIdx.x] = a2;
+  m3[threadIdx.x] = a3;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
+      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
+      m2[threadIdx.x] = fmax(m2[threadIdx.x], m2[threadIdx.x + s]);
+      m3[threadIdx.x] = fmax(m3[threadIdx.x], m3[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = m0[0];
+    out[1] = m1[0];
+    out[2] = m2[0];
+    if (q1) out[3] = m3[0];
+  }
+}
+
+void residual_norms(const double* y, const double* p1, const double* p2, const double* w,
+                    const double* q1, const double* q2, double* r, int n, double* out,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_norms, dim3(1), dim3(1024), 0, s, y, p1, p2, w, q1, q2, r, n, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_axpy(double* w, const double* d, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) w[i] += d[i];
+}
+
+void axpy(double* w, const double* d, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, w, d, n);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename U>
+__global__ void k_convert(T* dst, const U* src, size_t count) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += stride) dst[i] = (T)src[i];
+}
+
+template <typename T, typename U>
+void convert(T* dst, const U* src, size_t count, hipStream_t s) {
+  if (count == 0) return;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((k_convert<T, U>), dim3((unsigned)blocks), dim3(256), 0, s, dst, src, count);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+template void convert<float, double>(float*, const double*, size_t, hipStream_t);
+template void convert<double, float>(double*, const float*, size_t, hipStream_t);
+template void convert<double, double>(double*, const double*, size_t, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Recursive Cholesky + inverse.  For the block [A11 . ; A21 A22] of s tiles
+// (split s1 = s/2, s2 = s - s1):
+//   (L11, X11) = rec(A11)
+//   L21  = A21 X11^T                 (stored in X's 21 block meanwhile)
+//   A22 -= L21 L21^T                 (lower tiles)
+//   (L22, X22) = rec(A22)
+//   Tmp  = L21 X11
+//   X21  = -X22 Tmp
+// Only X = L^-1 leaves the recursion (the exact path needs A^-1 = X^T X,
+// alpha = X^T X y and predictions through X), so L is never stored.
+// Flops: n^3/3 for the factor plus n^3/3 for the inverse, the large products
+// sitting at the top of the recursion.
+// ---------------------------------------------------------------------------
+template <typename T>
+static void rec(T* A, T* X, T* Tmp, long ld, long ldt, int o, int s, double* logdiag, int* info,
+                hipStream_t st) {
+  auto at = [&](T* base, int r, int c) { return base + (long)r * kTile + (long)c * kTile * ld; };
+  if (s == 1) {
+    potrf_diag<T>(at(A, o, o), ld, at(X, o, o), ld, logdiag + (long)o * kTile, info, 1, 0, 0, 0, st);
+    return;
+  }
+  const int s1 = s / 2, s2 = s - s1;
+  const int o2 = o + s1;
+  rec<T>(A, X, Tmp, ld, ldt, o, s1, logdiag, info, st);
+  // L21 = A21 X11^T (X11 lower, stored N x K: k <= j)
+  gemm_call<T>(false, true, at(A, o2, o), ld, at(X, o, o), ld, at(X, o2, o), ld, s2 * kTile,
+               s1 * kTile, s1 * kTile, 1.0, 0.0, kKhiCol, 0, kOrderRowFastRev, st);
+  // A22 -= L21 L21^T
+  gemm_call<T>(false, true, at(X, o2, o), ld, at(X, o2, o), ld, at(A, o2, o2), ld, s2 * kTile,
+               s2 * kTile, s1 * kTile, -1.0, 1.0, 0, 1, kOrderRowFast, st);
+  rec<T>(A, X, Tmp, ld, ldt, o2, s2, logdiag, info, st);
+  // Tmp = L21 X11 (X11 lower K x N: k >= j)
+  gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tmp, ldt, s2 * kTile, s1 * kTile,
+               s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, st);
+  // X21 = -X22 Tmp (X22 lower: k <= i)
+  gemm_call<T>(false, false, at(X, o2, o2), ld, Tmp, ldt, at(X, o2, o), ld, s2 * kTile, s1 * kTile,
+               s2 * kTile, -1.0, 0.0, kKhiRow, 0, kOrderColFastRev, st);
+}
+
+template <typename T>
+void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdiag, int* info,
+                    hipStream_t s) {
+  rec<T>(A, X, Tmp, ld, ldt, 0, nt, logdiag, info, s);
+}
+
+template void factor_inverse<double>(double*, double*, double*, long, long, int, double*, int*,
+                                     hipStream_t);
+template void factor_inverse<float>(float*, float*, float*, long, long, int, double*, int*,
+                                    hipStream_t);
+
+}  // namespace gprx

@@ -1,0 +1,202 @@
+"""Exact GPR on the gfx950 backend, mirroring gprx's Gpr / FittedGpr API.
+
+    fitted = Gpr(kernel, GaussianLikelihood(0.1)).factor(x, n, d, y)
+    value = fitted.value_and_gradient_into(params, grad)      # objective seam
+    pred = fitted.predict(xs, q, d)                            # mean, variance
+
+Reference interfaces: Gpr::factor (src/gpr/trainer.rs:533-630),
+FittedGpr::value_and_gradient_into (src/gpr/fitted.rs:620),
+FittedGpr::predict / predict_with (src/gpr/fitted.rs:854-960).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .errors import raise_for
+from .kernels import KernelSpec
+
+
+class GaussianLikelihood:
+    """Noise variance sigma^2; its parameter is log sigma^2 (src/likelihood.rs:57-104)."""
+
+    def __init__(self, noise_variance: float):
+        if not (noise_variance > 0 and math.isfinite(noise_variance)):
+            raise ValueError("noise variance must be positive and finite")
+        self.noise_variance = float(noise_variance)
+
+    def log_param(self) -> float:
+        return math.log(self.noise_variance)
+
+
+@dataclass(frozen=True)
+class JitterPolicy:
+    """gprx's JitterPolicy (src/policy.rs:180-396)."""
+
+    initial: float = 0.0
+    multiplier: float = 1.0
+    max_jitter: float = 0.0
+    max_retries: int = 0
+
+    @staticmethod
+    def fixed(j: float) -> "JitterPolicy":
+        return JitterPolicy(j, 1.0, j, 1 if j > 0 else 0)
+
+    @staticmethod
+    def adaptive(initial: float, multiplier: float, max_retries: int, max_jitter: float):
+        return JitterPolicy(initial, multiplier, max_jitter, max_retries)
+
+    def c(self) -> L.JitterPolicyC:
+        return L.JitterPolicyC(self.initial, self.multiplier, self.max_jitter, self.max_retries)
+
+
+class Precision:
+    FP64 = L.FP64
+    MIXED = L.MIXED
+
+
+class VarianceKind:
+    OBSERVATION = L.VARIANCE_OBSERVATION
+    LATENT = L.VARIANCE_LATENT
+
+
+@dataclass
+class Prediction:
+    mean: np.ndarray
+    variance: np.ndarray | None
+
+
+def program_c(kernel: KernelSpec):
+    ops = np.ascontiguousarray(kernel.program(), dtype=np.int32)
+    prog = L.KernelProgram(ops.ctypes.data_as(C.POINTER(C.c_int32)), ops.size // 2)
+    return prog, ops  # keep ops alive with prog
+
+
+class Gpr:
+    def __init__(self, kernel: KernelSpec, likelihood: GaussianLikelihood):
+        self.kernel = kernel
+        self.likelihood = likelihood
+        self.jitter = JitterPolicy()
+        self.precision = Precision.FP64
+
+    def with_jitter_policy(self, policy: JitterPolicy) -> "Gpr":
+        self.jitter = policy
+        return self
+
+    def with_precision(self, precision: int) -> "Gpr":
+        self.precision = precision
+        return self
+
+    def factor(self, x, n: int, d: int, y) -> "FittedGpr":
+        """Gpr::<Fixed>::factor: the model at the spec's parameters, factored."""
+        fitted = self.prepare(x, n, d, y)
+        fitted.neg_log_marginal_likelihood()
+        return fitted
+
+    def prepare(self, x, n: int, d: int, y) -> "FittedGpr":
+        """Uploads the data without forming a factor."""
+        x = L.f64(x)
+        y = L.f64(y)
+        if x.size != n * d or y.size != n:
+            raise_for(L.LENGTH_MISMATCH, "x must hold n*d values and y n values")
+        theta = np.concatenate([self.kernel.get_params(), [self.likelihood.log_param()]])
+        prog, ops = program_c(self.kernel)
+        err = L.GprxError()
+        handle = C.c_void_p()
+        jit = self.jitter.c()
+        rc = L.lib().gprx_hip_exact_create(L.dptr(x), n, d, L.dptr(y), C.byref(prog),
+                                           L.dptr(theta), theta.size, self.precision,
+                                           C.byref(jit), C.byref(handle), C.byref(err))
+        raise_for(rc, err)
+        return FittedGpr(handle, theta, n, d)
+
+
+class FittedGpr:
+    def __init__(self, handle: C.c_void_p, theta: np.ndarray, n: int, d: int):
+        self._h = handle
+        self._p = theta.size
+        self.n, self.d = n, d
+        self._theta = theta.copy()
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gprx_hip_destroy(self._h)
+            self._h = None
+
+    def num_params(self) -> int:
+        return self._p
+
+    def get_params(self, out=None) -> np.ndarray:
+        buf = np.empty(self._p)
+        raise_for(L.lib().gprx_hip_get_params(self._h, L.dptr(buf), self._p), None)
+        if out is not None:
+            out[:] = buf
+        return buf
+
+    def value_and_gradient_into(self, params, out) -> float:
+        """NLML at log-params; writes the gradient into `out` (src/objective.rs:308-340)."""
+        p = L.f64(params)
+        g = np.empty(self._p)
+        val = C.c_double()
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_value_and_grad(self._h, L.dptr(p), p.size, C.byref(val),
+                                                   L.dptr(g), C.byref(err))
+        raise_for(rc, err)
+        out[:] = g
+        return val.value
+
+    def value(self, params) -> float:
+        p = L.f64(params)
+        val = C.c_double()
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_value_and_grad(self._h, L.dptr(p), p.size, C.byref(val),
+                                                   None, C.byref(err))
+        raise_for(rc, err)
+        return val.value
+
+    def neg_log_marginal_likelihood(self) -> float:
+        return self.value(self.get_params())
+
+    def predict(self, xs, q: int, d: int) -> Prediction:
+        return self.predict_with(xs, q, d, VarianceKind.OBSERVATION)
+
+    def predict_with(self, xs, q: int, d: int, variance_kind: int = VarianceKind.OBSERVATION,
+                     mean_only: bool = False) -> Prediction:
+        xs = L.f64(xs)
+        if xs.size != q * d:
+            raise_for(L.LENGTH_MISMATCH, "xs must hold q*d values")
+        mean = np.empty(q)
+        var = None if mean_only else np.empty(q)
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_predict(self._h, L.dptr(xs), q, d, variance_kind,
+                                            L.dptr(mean), None if var is None else L.dptr(var),
+                                            C.byref(err))
+        raise_for(rc, err)
+        return Prediction(mean, var)
+
+    # Diagnostics of the last evaluation.
+    def last_jitter(self) -> float:
+        return L.lib().gprx_hip_last_jitter(self._h)
+
+    def last_fallback(self) -> bool:
+        return bool(L.lib().gprx_hip_last_fallback(self._h))
+
+    def last_refine_steps(self) -> int:
+        return L.lib().gprx_hip_last_refine_steps(self._h)
+
+    def last_eval_ms(self) -> float:
+        return L.lib().gprx_hip_last_eval_ms(self._h)
+
+    def last_dense_ms(self) -> float:
+        return L.lib().gprx_hip_last_dense_ms(self._h)
+
+    def last_phase_ms(self) -> list:
+        """[fill, factor+inverse, alpha, X^T X, contraction] device ms."""
+        return [L.lib().gprx_hip_last_phase_ms(self._h, k) for k in range(5)]

@@ -1,0 +1,244 @@
+/*
+ * gprx_hip.h -- C ABI of the MI355X (gfx950) backend for gprx's GP hot path.
+ *
+ * The entry points replace the objective seam of the gprx crate
+ * (src/objective.rs:308-340, `Differentiable::value_and_gradient_into`) and
+ * the model calls around it.  Plain pointers and sizes only; nothing unwinds
+ * across this boundary; every call returns a gprx_status.  A handle owns its
+ * device buffers and one HIP stream and is not thread-safe: use one host
+ * thread per handle (the reference's models are `&mut self` the same way,
+ * src/gpr/objective.rs:14-26).
+ *
+ * Conventions shared with the reference:
+ *   - x is column-major n x d: x[k*n + i] (README.md:7).
+ *   - Parameters are log-values, ordered [kernel leaves depth-first |
+ *     log sigma^2] (src/kernel/spec.rs:29, src/gpr/objective.rs:10).
+ *   - The value is the negative log marginal likelihood including
+ *     (n/2) log 2 pi (src/gpr/factor.rs:254-267); the gradient is in log space.
+ *   - On failure the model keeps its previous parameters and factor
+ *     (src/gpr/exact_fit.rs:196-252).
+ */
+#ifndef GPRX_HIP_H
+#define GPRX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror the GprError variants the path can raise
+ * (src/error.rs; docs/design.md:799-827). */
+typedef enum gprx_status {
+  GPRX_OK = 0,
+  GPRX_CHOLESKY_FAILED = 1,          /* GprError::CholeskyFailed */
+  GPRX_NON_FINITE_KERNEL_VALUE = 2,  /* GprError::NonFiniteKernelValue */
+  GPRX_NON_FINITE_INPUT = 3,         /* GprError::NonFiniteInput */
+  GPRX_INVALID_HYPERPARAMETER = 4,   /* GprError::InvalidHyperparameter */
+  GPRX_INVALID_NOISE_VARIANCE = 5,   /* GprError::InvalidNoiseVariance */
+  GPRX_LENGTH_MISMATCH = 6,          /* GprError::LengthMismatch */
+  GPRX_EMPTY_INPUT = 7,              /* GprError::EmptyInput */
+  GPRX_UNSUPPORTED = 8,              /* a program or size this backend does not take */
+  GPRX_DEVICE_ERROR = 9              /* a HIP runtime failure */
+} gprx_status;
+
+/* CholeskyStage of GprError::CholeskyFailed (src/error.rs). */
+typedef enum gprx_stage { GPRX_STAGE_FIT = 0, GPRX_STAGE_PREDICT = 1 } gprx_stage;
+
+typedef struct gprx_error {
+  int32_t status;
+  int32_t stage;
+  double jitter;      /* last jitter tried (CholeskyFailed) */
+  size_t matrix_size; /* order of the failed factor */
+  char message[256];
+} gprx_error;
+
+/* Kernel program: a KernelSpec tree in postfix, `n_ops` pairs (op, arg).
+ * Leaves push one value; SUM and PROD pop `arg` children and push their sum
+ * or product (the flattened associative chains of src/kernel/spec.rs:384-416).
+ *   GPRX_OP_CONST     arg unused   theta = [log c]           (src/kernel/constant.rs:12-16)
+ *   GPRX_OP_RBF       arg unused   theta = [log l]           (src/kernel/rbf.rs:18-20)
+ *   GPRX_OP_RBF_ARD   arg = d      theta = [log l_0..l_d-1]  (src/kernel/rbf_ard.rs:16)
+ *   GPRX_OP_PERIODIC  arg unused   theta = [log l, log p]    (src/kernel/periodic.rs:17-23)
+ *   GPRX_OP_SUM       arg = k children
+ *   GPRX_OP_PROD      arg = k children                                             */
+enum {
+  GPRX_OP_CONST = 1,
+  GPRX_OP_RBF = 2,
+  GPRX_OP_RBF_ARD = 3,
+  GPRX_OP_PERIODIC = 4,
+  GPRX_OP_SUM = 5,
+  GPRX_OP_PROD = 6
+};
+
+typedef struct gprx_kernel_program {
+  const int32_t* ops; /* 2 * n_ops int32: op, arg */
+  size_t n_ops;
+} gprx_kernel_program;
+
+/* Arithmetic form.  FP64: everything in f64 (the oracle form).  MIXED: kernel
+ * fills and all dense trailing products in f32 on MFMA with f32 accumulation;
+ * f64 for the diagonal-block factorizations, the log-determinant, small
+ * m x m factors, and the iterative refinement of alpha with f64 residuals;
+ * falls back to FP64 when refinement stalls. */
+typedef enum gprx_precision { GPRX_FP64 = 0, GPRX_MIXED = 1 } gprx_precision;
+
+/* JitterPolicy (src/policy.rs:180-396).  The factor is tried without an
+ * offset first; on failure with offsets initial, initial*multiplier, ...,
+ * at most max_retries of them and none above max_jitter.  fixed(j) is
+ * {j, 1, j, 1}; the exact default fixed(0) is {0, 1, 0, 0}. */
+typedef struct gprx_jitter_policy {
+  double initial;
+  double multiplier;
+  double max_jitter;
+  int32_t max_retries;
+} gprx_jitter_policy;
+
+/* VarianceKind (src/prediction.rs). */
+typedef enum gprx_variance_kind {
+  GPRX_VARIANCE_OBSERVATION = 0, /* latent + sigma^2 (default) */
+  GPRX_VARIANCE_LATENT = 1
+} gprx_variance_kind;
+
+typedef struct gprx_hip_model gprx_hip_model;
+
+/* Library version and device name; never fails. */
+const char* gprx_hip_version(void);
+/* Number of visible GPUs (0 without one). */
+int gprx_hip_device_count(void);
+
+/* ------------------------------------------------------------------ exact */
+
+/* Replaces Gpr::factor / FittedGpr::prepare (src/gpr/trainer.rs:533-630,
+ * src/gpr/fitted.rs:210-270): validates the data, compiles the kernel
+ * program, uploads x and y, and sizes the device buffers.  log_theta
+ * (p = kernel params + 1) is the starting point; no factor is formed yet. */
+int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
+                          const gprx_kernel_program* kernel, const double* log_theta, size_t p,
+                          int32_t precision, const gprx_jitter_policy* jitter,
+                          gprx_hip_model** out, gprx_error* err);
+
+/* Replaces FittedGpr::value_and_gradient_into (src/gpr/fitted.rs:620) and
+ * GprObjective::value_and_gradient_into (src/gpr/objective.rs:125-132):
+ * NLML and its gradient at log_theta.  grad may be NULL (value only, as
+ * Objective::value, src/objective.rs:193).  On success the model's
+ * parameters become log_theta. */
+int gprx_hip_exact_value_and_grad(gprx_hip_model* model, const double* log_theta, size_t p,
+                                  double* nlml, double* grad, gprx_error* err);
+
+/* Replaces FittedGpr::predict_with (src/gpr/fitted.rs:854-960,
+ * src/gpr/shared.rs:754-816) at the model's current parameters.  var may be
+ * NULL (mean only). */
+int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                           int32_t variance_kind, double* mean, double* var, gprx_error* err);
+
+/* Current parameters (FittedGpr::get_params) and the jitter the last factor
+ * took. */
+int gprx_hip_get_params(const gprx_hip_model* model, double* log_theta, size_t p);
+double gprx_hip_last_jitter(const gprx_hip_model* model);
+/* For the mixed form: 1 if the last evaluation fell back to FP64, and the
+ * number of refinement corrections it took. */
+int gprx_hip_last_fallback(const gprx_hip_model* model);
+int gprx_hip_last_refine_steps(const gprx_hip_model* model);
+
+/* ------------------------------------------------------------------ sgpr */
+
+/* Replaces Sgpr::factor (src/sgpr/model.rs:448-513): x n x d, z m x d
+ * inducing points (column-major), adaptive K_mm jitter
+ * (src/policy.rs:222-226). */
+int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, const double* z,
+                         size_t m, const gprx_kernel_program* kernel, const double* log_theta,
+                         size_t p, int32_t precision, gprx_hip_model** out, gprx_error* err);
+
+/* Replaces FittedSgpr::value_and_gradient_into (src/sgpr/fitted.rs:306-326):
+ * the VFE bound (src/sgpr/factor/vfe.rs:338-369) and its gradient
+ * (src/sgpr/factor/derivatives.rs:283-366). */
+int gprx_hip_sgpr_value_and_grad(gprx_hip_model* model, const double* log_theta, size_t p,
+                                 double* value, double* grad, gprx_error* err);
+
+/* Replaces FittedSgpr::predict_with (src/sgpr/factor/predict.rs:102-260). */
+int gprx_hip_sgpr_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                          int32_t variance_kind, double* mean, double* var, gprx_error* err);
+
+/* ------------------------------------------------------------------ svgp */
+
+/* Replaces Svgp::factor's set-up (src/svgp/model.rs): data and inducing
+ * points; the variational parameters start at the prior (mean 0, L = I,
+ * src/svgp/factor/assemble.rs:144-156). */
+int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, const double* z,
+                         size_t m, const gprx_kernel_program* kernel, const double* log_theta,
+                         size_t n_theta, int32_t precision, gprx_hip_model** out, gprx_error* err);
+
+/* Total parameter count p = n_theta + m + m(m+1)/2 (src/svgp/factor/assemble.rs:158-164). */
+size_t gprx_hip_svgp_num_params(const gprx_hip_model* model);
+
+/* Replaces svgp_value_and_gradient_with (src/svgp/factor/gradient.rs:104-249):
+ * -ELBO over the batch (row indices into x) and its gradient at params
+ * ([log theta | mu | packed L_q]). */
+int gprx_hip_svgp_value_and_grad(gprx_hip_model* model, const double* params, size_t p,
+                                 const uint64_t* batch, size_t b, double* value, double* grad,
+                                 gprx_error* err);
+
+/* Replaces run_adam_fit (src/svgp/factor/adam.rs:92-141) for `steps` steps
+ * (not epochs) of Adam (src/optimizer/adam.rs:179-198) with batches drawn by
+ * the reference's Fisher-Yates epoch shuffle (src/rng.rs, seed).  params
+ * ([log theta | mu | packed L_q]) is read as the start and overwritten with
+ * the result.  values (may be NULL) receives the -ELBO of every step. */
+int gprx_hip_svgp_adam(gprx_hip_model* model, double* params, size_t p, size_t batch_size,
+                       size_t steps, double lr, double beta1, double beta2, double eps,
+                       uint64_t seed, double* values, gprx_error* err);
+
+/* Replaces the SVGP predict (src/svgp/factor/predict.rs) at params. */
+int gprx_hip_svgp_predict(gprx_hip_model* model, const double* params, size_t p, const double* xs,
+                          size_t q, size_t d, int32_t variance_kind, double* mean, double* var,
+                          gprx_error* err);
+
+/* ------------------------------------------------------------------ batch */
+
+/* One problem of a batch: independent exact GPR with its own data and
+ * program (the candidates of Lbfgs::with_restarts, src/optimizer/mod.rs:182-227,
+ * or separate datasets). */
+typedef struct gprx_problem {
+  const double* x;
+  size_t n, d;
+  const double* y;
+  const gprx_kernel_program* kernel;
+  const double* log_theta;
+  size_t p;
+} gprx_problem;
+
+/* Creates a handle over k problems that one call evaluates together. */
+int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precision,
+                          gprx_hip_model** out, gprx_error* err);
+
+/* NLML and gradient of every problem at its log_theta[i] (length p_i):
+ * nlml[k], grad[i] of length p_i; errs[k] receives each problem's status
+ * (a failed problem does not fail the others).  Returns GPRX_OK unless the
+ * call itself failed. */
+int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* log_theta,
+                                  double* nlml, double* const* grad, gprx_error* errs);
+
+/* ------------------------------------------------------------------ all */
+
+/* Frees a handle of any kind (NULL is ignored). */
+void gprx_hip_destroy(gprx_hip_model* model);
+
+/* Wall time in milliseconds of the last *_value_and_grad call measured with
+ * HIP events on the handle's stream, and the time of its dominant dense
+ * phase (factor + inverse), for roofline reporting. */
+double gprx_hip_last_eval_ms(const gprx_hip_model* model);
+double gprx_hip_last_dense_ms(const gprx_hip_model* model);
+
+/* Per-phase device time (ms) of the last exact evaluation, from HIP events
+ * on the handle's stream: 0 kernel fill, 1 factor + triangular inverse,
+ * 2 alpha (and refinement), 3 inverse product X^T X, 4 gradient contraction.
+ * No counterpart in the reference (its timing is wall clock around the whole
+ * call, compare/perf/gprx/src/timing.rs:5-22); for roofline reporting. */
+double gprx_hip_last_phase_ms(const gprx_hip_model* model, int32_t phase);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPRX_HIP_H */

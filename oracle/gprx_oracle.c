@@ -1,0 +1,1007 @@
+/*
+ * gprx_oracle.c -- CPU f64 restatement of gprx's exact / VFE / SVGP hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the HIP
+ * library under gprx_amd/csrc.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker or as the
+ * timed CPU baseline.  The product path never calls it.
+ *
+ * Every routine follows the algorithm of the gprx 0.2.0 crate (Rust, not
+ * buildable here: no cargo/rustc, dependencies not vendored); citations are
+ * to files of that crate.  Arithmetic is plain f64 in the same operation
+ * order where the reference fixes one (Cholesky, solve(L, I), W = aa' - A^-1).
+ *
+ * Pinning (what makes this restatement trustworthy):
+ *   kernels, NLML, gradients, predictions -- the sklearn goldens of
+ *     the compare/goldens JSON files (copied to tests/golden), 1e-8;
+ *   exact noise gradient -- closed-form 2x2 inverse (tests/analytic.rs);
+ *   VFE bound / predictions -- GPyTorch goldens sgpr_*.json, 1e-8;
+ *   VFE gradient -- central differences of the bound + Z = X equals exact;
+ *   SVGP -- GPyTorch prior-q goldens svgp_*.json + central differences +
+ *     Titsias-optimal q equals the VFE bound;
+ *   RNG -- first outputs for seeds 0 and 42 (src/rng.rs:105-116).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_OK 0
+#define ORC_CHOLESKY_FAILED 1
+#define ORC_BAD_PROGRAM 2
+#define ORC_BAD_INPUT 3
+
+/* ------------------------------------------------------------------------ */
+/* RNG: SplitMix64-seeded Xoshiro256++ (src/rng.rs:15-87).                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  uint64_t s[4];
+} orc_rng;
+
+static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void orc_rng_seed(orc_rng* r, uint64_t seed) {
+  uint64_t x = seed;
+  for (int i = 0; i < 4; i++) {
+    x += 0x9e3779b97f4a7c15ULL;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    r->s[i] = z ^ (z >> 31);
+  }
+}
+
+uint64_t orc_rng_next(orc_rng* r) {
+  uint64_t* s = r->s;
+  uint64_t result = rotl64(s[0] + s[3], 23) + s[0];
+  uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl64(s[3], 45);
+  return result;
+}
+
+double orc_rng_unit(orc_rng* r) {
+  return (double)(orc_rng_next(r) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* src/rng.rs:50-63, Lemire's widened-product rejection. */
+uint64_t orc_rng_up_to(orc_rng* r, uint64_t max) {
+  uint64_t range = max + 1;
+  if (range == 0) return orc_rng_next(r);
+  uint64_t threshold = (0 - range) % range;
+  for (;;) {
+    unsigned __int128 wide = (unsigned __int128)orc_rng_next(r) * range;
+    if ((uint64_t)wide >= threshold) return (uint64_t)(wide >> 64);
+  }
+}
+
+/* src/rng.rs:70-87 */
+static double orc_open_unit(orc_rng* r) {
+  double u = orc_rng_unit(r);
+  const double eps = 1.0 / 9007199254740992.0;
+  if (u <= eps) return eps;
+  if (u >= 1.0 - eps) return 1.0 - eps;
+  return u;
+}
+
+double orc_rng_normal(orc_rng* r) {
+  double u1 = orc_open_unit(r);
+  if (u1 < 2.2250738585072014e-308) u1 = 2.2250738585072014e-308;
+  double u2 = orc_open_unit(r);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* Bulk helpers so Python can draw whole arrays in the reference's order. */
+void orc_rng_fill_unit(orc_rng* r, double* out, size_t len) {
+  for (size_t i = 0; i < len; i++) out[i] = orc_rng_unit(r);
+}
+void orc_rng_fill_normal(orc_rng* r, double* out, size_t len) {
+  for (size_t i = 0; i < len; i++) out[i] = orc_rng_normal(r);
+}
+/* Fisher-Yates of src/svgp/factor/adam.rs:85-90 */
+void orc_shuffle(orc_rng* r, uint64_t* idx, size_t len) {
+  for (size_t i = len; i-- > 1;) {
+    uint64_t j = orc_rng_up_to(r, i);
+    uint64_t t = idx[i];
+    idx[i] = idx[j];
+    idx[j] = t;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernel programs.  A KernelSpec tree (src/kernel/spec.rs:106-132) in       */
+/* postfix: pairs (op, arg).  Leaves push, SUM/PROD pop `arg` children.      */
+/* Parameters are log-values in depth-first leaf order (spec.rs:29).         */
+/*   CONST    theta = [log c]                      (constant.rs:12-16)       */
+/*   RBF      theta = [log l]                      (rbf.rs:18-20)            */
+/*   RBF_ARD  theta = [log l_0 .. log l_{d-1}]     (rbf_ard.rs:16)           */
+/*   PERIODIC theta = [log l, log p]               (periodic.rs:17-23)       */
+/* ------------------------------------------------------------------------ */
+enum { OP_CONST = 1, OP_RBF = 2, OP_RBF_ARD = 3, OP_PERIODIC = 4, OP_SUM = 5, OP_PROD = 6 };
+
+#define ORC_MAX_STACK 32
+#define ORC_MAX_P 80
+
+int orc_num_params(const int32_t* ops, int nops, int d) {
+  int p = 0, depth = 0;
+  for (int k = 0; k < nops; k++) {
+    int op = ops[2 * k], arg = ops[2 * k + 1];
+    switch (op) {
+      case OP_CONST: p += 1; depth++; break;
+      case OP_RBF: p += 1; depth++; break;
+      case OP_RBF_ARD:
+        if (arg != d) return -1;
+        p += d; depth++; break;
+      case OP_PERIODIC: p += 2; depth++; break;
+      case OP_SUM:
+      case OP_PROD:
+        if (arg < 1 || arg > depth) return -1;
+        depth -= arg - 1;
+        break;
+      default: return -1;
+    }
+    if (depth > ORC_MAX_STACK) return -1;
+  }
+  if (depth != 1 || p > ORC_MAX_P) return -1;
+  return p;
+}
+
+/* Value and (optionally) the gradient w.r.t. every kernel parameter of one
+ * entry k(a, b); a and b are d-vectors with strides sa, sb.  Forward mode on
+ * the postfix stack: sums add, products use the product rule
+ * (src/kernel/compiled/weighted.rs:274-460 computes the same tree walk in
+ * reverse; the result is the same derivative). */
+static double eval_entry(const int32_t* ops, int nops, const double* theta, int P, const double* a,
+                         size_t sa, const double* b, size_t sb, int d, double* grad /* P or NULL */) {
+  double val[ORC_MAX_STACK];
+  double gst[ORC_MAX_STACK][ORC_MAX_P];
+  int top = 0, poff = 0;
+  const int want = grad != NULL;
+  for (int k = 0; k < nops; k++) {
+    int op = ops[2 * k], arg = ops[2 * k + 1];
+    if (op <= OP_PERIODIC) {
+      if (want) memset(gst[top], 0, sizeof(double) * P);
+      double v = 0.0;
+      if (op == OP_CONST) {
+        v = exp(theta[poff]);
+        if (want) gst[top][poff] = v;
+        poff += 1;
+      } else if (op == OP_RBF) {
+        double ell = exp(theta[poff]);
+        double s = 0.0;
+        for (int t = 0; t < d; t++) {
+          double df = a[t * sa] - b[t * sb];
+          s += df * df;
+        }
+        v = exp(-s / (2.0 * ell * ell));
+        if (want) gst[top][poff] = v * s / (ell * ell);
+        poff += 1;
+      } else if (op == OP_RBF_ARD) {
+        double r2 = 0.0;
+        double q[ORC_MAX_P];
+        for (int t = 0; t < d; t++) {
+          double ell = exp(theta[poff + t]);
+          double df = (a[t * sa] - b[t * sb]) / ell;
+          q[t] = df * df;
+          r2 += q[t];
+        }
+        v = exp(-0.5 * r2);
+        if (want)
+          for (int t = 0; t < d; t++) gst[top][poff + t] = v * q[t];
+        poff += d;
+      } else { /* periodic on Euclidean r (periodic.rs:593-637) */
+        double ell = exp(theta[poff]), per = exp(theta[poff + 1]);
+        double s = 0.0;
+        for (int t = 0; t < d; t++) {
+          double df = a[t * sa] - b[t * sb];
+          s += df * df;
+        }
+        double r = sqrt(s);
+        double al = M_PI * r / per;
+        double sn = sin(al);
+        double il2 = 1.0 / (ell * ell);
+        v = exp(-2.0 * sn * sn * il2);
+        if (want) {
+          gst[top][poff] = v * 4.0 * sn * sn * il2;
+          gst[top][poff + 1] = v * 4.0 * sn * cos(al) * al * il2;
+        }
+        poff += 2;
+      }
+      val[top++] = v;
+    } else {
+      int base = top - arg;
+      double acc = val[base];
+      for (int c = 1; c < arg; c++) {
+        double vc = val[base + c];
+        if (op == OP_SUM) {
+          if (want)
+            for (int p = 0; p < P; p++) gst[base][p] += gst[base + c][p];
+          acc += vc;
+        } else {
+          if (want)
+            for (int p = 0; p < P; p++) gst[base][p] = gst[base][p] * vc + acc * gst[base + c][p];
+          acc *= vc;
+        }
+      }
+      val[base] = acc;
+      top = base + 1;
+    }
+  }
+  if (want) memcpy(grad, gst[0], sizeof(double) * P);
+  return val[0];
+}
+
+/* K(a_i, b_j) into out (na x nb, column-major, ld = na).  a is na x d
+ * column-major (x[k*na + i], README.md:7), b likewise. */
+int orc_cross(const int32_t* ops, int nops, const double* theta, const double* a, size_t na,
+              const double* b, size_t nb, int d, double* out) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < (long)nb; j++)
+    for (size_t i = 0; i < na; i++)
+      out[i + j * na] = eval_entry(ops, nops, theta, P, a + i, na, b + j, nb, d, NULL);
+  return ORC_OK;
+}
+
+/* sum_ij V_ij dK(a_i, b_j)/dtheta_p  (the <V, dK> contraction of
+ * src/kernel/compiled/weighted.rs:274-300), V is na x nb column-major.
+ * If sym != 0, a == b and only the lower triangle of V is read, counting
+ * strict-lower entries twice (weighted.rs, diagonal + 2 * strict). */
+int orc_contract(const int32_t* ops, int nops, const double* theta, const double* a, size_t na,
+                 const double* b, size_t nb, int d, const double* V, int sym, double* out) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  for (int p = 0; p < P; p++) out[p] = 0.0;
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  double* part = (double*)calloc((size_t)nthreads * P, sizeof(double));
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double g[ORC_MAX_P];
+    double* acc = part + (size_t)tid * P;
+#pragma omp for schedule(static)
+    for (long j = 0; j < (long)nb; j++) {
+      size_t i0 = sym ? (size_t)j : 0;
+      for (size_t i = i0; i < na; i++) {
+        double w = V[i + j * na];
+        if (sym && i != (size_t)j) w *= 2.0;
+        eval_entry(ops, nops, theta, P, a + i, na, b + j, nb, d, g);
+        for (int p = 0; p < P; p++) acc[p] += w * g[p];
+      }
+    }
+  }
+  for (int t = 0; t < nthreads; t++)
+    for (int p = 0; p < P; p++) out[p] += part[(size_t)t * P + p];
+  free(part);
+  return ORC_OK;
+}
+
+/* k(a_i, a_i) and sum_i w * dk(a_i,a_i)/dtheta (add_diag_contraction,
+ * src/sparse/scratch.rs:283-421). */
+int orc_diag(const int32_t* ops, int nops, const double* theta, const double* a, size_t na, int d,
+             double* kdiag, double w, double* dsum) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  double g[ORC_MAX_P];
+  if (dsum)
+    for (int p = 0; p < P; p++) dsum[p] = 0.0;
+  for (size_t i = 0; i < na; i++) {
+    double v = eval_entry(ops, nops, theta, P, a + i, na, a + i, na, d, dsum ? g : NULL);
+    if (kdiag) kdiag[i] = v;
+    if (dsum)
+      for (int p = 0; p < P; p++) dsum[p] += w * g[p];
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dense f64 linear algebra (the faer calls of src/linalg/chol.rs).          */
+/* ------------------------------------------------------------------------ */
+
+/* In-place lower Cholesky of the n x n column-major a (ld = n).  Fails on a
+ * non-positive or non-finite pivot, like faer's LLT (src/linalg/chol.rs:193-219).
+ * Left-looking by columns; the column update is parallel over rows. */
+static int chol_lower(double* a, size_t n) {
+  for (size_t j = 0; j < n; j++) {
+    double* cj = a + j * n;
+    /* cj[j..n) -= sum_{k<j} L[j..n,k] * L[j,k] */
+    for (size_t k = 0; k < j; k++) {
+      const double ljk = a[j + k * n];
+      const double* ck = a + k * n;
+      for (size_t i = j; i < n; i++) cj[i] -= ck[i] * ljk;
+    }
+    double piv = cj[j];
+    if (!(piv > 0.0) || !isfinite(piv)) return ORC_CHOLESKY_FAILED;
+    double lj = sqrt(piv);
+    cj[j] = lj;
+    double inv = 1.0 / lj;
+    for (size_t i = j + 1; i < n; i++) cj[i] *= inv;
+  }
+  for (size_t j = 0; j < n; j++)
+    for (size_t i = 0; i < j; i++) a[i + j * n] = 0.0;
+  return ORC_OK;
+}
+
+/* Blocked right-looking version of chol_lower for large n, used by the CPU
+ * baseline; same factor up to summation order. */
+static int chol_lower_blocked(double* a, size_t n) {
+  const size_t nb = 64;
+  for (size_t k0 = 0; k0 < n; k0 += nb) {
+    size_t kb = (n - k0 < nb) ? n - k0 : nb;
+    /* factor the panel columns k0..k0+kb over rows k0..n */
+    for (size_t j = k0; j < k0 + kb; j++) {
+      double* cj = a + j * n;
+      for (size_t k = k0; k < j; k++) {
+        const double ljk = a[j + k * n];
+        const double* ck = a + k * n;
+        for (size_t i = j; i < n; i++) cj[i] -= ck[i] * ljk;
+      }
+      double piv = cj[j];
+      if (!(piv > 0.0) || !isfinite(piv)) return ORC_CHOLESKY_FAILED;
+      double lj = sqrt(piv);
+      cj[j] = lj;
+      double inv = 1.0 / lj;
+      for (size_t i = j + 1; i < n; i++) cj[i] *= inv;
+    }
+    /* trailing update A22 -= L21 L21^T (lower), parallel over columns */
+    size_t t0 = k0 + kb;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long jj = (long)t0; jj < (long)n; jj++) {
+      double* cj = a + (size_t)jj * n;
+      for (size_t k = k0; k < k0 + kb; k++) {
+        const double ljk = a[(size_t)jj + k * n];
+        const double* ck = a + k * n;
+        for (size_t i = (size_t)jj; i < n; i++) cj[i] -= ck[i] * ljk;
+      }
+    }
+  }
+  for (size_t j = 0; j < n; j++)
+    for (size_t i = 0; i < j; i++) a[i + j * n] = 0.0;
+  return ORC_OK;
+}
+
+/* Solve L x = b in place (forward), then L^T x = b (backward), for nrhs
+ * columns of B (ld = n).  Parallel over right-hand sides. */
+static void trsm_lower(const double* L, size_t n, double* B, size_t nrhs) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long c = 0; c < (long)nrhs; c++) {
+    double* x = B + (size_t)c * n;
+    for (size_t j = 0; j < n; j++) {
+      double v = x[j] / L[j + j * n];
+      x[j] = v;
+      const double* cl = L + j * n;
+      for (size_t i = j + 1; i < n; i++) x[i] -= cl[i] * v;
+    }
+  }
+}
+static void trsm_lower_t(const double* L, size_t n, double* B, size_t nrhs) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long c = 0; c < (long)nrhs; c++) {
+    double* x = B + (size_t)c * n;
+    for (size_t jj = n; jj-- > 0;) {
+      const double* cl = L + jj * n;
+      double s = x[jj];
+      for (size_t i = jj + 1; i < n; i++) s -= cl[i] * x[i];
+      x[jj] = s / L[jj + jj * n];
+    }
+  }
+}
+
+/* Cholesky of a + j*I with gprx's retry loop (src/linalg/chol.rs:20-43):
+ * first with no offset, then each offset of the jitter iterator
+ * (src/policy.rs:306-396): `initial`, times `mult`, at most `retries` tries,
+ * stopping above `max_j`.  Returns the offset taken in *taken. */
+static int chol_with_jitter(const double* a_in, size_t n, double* out, double initial, double mult,
+                            int retries, double max_j, double* taken, int blocked) {
+  double j = 0.0;
+  double next = initial;
+  int remaining = retries;
+  for (;;) {
+    memcpy(out, a_in, sizeof(double) * n * n);
+    for (size_t i = 0; i < n; i++) out[i + i * n] += j;
+    int rc = blocked ? chol_lower_blocked(out, n) : chol_lower(out, n);
+    if (rc == ORC_OK) {
+      if (taken) *taken = j;
+      return ORC_OK;
+    }
+    if (remaining == 0 || !(next > 0.0) || next > max_j) return ORC_CHOLESKY_FAILED;
+    remaining--;
+    j = next;
+    next = next * mult;
+    if (!isfinite(next)) remaining = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact GPR.                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* NLML and its gradient at log-theta = [kernel params | log sigma^2].
+ *   A = K + sigma^2 I (+ jitter)              src/gpr/factor.rs:135-147
+ *   L = chol(A), alpha = A^-1 y               src/linalg/chol.rs:365-391
+ *   NLML = 1/2 y'a + sum log L_ii + n/2 log 2pi   src/gpr/factor.rs:254-267
+ *   A^-1 = solve(L, I), W = aa' - A^-1        src/workspace.rs:351-425
+ *   g_p = -1/2 <W, dK/dtheta_p>               src/gpr/exact_fit.rs:690-729
+ *   g_noise = -1/2 sigma^2 sum W_ii            src/gpr/exact_fit.rs:676-682
+ * jitter_* describe the retry policy; the default exact policy is fixed(0)
+ * (src/policy.rs:230-233): pass retries = 0.
+ * Optional outputs: alpha (n), logdet (scalar). */
+int orc_exact_value_grad(const int32_t* ops, int nops, const double* theta, const double* x,
+                         size_t n, int d, const double* y, double jit_initial, double jit_mult,
+                         int jit_retries, double jit_max, double* nlml, double* grad,
+                         double* alpha_out, double* jitter_taken, int blocked) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  double noise = exp(theta[P]);
+  double* K = (double*)malloc(sizeof(double) * n * n);
+  double* L = (double*)malloc(sizeof(double) * n * n);
+  int rc = orc_cross(ops, nops, theta, x, n, x, n, d, K);
+  for (size_t i = 0; i < n; i++) K[i + i * n] += noise;
+  rc = chol_with_jitter(K, n, L, jit_initial, jit_mult, jit_retries, jit_max, jitter_taken,
+                        blocked);
+  if (rc != ORC_OK) {
+    free(K);
+    free(L);
+    return rc;
+  }
+  double* alpha = (double*)malloc(sizeof(double) * n);
+  memcpy(alpha, y, sizeof(double) * n);
+  trsm_lower(L, n, alpha, 1);
+  trsm_lower_t(L, n, alpha, 1);
+  double quad = 0.0, logdet = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    quad += y[i] * alpha[i];
+    logdet += log(L[i + i * n]);
+  }
+  *nlml = 0.5 * quad + logdet + 0.5 * (double)n * log(2.0 * M_PI);
+  if (grad) {
+    /* A^-1 = solve(L, I): reuse K's storage. */
+    double* Ai = K;
+    memset(Ai, 0, sizeof(double) * n * n);
+    for (size_t i = 0; i < n; i++) Ai[i + i * n] = 1.0;
+    trsm_lower(L, n, Ai, n);
+    trsm_lower_t(L, n, Ai, n);
+    /* W lower = aa' - A^-1 */
+    for (size_t j = 0; j < n; j++)
+      for (size_t i = j; i < n; i++) Ai[i + j * n] = alpha[i] * alpha[j] - Ai[i + j * n];
+    orc_contract(ops, nops, theta, x, n, x, n, d, Ai, 1, grad);
+    for (int p = 0; p < P; p++) grad[p] *= -0.5;
+    double trw = 0.0;
+    for (size_t i = 0; i < n; i++) trw += Ai[i + i * n];
+    grad[P] = -0.5 * noise * trw;
+  }
+  if (alpha_out) memcpy(alpha_out, alpha, sizeof(double) * n);
+  free(alpha);
+  free(K);
+  free(L);
+  return ORC_OK;
+}
+
+/* Predictive mean and variances at q points xs (src/gpr/shared.rs:754-816):
+ *   mean = K*' alpha; latent = k** - ||L^-1 k*||^2 clamped at 0;
+ *   observation = latent + sigma^2. */
+int orc_exact_predict(const int32_t* ops, int nops, const double* theta, const double* x, size_t n,
+                      int d, const double* y, const double* xs, size_t q, double* mean,
+                      double* latent, double* observation) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  double noise = exp(theta[P]);
+  double* K = (double*)malloc(sizeof(double) * n * n);
+  double* L = (double*)malloc(sizeof(double) * n * n);
+  orc_cross(ops, nops, theta, x, n, x, n, d, K);
+  for (size_t i = 0; i < n; i++) K[i + i * n] += noise;
+  int rc = chol_with_jitter(K, n, L, 0.0, 1.0, 0, 0.0, NULL, 0);
+  if (rc != ORC_OK) {
+    free(K);
+    free(L);
+    return rc;
+  }
+  double* alpha = (double*)malloc(sizeof(double) * n);
+  memcpy(alpha, y, sizeof(double) * n);
+  trsm_lower(L, n, alpha, 1);
+  trsm_lower_t(L, n, alpha, 1);
+  double* Ks = (double*)malloc(sizeof(double) * n * q);
+  orc_cross(ops, nops, theta, x, n, xs, q, d, Ks);
+  double* kss = (double*)malloc(sizeof(double) * q);
+  orc_diag(ops, nops, theta, xs, q, d, kss, 0.0, NULL);
+  for (size_t j = 0; j < q; j++) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; i++) m += Ks[i + j * n] * alpha[i];
+    mean[j] = m;
+  }
+  trsm_lower(L, n, Ks, q);
+  for (size_t j = 0; j < q; j++) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) s += Ks[i + j * n] * Ks[i + j * n];
+    double lat = kss[j] - s;
+    if (lat < 0.0) lat = 0.0;
+    if (latent) latent[j] = lat;
+    if (observation) observation[j] = lat + noise;
+  }
+  free(kss);
+  free(Ks);
+  free(alpha);
+  free(K);
+  free(L);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse VFE (Sgpr).  Follows src/sgpr/factor/vfe.rs:191-369 and the       */
+/* adjoint of src/sgpr/factor/derivatives.rs:94-201,283-366.                 */
+/* x is n x d, z is m x d (column-major).  theta = [kernel | log sigma^2].    */
+/* K_mm takes the adaptive jitter policy (1e-8, x10, 5, <= 1e-3)             */
+/* (src/policy.rs:222-226).                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  size_t n, m;
+  double noise;
+  double* L;   /* m x m chol(K_mm + j) */
+  double* A;   /* m x n  L^-1 K_mn */
+  double* LB;  /* m x m chol(A A' + s2 I) */
+  double* w;   /* m      B^-1 A y */
+  double kdiag_sum, a_f2, kmm_jitter;
+} vfe_state;
+
+static void vfe_free(vfe_state* s) {
+  free(s->L);
+  free(s->A);
+  free(s->LB);
+  free(s->w);
+}
+
+static int vfe_assemble(const int32_t* ops, int nops, const double* theta, int P, const double* x,
+                        size_t n, int d, const double* y, const double* z, size_t m,
+                        vfe_state* s) {
+  memset(s, 0, sizeof(*s));
+  s->n = n;
+  s->m = m;
+  s->noise = exp(theta[P]);
+  double* Kmm = (double*)malloc(sizeof(double) * m * m);
+  orc_cross(ops, nops, theta, z, m, z, m, d, Kmm);
+  s->L = (double*)malloc(sizeof(double) * m * m);
+  int rc = chol_with_jitter(Kmm, m, s->L, 1e-8, 10.0, 5, 1e-3, &s->kmm_jitter, 0);
+  free(Kmm);
+  if (rc != ORC_OK) return rc;
+  s->A = (double*)malloc(sizeof(double) * m * n);
+  orc_cross(ops, nops, theta, z, m, x, n, d, s->A);
+  trsm_lower(s->L, m, s->A, n);
+  /* B = A A' + s2 I (dense.rs:231-239) */
+  double* B = (double*)calloc(m * m, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long j = 0; j < (long)m; j++)
+    for (size_t i = (size_t)j; i < m; i++) {
+      double acc = 0.0;
+      for (size_t c = 0; c < n; c++) acc += s->A[i + c * m] * s->A[(size_t)j + c * m];
+      B[i + (size_t)j * m] = acc;
+      B[(size_t)j + i * m] = acc;
+    }
+  for (size_t i = 0; i < m; i++) B[i + i * m] += s->noise;
+  s->LB = (double*)malloc(sizeof(double) * m * m);
+  /* B takes JitterPolicy::default() retries (vfe.rs:244-250): none. */
+  rc = chol_with_jitter(B, m, s->LB, 0.0, 1.0, 0, 0.0, NULL, 0);
+  free(B);
+  if (rc != ORC_OK) return rc;
+  double* kd = (double*)malloc(sizeof(double) * n);
+  orc_diag(ops, nops, theta, x, n, d, kd, 0.0, NULL);
+  s->kdiag_sum = 0.0;
+  for (size_t i = 0; i < n; i++) s->kdiag_sum += kd[i];
+  free(kd);
+  s->a_f2 = 0.0;
+  for (size_t i = 0; i < m * n; i++) s->a_f2 += s->A[i] * s->A[i];
+  s->w = (double*)calloc(m, sizeof(double));
+  for (size_t c = 0; c < n; c++)
+    for (size_t i = 0; i < m; i++) s->w[i] += s->A[i + c * m] * y[c];
+  trsm_lower(s->LB, m, s->w, 1);
+  trsm_lower_t(s->LB, m, s->w, 1);
+  return ORC_OK;
+}
+
+static double vfe_bound(const vfe_state* s, const double* y, double* quad_out, double* trace_out) {
+  size_t n = s->n, m = s->m;
+  double logdet_b = 0.0;
+  for (size_t i = 0; i < m; i++) logdet_b += log(s->LB[i + i * m]);
+  logdet_b *= 2.0;
+  double logdet = ((double)n - (double)m) * log(s->noise) + logdet_b;
+  double y2 = 0.0, ayw = 0.0;
+  for (size_t c = 0; c < n; c++) {
+    y2 += y[c] * y[c];
+    double acw = 0.0;
+    for (size_t i = 0; i < m; i++) acw += s->A[i + c * m] * s->w[i];
+    ayw += acw * y[c];
+  }
+  double quad = (y2 - ayw) / s->noise;
+  double trace = (s->kdiag_sum - s->a_f2) / (2.0 * s->noise);
+  if (quad_out) *quad_out = quad;
+  if (trace_out) *trace_out = trace;
+  return 0.5 * ((double)n * log(2.0 * M_PI) + logdet + quad) + trace;
+}
+
+int orc_vfe_value_grad(const int32_t* ops, int nops, const double* theta, const double* x,
+                       size_t n, int d, const double* y, const double* z, size_t m, double* value,
+                       double* grad) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  vfe_state s;
+  int rc = vfe_assemble(ops, nops, theta, P, x, n, d, y, z, m, &s);
+  if (rc != ORC_OK) {
+    vfe_free(&s);
+    return rc;
+  }
+  double quad, trace;
+  *value = vfe_bound(&s, y, &quad, &trace);
+  if (grad) {
+    double noise = s.noise;
+    /* adjoint (derivatives.rs:102-144) */
+    double* G = (double*)malloc(sizeof(double) * m * n);
+    memcpy(G, s.A, sizeof(double) * m * n);
+    trsm_lower(s.LB, m, G, n);
+    trsm_lower_t(s.LB, m, G, n);
+    double* r = (double*)malloc(sizeof(double) * n);
+    for (size_t c = 0; c < n; c++) {
+      double acc = y[c];
+      for (size_t i = 0; i < m; i++) acc -= s.A[i + c * m] * s.w[i];
+      r[c] = acc;
+    }
+    for (size_t c = 0; c < n; c++)
+      for (size_t i = 0; i < m; i++) G[i + c * m] -= (s.w[i] * r[c] + s.A[i + c * m]) / noise;
+    double* H = (double*)calloc(m * m, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < (long)m; j++)
+      for (size_t i = 0; i < m; i++) {
+        double acc = 0.0;
+        for (size_t c = 0; c < n; c++) acc += G[i + c * m] * s.A[(size_t)j + c * m];
+        H[i + (size_t)j * m] = acc;
+      }
+    /* w_mn = L^-T G */
+    trsm_lower_t(s.L, m, G, n);
+    /* X = L^-T tril_half(H); Xt = L^-T X^T; w_mm = -1/2 (Xt^T + Xt) */
+    double* X = (double*)calloc(m * m, sizeof(double));
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = j; i < m; i++) X[i + j * m] = (i == j) ? 0.5 * H[i + j * m] : H[i + j * m];
+    trsm_lower_t(s.L, m, X, m);
+    double* Xt = (double*)malloc(sizeof(double) * m * m);
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = 0; i < m; i++) Xt[i + j * m] = X[j + i * m];
+    trsm_lower_t(s.L, m, Xt, m);
+    double* wmm = X;
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = 0; i < m; i++) wmm[i + j * m] = -0.5 * (Xt[j + i * m] + Xt[i + j * m]);
+    double* g1 = (double*)calloc(P, sizeof(double));
+    double* g2 = (double*)calloc(P, sizeof(double));
+    double* g3 = (double*)calloc(P, sizeof(double));
+    orc_contract(ops, nops, theta, z, m, z, m, d, wmm, 0, g1);
+    orc_contract(ops, nops, theta, z, m, x, n, d, G, 0, g2);
+    orc_diag(ops, nops, theta, x, n, d, NULL, 0.5 / noise, g3);
+    for (int p = 0; p < P; p++) grad[p] = g1[p] + g2[p] + g3[p];
+    /* noise: directional_noise (derivatives.rs:186-201) */
+    double trb = 0.0; /* tr(B^-1) via L_B */
+    {
+      double* Bi = (double*)calloc(m * m, sizeof(double));
+      for (size_t i = 0; i < m; i++) Bi[i + i * m] = 1.0;
+      trsm_lower(s.LB, m, Bi, m);
+      for (size_t i = 0; i < m * m; i++) trb += Bi[i] * Bi[i];
+      free(Bi);
+    }
+    double w2 = 0.0;
+    for (size_t i = 0; i < m; i++) w2 += s.w[i] * s.w[i];
+    double d_logdet_b = noise * trb;
+    double d_q = -noise * w2;
+    double d_quad = -d_q / noise - quad;
+    double d_trace = -trace;
+    grad[P] = 0.5 * (((double)n - (double)m) + d_logdet_b + d_quad) + d_trace;
+    free(g1);
+    free(g2);
+    free(g3);
+    free(Xt);
+    free(X);
+    free(H);
+    free(r);
+    free(G);
+  }
+  vfe_free(&s);
+  return ORC_OK;
+}
+
+/* VFE predictions (src/sgpr/factor/predict.rs:102-260):
+ *   a* = L^-1 k_m*, mean = a*' w,
+ *   latent = k** - ||a*||^2 + s2 ||L_B^-1 a*||^2 (clamped at 0). */
+int orc_vfe_predict(const int32_t* ops, int nops, const double* theta, const double* x, size_t n,
+                    int d, const double* y, const double* z, size_t m, const double* xs, size_t q,
+                    double* mean, double* latent, double* observation) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  vfe_state s;
+  int rc = vfe_assemble(ops, nops, theta, P, x, n, d, y, z, m, &s);
+  if (rc != ORC_OK) {
+    vfe_free(&s);
+    return rc;
+  }
+  double* As = (double*)malloc(sizeof(double) * m * q);
+  orc_cross(ops, nops, theta, z, m, xs, q, d, As);
+  trsm_lower(s.L, m, As, q);
+  double* kss = (double*)malloc(sizeof(double) * q);
+  orc_diag(ops, nops, theta, xs, q, d, kss, 0.0, NULL);
+  double* Bs = (double*)malloc(sizeof(double) * m * q);
+  memcpy(Bs, As, sizeof(double) * m * q);
+  trsm_lower(s.LB, m, Bs, q);
+  for (size_t j = 0; j < q; j++) {
+    double mu = 0.0, a2 = 0.0, b2 = 0.0;
+    for (size_t i = 0; i < m; i++) {
+      mu += As[i + j * m] * s.w[i];
+      a2 += As[i + j * m] * As[i + j * m];
+      b2 += Bs[i + j * m] * Bs[i + j * m];
+    }
+    double lat = kss[j] - a2 + s.noise * b2;
+    if (lat < 0.0) lat = 0.0;
+    mean[j] = mu;
+    if (latent) latent[j] = lat;
+    if (observation) observation[j] = lat + s.noise;
+  }
+  free(Bs);
+  free(kss);
+  free(As);
+  vfe_free(&s);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SVGP, whitened q(u) = N(mu, Lq Lq').  Follows                             */
+/* src/svgp/factor/gradient.rs:104-470 and assemble.rs:229-280.              */
+/* params = [kernel theta | log s2 | mu (m) | packed Lq (m(m+1)/2, columns   */
+/* j, rows i >= j)].  The batch is `b` row indices into x.                    */
+/* Returns -ELBO = KL - (n/b) sum_batch ell_i and its gradient (user space:  */
+/* raw Lq entries, log theta).                                               */
+/* ------------------------------------------------------------------------ */
+int orc_svgp_value_grad(const int32_t* ops, int nops, const double* params, const double* x,
+                        size_t n, int d, const double* y, const double* z, size_t m,
+                        const uint64_t* batch, size_t b, double* value, double* grad) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  if (b == 0) return ORC_BAD_INPUT;
+  const int nth = P + 1;
+  const double* theta = params;
+  const double noise = exp(params[P]);
+  const double inv_noise = 1.0 / noise;
+  const double scale = (double)n / (double)b;
+  const double* qm = params + nth;
+  const double* qp = params + nth + m;
+  size_t pl = m * (m + 1) / 2;
+  size_t ptot = (size_t)nth + m + pl;
+  /* unpack Lq (assemble.rs:186-225) */
+  double* Lq = (double*)calloc(m * m, sizeof(double));
+  {
+    size_t k = 0;
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = j; i < m; i++) Lq[i + j * m] = qp[k++];
+  }
+  /* K_mm + chol with the K_mm default adaptive jitter (step.rs:73-126) */
+  double* Kmm = (double*)malloc(sizeof(double) * m * m);
+  orc_cross(ops, nops, theta, z, m, z, m, d, Kmm);
+  double* L = (double*)malloc(sizeof(double) * m * m);
+  int rc = chol_with_jitter(Kmm, m, L, 1e-8, 10.0, 5, 1e-3, NULL, 0);
+  free(Kmm);
+  if (rc != ORC_OK) {
+    free(L);
+    free(Lq);
+    return rc;
+  }
+  /* batch points */
+  double* xb = (double*)malloc(sizeof(double) * b * d);
+  for (size_t r = 0; r < b; r++)
+    for (int t = 0; t < d; t++) xb[r + t * b] = x[batch[r] + t * n];
+  double* A = (double*)malloc(sizeof(double) * m * b);
+  orc_cross(ops, nops, theta, z, m, xb, b, d, A);
+  trsm_lower(L, m, A, b);
+  double* kd = (double*)malloc(sizeof(double) * b);
+  orc_diag(ops, nops, theta, xb, b, d, kd, 0.0, NULL);
+  for (size_t i = 0; i < ptot; i++) grad[i] = 0.0;
+  /* KL and its gradient (accumulate_kl_grad) */
+  double trs = 0.0, logdet_s = 0.0, mn2 = 0.0;
+  {
+    size_t k = 0;
+    for (size_t j = 0; j < m; j++) {
+      logdet_s += log(Lq[j + j * m]);
+      for (size_t i = j; i < m; i++) {
+        double v = Lq[i + j * m];
+        trs += v * v;
+        grad[nth + m + k] = (i == j) ? v - 1.0 / v : v;
+        k++;
+      }
+    }
+    logdet_s *= 2.0;
+    for (size_t k2 = 0; k2 < m; k2++) {
+      mn2 += qm[k2] * qm[k2];
+      grad[nth + k2] = qm[k2];
+    }
+  }
+  double kl = 0.5 * (trs + mn2 - (double)m - logdet_s);
+  /* U = Lq' A */
+  double* U = (double*)calloc(m * b, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < (long)b; c++)
+    for (size_t jj = 0; jj < m; jj++) {
+      double acc = 0.0;
+      for (size_t i = jj; i < m; i++) acc += Lq[i + jj * m] * A[i + (size_t)c * m];
+      U[jj + (size_t)c * m] = acc;
+    }
+  /* point terms */
+  double* resid = (double*)malloc(sizeof(double) * b);
+  double ell = 0.0, r2v = 0.0;
+  const double log2pin = log(2.0 * M_PI * noise);
+  for (size_t c = 0; c < b; c++) {
+    double mu = 0.0, a2 = 0.0, u2 = 0.0;
+    for (size_t i = 0; i < m; i++) {
+      double av = A[i + c * m], uv = U[i + c * m];
+      mu += av * qm[i];
+      a2 += av * av;
+      u2 += uv * uv;
+    }
+    double var = kd[c] - a2 + u2;
+    double rr = y[batch[c]] - mu;
+    resid[c] = rr;
+    ell += -0.5 * log2pin - 0.5 * inv_noise * (rr * rr + var);
+    r2v += rr * rr + var;
+  }
+  const double cc = scale * inv_noise;
+  /* mean grad: -= c * A resid */
+  for (size_t i = 0; i < m; i++) {
+    double acc = 0.0;
+    for (size_t c = 0; c < b; c++) acc += A[i + c * m] * resid[c];
+    grad[nth + i] -= cc * acc;
+  }
+  /* packed L grad: += c * (A U')_ij for i >= j */
+  {
+    size_t k = 0;
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = j; i < m; i++) {
+        double acc = 0.0;
+        for (size_t c = 0; c < b; c++) acc += A[i + c * m] * U[j + c * m];
+        grad[nth + m + k] += cc * acc;
+        k++;
+      }
+  }
+  grad[P] = -scale * (-0.5 * (double)b + 0.5 * inv_noise * r2v);
+  /* adjoint weights (gradient.rs:410-470) */
+  double* wmn = (double*)malloc(sizeof(double) * m * b);
+  for (size_t c = 0; c < b; c++)
+    for (size_t i = 0; i < m; i++) {
+      double lu = 0.0;
+      for (size_t k = 0; k <= i; k++) lu += Lq[i + k * m] * U[k + c * m];
+      wmn[i + c * m] = -inv_noise * lu + inv_noise * (qm[i] * resid[c] + A[i + c * m]);
+    }
+  double* wmm = (double*)calloc(m * m, sizeof(double));
+  for (size_t j = 0; j < m; j++)
+    for (size_t i = 0; i < m; i++) {
+      double acc = 0.0;
+      for (size_t c = 0; c < b; c++) acc += wmn[i + c * m] * A[j + c * m];
+      wmm[i + j * m] = acc;
+    }
+  trsm_lower_t(L, m, wmn, b);
+  double* half = (double*)calloc(m * m, sizeof(double));
+  for (size_t j = 0; j < m; j++)
+    for (size_t i = j; i < m; i++) half[i + j * m] = (i == j) ? 0.5 * wmm[i + j * m] : wmm[i + j * m];
+  trsm_lower_t(L, m, half, m);
+  for (size_t j = 0; j < m; j++)
+    for (size_t i = 0; i < m; i++) wmm[i + j * m] = half[j + i * m];
+  trsm_lower_t(L, m, wmm, m);
+  for (size_t j = 0; j < m; j++)
+    for (size_t i = j; i < m; i++) {
+      double sym = -0.5 * (wmm[i + j * m] + wmm[j + i * m]);
+      wmm[i + j * m] = sym;
+      wmm[j + i * m] = sym;
+    }
+  double* g1 = (double*)calloc(P, sizeof(double));
+  double* g2 = (double*)calloc(P, sizeof(double));
+  double* g3 = (double*)calloc(P, sizeof(double));
+  orc_contract(ops, nops, theta, z, m, z, m, d, wmm, 0, g1);
+  orc_contract(ops, nops, theta, z, m, xb, b, d, wmn, 0, g2);
+  orc_diag(ops, nops, theta, xb, b, d, NULL, -0.5 * inv_noise, g3);
+  for (int p = 0; p < P; p++) grad[p] = -scale * (g1[p] + g2[p] + g3[p]);
+  *value = kl - scale * ell;
+  free(g1);
+  free(g2);
+  free(g3);
+  free(half);
+  free(wmm);
+  free(wmn);
+  free(resid);
+  free(U);
+  free(kd);
+  free(A);
+  free(xb);
+  free(L);
+  free(Lq);
+  return ORC_OK;
+}
+
+/* SVGP predictions with q (src/svgp/factor/predict.rs): a = L^-1 k_m*,
+ * mean = a' mu, latent = k** - ||a||^2 + ||Lq' a||^2. */
+int orc_svgp_predict(const int32_t* ops, int nops, const double* params, int d, const double* z,
+                     size_t m, const double* xs, size_t q, double* mean, double* latent,
+                     double* observation) {
+  int P = orc_num_params(ops, nops, d);
+  if (P < 0) return ORC_BAD_PROGRAM;
+  const double noise = exp(params[P]);
+  const double* qm = params + P + 1;
+  const double* qp = params + P + 1 + m;
+  double* Lq = (double*)calloc(m * m, sizeof(double));
+  {
+    size_t k = 0;
+    for (size_t j = 0; j < m; j++)
+      for (size_t i = j; i < m; i++) Lq[i + j * m] = qp[k++];
+  }
+  double* Kmm = (double*)malloc(sizeof(double) * m * m);
+  orc_cross(ops, nops, params, z, m, z, m, d, Kmm);
+  double* L = (double*)malloc(sizeof(double) * m * m);
+  int rc = chol_with_jitter(Kmm, m, L, 1e-8, 10.0, 5, 1e-3, NULL, 0);
+  free(Kmm);
+  if (rc != ORC_OK) {
+    free(L);
+    free(Lq);
+    return rc;
+  }
+  double* A = (double*)malloc(sizeof(double) * m * q);
+  orc_cross(ops, nops, params, z, m, xs, q, d, A);
+  trsm_lower(L, m, A, q);
+  double* kss = (double*)malloc(sizeof(double) * q);
+  orc_diag(ops, nops, params, xs, q, d, kss, 0.0, NULL);
+  for (size_t c = 0; c < q; c++) {
+    double mu = 0.0, a2 = 0.0, u2 = 0.0;
+    for (size_t jj = 0; jj < m; jj++) {
+      double av = A[jj + c * m];
+      mu += av * qm[jj];
+      a2 += av * av;
+      double u = 0.0;
+      for (size_t i = jj; i < m; i++) u += Lq[i + jj * m] * A[i + c * m];
+      u2 += u * u;
+    }
+    double lat = kss[c] - a2 + u2;
+    if (lat < 0.0) lat = 0.0;
+    mean[c] = mu;
+    if (latent) latent[c] = lat;
+    if (observation) observation[c] = lat + noise;
+  }
+  free(kss);
+  free(A);
+  free(L);
+  free(Lq);
+  return ORC_OK;
+}
+
+/* Kingma-Ba Adam step with bias correction (src/optimizer/adam.rs:179-198). */
+void orc_adam_step(double* zv, const double* g, double* m1, double* m2, uint64_t* t, size_t len,
+                   double lr, double b1, double b2, double eps) {
+  *t += 1;
+  double tt = (double)*t;
+  double c1 = 1.0 - pow(b1, tt), c2 = 1.0 - pow(b2, tt);
+  for (size_t i = 0; i < len; i++) {
+    m1[i] = b1 * m1[i] + (1.0 - b1) * g[i];
+    m2[i] = b2 * m2[i] + (1.0 - b2) * g[i] * g[i];
+    double mh = m1[i] / c1, vh = m2[i] / c2;
+    zv[i] -= lr * mh / (sqrt(vh) + eps);
+  }
+}
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,156 @@
+"""Exact GPR on the GPU against the reference goldens and the CPU oracle.
+
+FP64 form: NLML, every gradient component, mean and variance within 1e-10
+(|a - e| / max(|e|, 1)) of the oracle, and within the reference's 1e-8 of
+its sklearn goldens.  MIXED form: within 1e-6 of the oracle on these
+well-conditioned problems (f32 factor, f64 refined alpha).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gprx_amd as g
+import oracle
+from conftest import EXACT_COMPOSITE, EXACT_RBF, assert_close, golden_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _fit(spec, noise, x, n, d, y, precision=g.Precision.FP64, jitter=None):
+    gp = g.Gpr(spec, g.GaussianLikelihood(noise)).with_precision(precision)
+    if jitter is not None:
+        gp = gp.with_jitter_policy(jitter)
+    return gp.prepare(x, n, d, y)
+
+
+@pytest.mark.parametrize("name", EXACT_RBF + EXACT_COMPOSITE)
+def test_goldens_fp64(gpu, name):
+    spec, th, x, n, d, y, xs, q, gd = golden_problem(name)
+    print(f"[exact golden] {name} n={n} d={d}", flush=True)
+    m = _fit(spec, gd["noise_variance"], x, n, d, y)
+    grad = np.empty(th.size)
+    val = m.value_and_gradient_into(th, grad)
+    assert_close(-val, gd["log_marginal_likelihood"], 1e-8, "lml")
+    if "grad_theta" in gd:
+        assert_close(-grad[: len(gd["grad_theta"])], gd["grad_theta"], 1e-8, "grad")
+    if "grad_log_lengthscale" in gd:
+        assert_close(-grad[0], gd["grad_log_lengthscale"], 1e-8, "grad")
+    ov, og, _, _ = oracle.exact_value_grad(spec.program(), th, x, n, d, y)
+    assert_close(val, ov, TOL64, "nlml vs oracle")
+    assert_close(grad, og, TOL64, "grad vs oracle")
+    p = m.predict(xs, q, d)
+    lat = m.predict_with(xs, q, d, g.VarianceKind.LATENT)
+    assert_close(p.mean, gd["mean"], 1e-8, "mean")
+    assert_close(p.variance, gd["observation_variance"], 1e-8, "obs var")
+    assert_close(lat.variance, gd["latent_variance"], 1e-8, "latent var")
+
+
+def _problem(n, d, seed, kname):
+    rng = np.random.default_rng(seed)
+    x = rng.random(n * d)
+    y = np.sin(3 * x[:n]) + (np.cos(2 * x[n:2 * n]) if d > 1 else 0) + 0.1 * rng.standard_normal(n)
+    ls = list(0.3 + 0.7 * rng.random(d))
+    specs = {
+        "rbf": g.RbfKernel(0.6),
+        "const_ard": g.ConstantKernel(1.3) * g.RbfArdKernel(ls),
+        "const_ard_plus_periodic": g.ConstantKernel(0.8) * g.RbfArdKernel(ls) + g.PeriodicKernel(1.1, 2.3),
+        "sum_times": (g.RbfKernel(0.5) + g.PeriodicKernel(1.0, 1.5)) * (g.ConstantKernel(2.0) + g.RbfKernel(1.5)),
+    }
+    return x, y, specs[kname]
+
+
+CASES = [(1, 1, "rbf"), (7, 2, "const_ard"), (128, 3, "const_ard"), (129, 4, "const_ard_plus_periodic"),
+         (300, 8, "const_ard"), (257, 5, "sum_times"), (1000, 8, "const_ard"), (700, 16, "const_ard")]
+
+
+@pytest.mark.parametrize("n,d,kname", CASES)
+def test_random_vs_oracle_fp64(gpu, n, d, kname):
+    print(f"[exact fp64] n={n} d={d} kernel={kname}", flush=True)
+    x, y, spec = _problem(n, d, n + d, kname)
+    noise = 0.05
+    th = np.concatenate([spec.get_params(), [math.log(noise)]])
+    m = _fit(spec, noise, x, n, d, y)
+    grad = np.empty(th.size)
+    val = m.value_and_gradient_into(th, grad)
+    ov, og, _, _ = oracle.exact_value_grad(spec.program(), th, x, n, d, y)
+    assert_close(val, ov, TOL64, "nlml")
+    assert_close(grad, og, TOL64, "grad")
+    q = 37
+    xs = np.random.default_rng(5).random(q * d)
+    p = m.predict(xs, q, d)
+    om, ol, oo = oracle.exact_predict(spec.program(), th, x, n, d, y, xs, q)
+    assert_close(p.mean, om, TOL64, "mean")
+    assert_close(p.variance, oo, TOL64, "var")
+
+
+@pytest.mark.parametrize("n,d,kname", [(300, 8, "const_ard"), (1000, 8, "const_ard"), (257, 5, "sum_times")])
+def test_random_vs_oracle_mixed(gpu, n, d, kname):
+    print(f"[exact mixed] n={n} d={d} kernel={kname}", flush=True)
+    x, y, spec = _problem(n, d, n + d, kname)
+    noise = 0.05
+    th = np.concatenate([spec.get_params(), [math.log(noise)]])
+    m = _fit(spec, noise, x, n, d, y, precision=g.Precision.MIXED)
+    grad = np.empty(th.size)
+    val = m.value_and_gradient_into(th, grad)
+    ov, og, _, _ = oracle.exact_value_grad(spec.program(), th, x, n, d, y)
+    print(f"  nlml err {rel_err(val, ov):.2e} grad err {rel_err(grad, og):.2e} "
+          f"refine {m.last_refine_steps()} fallback {m.last_fallback()}")
+    assert_close(val, ov, 1e-6, "nlml")
+    assert_close(grad, og, 1e-5, "grad")
+
+
+def test_value_only_and_repeat_deterministic(gpu):
+    x, y, spec = _problem(500, 4, 11, "const_ard")
+    th = np.concatenate([spec.get_params(), [math.log(0.1)]])
+    m = _fit(spec, 0.1, x, 500, 4, y)
+    g1, g2 = np.empty(th.size), np.empty(th.size)
+    v1 = m.value_and_gradient_into(th, g1)
+    v2 = m.value_and_gradient_into(th, g2)
+    assert v1 == v2 and np.array_equal(g1, g2)
+    assert m.value(th) == v1
+
+
+def test_cholesky_failure_keeps_parameters(gpu):
+    x = np.array([0.0, 0.0, 1.0, 2.0])
+    y = np.array([1.0, 1.0, 0.0, 0.5])
+    spec = g.RbfKernel(1.0)
+    good = np.array([0.0, math.log(0.1)])
+    bad = np.array([0.0, math.log(1e-300)])
+    m = _fit(spec, 0.1, x, 4, 1, y)
+    gr = np.empty(2)
+    v0 = m.value_and_gradient_into(good, gr)
+    with pytest.raises(g.CholeskyFailed):
+        m.value_and_gradient_into(bad, gr)
+    assert np.array_equal(m.get_params(), good)
+    assert m.value(good) == v0
+    p = m.predict(np.array([0.5]), 1, 1)
+    assert np.isfinite(p.mean[0])
+
+
+def test_adaptive_jitter_matches_oracle(gpu):
+    x = np.array([0.0, 0.0, 1.0, 2.0])
+    y = np.array([1.0, 1.0, 0.0, 0.5])
+    spec = g.RbfKernel(1.0)
+    th = np.array([0.0, math.log(1e-300)])
+    m = _fit(spec, 0.1, x, 4, 1, y, jitter=g.JitterPolicy.adaptive(1e-8, 10.0, 5, 1e-3))
+    gr = np.empty(2)
+    val = m.value_and_gradient_into(th, gr)
+    ov, og, _, taken = oracle.exact_value_grad(spec.program(), th, x, 4, 1, y, jitter=(1e-8, 10.0, 5, 1e-3))
+    assert m.last_jitter() == taken
+    assert_close(val, ov, 1e-9, "nlml")
+
+
+def test_input_errors(gpu):
+    spec = g.RbfKernel(1.0)
+    with pytest.raises(g.NonFiniteInput):
+        _fit(spec, 0.1, np.array([0.0, np.nan]), 2, 1, np.array([0.0, 1.0]))
+    m = _fit(spec, 0.1, np.array([0.0, 1.0]), 2, 1, np.array([0.0, 1.0]))
+    with pytest.raises(g.LengthMismatch):
+        m.value_and_gradient_into(np.zeros(3), np.empty(3))
+    with pytest.raises(g.InvalidNoiseVariance):
+        m.value_and_gradient_into(np.array([0.0, np.inf]), np.empty(2))
+    with pytest.raises(g.LengthMismatch):
+        _fit(g.RbfArdKernel([1.0, 1.0]), 0.1, np.zeros(3), 3, 1, np.zeros(3))

@@ -10,6 +10,9 @@ from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InvalidH
                      Unsupported)
 from .gpr import (FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision, Prediction,
                   VarianceKind)
+from .batch import GprBatch
+from .sgpr import FittedSgpr, Sgpr
+from .svgp import FittedSvgp, Svgp, prior_params
 from .kernels import (ConstantKernel, KernelSpec, PeriodicKernel, Product, RbfArdKernel, RbfKernel,
                       Sum)
 
@@ -18,5 +21,5 @@ __all__ = [
     "InvalidNoiseVariance", "LengthMismatch", "NonFiniteInput", "NonFiniteKernelValue",
     "Unsupported", "FittedGpr", "GaussianLikelihood", "Gpr", "JitterPolicy", "Precision",
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
-    "RbfArdKernel", "RbfKernel", "Sum",
+    "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
 ]

@@ -57,3 +57,26 @@ def exact_rbf_small(n: int = 1024, d: int = 4, q: int = 256) -> ExactWorkload:
     y = np.sin(3 * X[0]) + np.cos(2 * X[1]) + 0.5 * X[2] * X[3] + 0.1 * eps
     xs = SeededRng(1).units(q * d)
     return ExactWorkload(n, d, x, y, xs, q, np.ones(1), 1.0, 0.1)
+
+
+def batched_problems(k: int = 256, seed: int = 5, n_lo: int = 256, n_hi: int = 4096):
+    """k independent ARD-RBF problems: n_i ~ U{n_lo..n_hi}, d_i in {2, 4, 8, 16},
+    lengthscales log-U[0.1, 2], noise log-U[1e-3, 1e-1], x ~ U[0,1]^d, y from a
+    256-feature random Fourier draw plus noise; one stream (seed)."""
+    r = SeededRng(seed)
+    out = []
+    for _ in range(k):
+        n = n_lo + r.up_to(n_hi - n_lo)
+        d = (2, 4, 8, 16)[r.up_to(3)]
+        ls = np.array([math.exp(math.log(0.1) + r.unit() * math.log(20.0)) for _ in range(d)])
+        noise = math.exp(math.log(1e-3) + r.unit() * math.log(100.0))
+        x = r.units(n * d)
+        nf = 256
+        W = r.normals(nf * d).reshape(nf, d) / ls[None, :]
+        b = 2 * math.pi * r.units(nf)
+        a = r.normals(nf)
+        X = x.reshape(d, n).T
+        f = (np.cos(X @ W.T + b[None, :]) @ a) * math.sqrt(2.0 / nf)
+        y = f + math.sqrt(noise) * r.normals(n)
+        out.append((n, d, x, y, ls, noise))
+    return out

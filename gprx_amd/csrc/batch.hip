@@ -1,0 +1,867 @@
+// Many independent exact-GPR problems of uneven n in grouped launches.
+//
+// Every problem is padded to whole 128-tiles and lives in two arenas (the
+// kernel matrix / its factor, and the inverse X = L^-1).  The factor is
+// right-looking and level-synchronous: step k of every problem with more
+// than k tiles runs in the same three launches,
+//   1. diagonal block k: Cholesky + inverse in LDS (f64)           -> X_kk
+//   2. panel L_ik = A_ik X_kk^T (i > k) and P_kj <- X_kk P_kj (j < k)
+//   3. trailing A_ij -= L_ik L_jk^T (k < j <= i) and P_ij -= L_ik P_kj
+//      (i > k, j <= k)
+// where P accumulates L^-1 = E_{nt-1}^-1 ... E_0^-1 (E_k the block column k
+// of L), so the inverse costs no extra pass: its updates ride in the
+// factor's launches.  Then A^-1 = X^T X, alpha, the NLML and the fused
+// gradient contraction run as one grouped launch each.  Each launch reads a
+// task list (one 128 x 128 output tile per workgroup) built once at set-up,
+// longest reductions first.  The whole evaluation can be captured in one
+// hipGraph and replayed per optimizer iteration with only the per-problem
+// parameters re-uploaded.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "kernel_eval.hpp"
+#include "model.hpp"
+#include "tile_gemm.hpp"
+
+namespace gprx {
+int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err);
+}
+
+using namespace gprx;
+
+namespace {
+
+constexpr double kLog2Pi = 1.8378770664093453;
+
+// One output tile of a grouped product: C(c) = beta C + alpha op(A) op(B),
+// op(A) 128 x K at a, op(B) K x 128 at b (element offsets into the arenas).
+struct TileTask {
+  long a, b, c;
+  int ld;
+  int K;
+  int tb;     // B stored N x K (NT) instead of K x N (NN)
+  int ta;     // A stored K x M (TN)
+  int lower;  // diagonal tile: keep r >= c
+  float alpha, beta;
+};
+
+template <typename T, bool TA, bool TB>
+__device__ __forceinline__ void run_task(const TileTask& t, const T* Aar, const T* Bar, T* Car,
+                                         TileSmem<T>& sm) {
+  using AV = typename Acc<T>::V;
+  AV acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
+  tile_mainloop<T, TA, TB>(Aar + t.a, t.ld, Bar + t.b, t.ld, 0, t.K / Cfg<T>::BK, sm, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  T* C = Car + t.c;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = wm * 64 + i * 16 + (lane & 15);
+        const int c = wn * 64 + j * 16 + drow<T>(lane, q);
+        if (t.lower && r < c) continue;
+        T* p = C + (long)c * t.ld + r;
+        double v = (double)t.alpha * (double)acc[i][j][q];
+        if (t.beta != 0.f) v += (double)t.beta * (double)(*p);
+        *p = (T)v;
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tasks(const TileTask* __restrict__ tasks, int count,
+                                                    const T* Aar, const T* Bar, T* Car) {
+  __shared__ TileSmem<T> sm;
+  const int id = xcd_remap(blockIdx.x, count);
+  const TileTask t = tasks[id];
+  if (t.ta)
+    run_task<T, true, false>(t, Aar, Bar, Car, sm);
+  else if (t.tb)
+    run_task<T, false, true>(t, Aar, Bar, Car, sm);
+  else
+    run_task<T, false, false>(t, Aar, Bar, Car, sm);
+}
+
+// Per-problem descriptors on the device.
+struct ProbDesc {
+  long xoff;   // points (rows npad, dim dp) in the point arena
+  long voff;   // vectors (npad) in the vector arena
+  long moff;   // matrices (npad x npad) in the arenas
+  int n, npad;
+};
+
+// Lower-tile fill of every problem's K + noise I.
+struct FillTask {
+  int prob, ti, tj;
+};
+
+template <int DP, typename T, bool DUAL>
+__global__ __launch_bounds__(256) void k_fill_tasks(const FillTask* __restrict__ tasks,
+                                                    const ProbDesc* __restrict__ probs,
+                                                    const DevProgram* __restrict__ progs,
+                                                    const double* __restrict__ xar, T* Aar,
+                                                    double* K64, int* info) {
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  const FillTask t = tasks[blockIdx.x];
+  const ProbDesc pd = probs[t.prob];
+  const DevProgram& P = progs[t.prob];
+  const int r0 = t.ti * kTile, c0 = t.tj * kTile;
+  load_points<DP>(xar + pd.xoff, pd.npad, r0, xr);
+  load_points<DP>(xar + pd.xoff, pd.npad, c0, xc);
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  bool bad = false;
+  T* A = Aar + pd.moff;
+  for (int jj = 0; jj < 64; ++jj) {
+    const int j = jh + 2 * jj;
+    const int gj = c0 + j;
+    if (gi < gj) continue;
+    double val;
+    if (gi >= pd.n || gj >= pd.n) {
+      val = gi == gj ? 1.0 : 0.0;
+    } else {
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      val = eval_entry<DP, false>(P, dx, L);
+      if (!isfinite(val)) bad = true;
+      if (gi == gj) val += P.noise;
+    }
+    A[gi + (long)gj * pd.npad] = (T)val;
+    if (DUAL) K64[pd.moff + gi + (long)gj * pd.npad] = val;
+  }
+  if (bad) atomicOr(info + t.prob, 2);
+}
+
+// Fused gradient contraction over every problem's lower tiles; W_ij =
+// alpha_i alpha_j - Ainv_ij.  Partials per task (tasks of a problem are
+// consecutive).
+template <int DP, typename T>
+__global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restrict__ tasks,
+                                                        const ProbDesc* __restrict__ probs,
+                                                        const DevProgram* __restrict__ progs,
+                                                        const double* __restrict__ xar,
+                                                        const T* __restrict__ Ainv,
+                                                        const double* __restrict__ alpha,
+                                                        double* __restrict__ partial) {
+  __shared__ double xr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  __shared__ double ar[kTile], ac[kTile];
+  __shared__ double red[4 * kMaxAcc];
+  const FillTask t = tasks[blockIdx.x];
+  const ProbDesc pd = probs[t.prob];
+  const DevProgram& P = progs[t.prob];
+  const int r0 = t.ti * kTile, c0 = t.tj * kTile;
+  load_points<DP>(xar + pd.xoff, pd.npad, r0, xr);
+  load_points<DP>(xar + pd.xoff, pd.npad, c0, xc);
+  if (threadIdx.x < kTile) {
+    ar[threadIdx.x] = alpha[pd.voff + r0 + threadIdx.x];
+    ac[threadIdx.x] = alpha[pd.voff + c0 + threadIdx.x];
+  }
+  __syncthreads();
+  const int i = threadIdx.x & 127, jh = threadIdx.x >> 7;
+  const int gi = r0 + i;
+  double xi[DP];
+#pragma unroll
+  for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
+  double acc[kMaxLeaves][kAccPerLeaf];
+  double accd[DP];
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) accd[q] = 0.0;
+  const T* M = Ainv + pd.moff;
+  if (gi < pd.n) {
+    for (int jj = 0; jj < 64; ++jj) {
+      const int j = jh + 2 * jj;
+      const int gj = c0 + j;
+      if (gj > gi || gj >= pd.n) continue;
+      double w = ar[i] * ac[j] - (double)M[gi + (long)gj * pd.npad];
+      if (gi != gj) w *= 2.0;
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
+      Leaves L;
+      eval_entry<DP, true>(P, dx, L);
+      accumulate_entry<DP>(P, dx, L, w, acc, accd);
+    }
+  }
+  store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);
+}
+
+// Sums each problem's partial rows [first[p], first[p+1]) into out[p].
+__global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial, const int* first,
+                                                           double* out) {
+  __shared__ double red[256];
+  const int p = blockIdx.x / kMaxAcc, k = blockIdx.x % kMaxAcc;
+  double acc = 0.0;
+  for (int t = first[p] + threadIdx.x; t < first[p + 1]; t += 256) acc += partial[(long)t * kMaxAcc + k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(long)p * kMaxAcc + k] = red[0];
+}
+
+// out_v[i] = sum_{c <= i (strict: c < i)} M[i, c] v[c] per problem; one
+// thread per row, one workgroup per (problem, 256-row block).
+struct RowTask {
+  int prob, r0;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trmv_n_tasks(const RowTask* tasks, const ProbDesc* probs,
+                                                      const T* Mar, const double* v, double* out,
+                                                      int strict, int abs_mode) {
+  const RowTask t = tasks[blockIdx.x];
+  const ProbDesc pd = probs[t.prob];
+  const int i = t.r0 + threadIdx.x;
+  if (i >= pd.npad) return;
+  const T* M = Mar + pd.moff;
+  const double* vv = v + pd.voff;
+  const int c1 = strict ? i : i + 1;
+  double acc = 0.0;
+  for (int c = 0; c < c1; ++c) {
+    const double m = (double)M[i + (long)c * pd.npad];
+    acc += (abs_mode ? fabs(m) : m) * vv[c];
+  }
+  out[pd.voff + i] = acc;
+}
+
+// out_v[j] = sum_{i >= j (strict: i > j)} M[i, j] v[i]; one wave per column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trmv_t_tasks(const RowTask* tasks, const ProbDesc* probs,
+                                                      const T* Mar, const double* v, double* out,
+                                                      int strict, int abs_mode) {
+  const RowTask t = tasks[blockIdx.x];
+  const ProbDesc pd = probs[t.prob];
+  const int lane = threadIdx.x & 63;
+  const T* M = Mar + pd.moff;
+  const double* vv = v + pd.voff;
+  for (int jj = threadIdx.x >> 6; jj < 256; jj += 4) {
+    const int j = t.r0 + jj;
+    if (j >= pd.npad) break;
+    double acc = 0.0;
+    for (int i = (strict ? j + 1 : j) + lane; i < pd.npad; i += 64) {
+      const double m = (double)M[i + (long)j * pd.npad];
+      acc += (abs_mode ? fabs(m) : m) * vv[i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) out[pd.voff + j] = acc;
+  }
+}
+
+// Per-problem scalars, one block each:
+// out[p*8 + 0] = y'alpha, 1 = sum log L_ii, 2 = sum (alpha^2 - Ainv_ii) (if Ainv).
+template <typename T>
+__global__ __launch_bounds__(256) void k_problem_scalars(const ProbDesc* probs, const double* y,
+                                                         const double* alpha, const double* logd,
+                                                         const T* Ainv, double* out) {
+  __shared__ double r0[256], r1[256], r2[256];
+  const ProbDesc pd = probs[blockIdx.x];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < pd.n; i += 256) {
+    const double al = alpha[pd.voff + i];
+    a += y[pd.voff + i] * al;
+    b += logd[pd.voff + i];
+    if (Ainv) c += al * al - (double)Ainv[pd.moff + (long)i * (pd.npad + 1)];
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  r2[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 8 + 0] = r0[0];
+    out[blockIdx.x * 8 + 1] = r1[0];
+    if (Ainv) out[blockIdx.x * 8 + 2] = r2[0];
+  }
+}
+
+// Refinement: r = y - (p1 + p2); out[p*8 + 4..7] = max|r|, max|alpha|,
+// max|y|, max row-abs-sum (q1 + q2, first pass).
+__global__ __launch_bounds__(256) void k_problem_residual(const ProbDesc* probs, const double* y,
+                                                          const double* p1, const double* p2,
+                                                          const double* alpha, const double* q1,
+                                                          const double* q2, double* r,
+                                                          double* out) {
+  __shared__ double m0[256], m1[256], m2[256], m3[256];
+  const ProbDesc pd = probs[blockIdx.x];
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int i = threadIdx.x; i < pd.npad; i += 256) {
+    const long k = pd.voff + i;
+    if (i >= pd.n) {
+      r[k] = 0.0;
+      continue;
+    }
+    const double ri = y[k] - (p1[k] + p2[k]);
+    r[k] = ri;
+    a0 = fmax(a0, fabs(ri));
+    a1 = fmax(a1, fabs(alpha[k]));
+    a2 = fmax(a2, fabs(y[k]));
+    if (q1) a3 = fmax(a3, q1[k] + q2[k]);
+  }
+  m0[threadIdx.x] = a0;
+  m1[threadIdx.x] = a1;
+  m2[threadIdx.x] = a2;
+  m3[threadIdx.x] = a3;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
+      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
+      m2[threadIdx.x] = fmax(m2[threadIdx.x], m2[threadIdx.x + s]);
+      m3[threadIdx.x] = fmax(m3[threadIdx.x], m3[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 8 + 4] = m0[0];
+    out[blockIdx.x * 8 + 5] = m1[0];
+    out[blockIdx.x * 8 + 6] = m2[0];
+    if (q1) out[blockIdx.x * 8 + 7] = m3[0];
+  }
+}
+
+// alpha += d on the problems whose flag is set.
+__global__ void k_masked_axpy(const ProbDesc* probs, const int* active, double* alpha,
+                              const double* d) {
+  const ProbDesc pd = probs[blockIdx.y];
+  if (!active[blockIdx.y]) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < pd.n) alpha[pd.voff + i] += d[pd.voff + i];
+}
+
+template <typename F>
+int guarded(gprx_error* err, F&& f) {
+  try {
+    return f();
+  } catch (const DeviceError& e) {
+    set_error(err, GPRX_DEVICE_ERROR, e.msg);
+    return GPRX_DEVICE_ERROR;
+  } catch (...) {
+    set_error(err, GPRX_DEVICE_ERROR, "unexpected failure");
+    return GPRX_DEVICE_ERROR;
+  }
+}
+
+struct HostProb {
+  HostProgram hp;
+  int n, d, npad, nt;
+  long xoff, voff, moff;
+  std::vector<double> theta;
+};
+
+template <typename V>
+void upload(DevBuf<V>& dst, const std::vector<V>& src, hipStream_t s) {
+  dst.alloc(std::max<size_t>(src.size(), 1));
+  if (!src.empty())
+    GPRX_HIP_CHECK(hipMemcpyAsync(dst.p, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice, s));
+}
+
+struct BatchModel : gprx_hip_model {
+  BatchModel() : gprx_hip_model(kBatch) {}
+  ~BatchModel() override {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  int K = 0, dp = 1, maxnt = 0, precision = GPRX_FP64;
+  long xtot = 0, vtot = 0, mtot = 0;
+  std::vector<HostProb> hp;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  DevBuf<double> xar, yar, logd, alpha, tv, r, dlt, p1, p2, q1, q2, K64, A64, P64, partial, accs, scal;
+  DevBuf<float> A32, P32;
+  DevBuf<ProbDesc> probs;
+  DevBuf<DevProgram> progs;
+  HostBuf<DevProgram> hprogs;
+  DevBuf<int> info, active, first;
+  HostBuf<int> hinfo, hactive;
+  HostBuf<double> hscal, hacc;
+  // task lists
+  struct Range {
+    int off = 0, count = 0;
+  };
+  DevBuf<TileTask> tasks;
+  // per step: panel (A, P -> A), row block (P, P -> P), trailing factor
+  // update (A, A -> A), trailing inverse update (A, P -> P)
+  std::vector<Range> step_pa, step_pb, step_ta, step_tb;
+  Range lauum;
+  DevBuf<long> d_aoff, d_xoff, d_ld, d_logoff;
+  DevBuf<int> d_infoidx;
+  std::vector<Range> step_diag;
+  DevBuf<FillTask> fill_tasks;
+  int n_fill = 0;
+  DevBuf<RowTask> row_tasks;
+  int n_rows = 0;
+
+  bool mixed() const { return precision == GPRX_MIXED; }
+
+  void build_tasks() {
+    std::vector<TileTask> tl;
+    std::vector<long> aoff, xoff, ld, logoff;
+    std::vector<int> infoidx;
+    auto push_range = [&](std::vector<TileTask>& v, Range& rg, std::vector<TileTask>&& add,
+                          bool sort_k) {
+      if (sort_k)
+        std::stable_sort(add.begin(), add.end(), [](const TileTask& a, const TileTask& b) { return a.K > b.K; });
+      rg.off = (int)v.size();
+      rg.count = (int)add.size();
+      v.insert(v.end(), add.begin(), add.end());
+    };
+    for (auto* v : {&step_pa, &step_pb, &step_ta, &step_tb}) v->assign(maxnt, Range{});
+    step_diag.assign(maxnt, Range{});
+    for (int k = 0; k < maxnt; ++k) {
+      step_diag[k].off = (int)aoff.size();
+      for (int p = 0; p < K; ++p) {
+        const HostProb& h = hp[p];
+        if (h.nt <= k) continue;
+        const long ldl = h.npad;
+        const long kk = (long)k * kTile;
+        aoff.push_back(h.moff + kk + kk * ldl);
+        xoff.push_back(h.moff + kk + kk * ldl);
+        ld.push_back(ldl);
+        logoff.push_back(h.voff + kk);
+        infoidx.push_back(p);
+      }
+      step_diag[k].count = (int)aoff.size() - step_diag[k].off;
+      std::vector<TileTask> pa, pb, ta, tb;
+      for (int p = 0; p < K; ++p) {
+        const HostProb& h = hp[p];
+        if (h.nt <= k) continue;
+        const long L = h.npad;
+        auto at = [&](int r, int c) { return h.moff + (long)r * kTile + (long)c * kTile * L; };
+        // L_ik = A_ik X_kk^T  (A arena; X_kk in the P arena)
+        for (int i = k + 1; i < h.nt; ++i)
+          pa.push_back(TileTask{at(i, k), at(k, k), at(i, k), (int)L, kTile, 1, 0, 0, 1.f, 0.f});
+        // P_kj <- X_kk P_kj (in place), j < k
+        for (int j = 0; j < k; ++j)
+          pb.push_back(TileTask{at(k, k), at(k, j), at(k, j), (int)L, kTile, 0, 0, 0, 1.f, 0.f});
+        // A_ij -= L_ik L_jk^T
+        for (int i = k + 1; i < h.nt; ++i)
+          for (int j = k + 1; j <= i; ++j)
+            ta.push_back(TileTask{at(i, k), at(j, k), at(i, j), (int)L, kTile, 1, 0, i == j, -1.f, 1.f});
+        // P_ij -= L_ik P_kj (j <= k; first touch of column k: beta 0)
+        for (int i = k + 1; i < h.nt; ++i)
+          for (int j = 0; j <= k; ++j)
+            tb.push_back(TileTask{at(i, k), at(k, j), at(i, j), (int)L, kTile, 0, 0, 0, -1.f,
+                                  j == k ? 0.f : 1.f});
+      }
+      push_range(tl, step_pa[k], std::move(pa), false);
+      push_range(tl, step_pb[k], std::move(pb), false);
+      push_range(tl, step_ta[k], std::move(ta), false);
+      push_range(tl, step_tb[k], std::move(tb), false);
+    }
+    // A^-1 (lower) = X^T X: C_ij = sum_{k >= i} X_ki^T X_kj
+    std::vector<TileTask> la;
+    for (int p = 0; p < K; ++p) {
+      const HostProb& h = hp[p];
+      const long L = h.npad;
+      for (int i = 0; i < h.nt; ++i)
+        for (int j = 0; j <= i; ++j) {
+          const long a = h.moff + (long)i * kTile + (long)i * kTile * L;  // X(i*128.., i*128..)
+          const long b = h.moff + (long)i * kTile + (long)j * kTile * L;  // X(i*128.., j*128..)
+          const long c = h.moff + (long)i * kTile + (long)j * kTile * L;
+          la.push_back(TileTask{a, b, c, (int)L, (h.nt - i) * kTile, 0, 1, i == j, 1.f, 0.f});
+        }
+    }
+    push_range(tl, lauum, std::move(la), true);
+    upload(tasks, tl, stream);
+    upload(d_aoff, aoff, stream);
+    upload(d_xoff, xoff, stream);
+    upload(d_ld, ld, stream);
+    upload(d_logoff, logoff, stream);
+    upload(d_infoidx, infoidx, stream);
+    // fill / contraction tiles (problem-major), row tasks
+    std::vector<FillTask> ft;
+    std::vector<int> fst(K + 1, 0);
+    std::vector<RowTask> rt;
+    for (int p = 0; p < K; ++p) {
+      fst[p] = (int)ft.size();
+      for (int i = 0; i < hp[p].nt; ++i)
+        for (int j = 0; j <= i; ++j) ft.push_back(FillTask{p, i, j});
+      for (int r0 = 0; r0 < hp[p].npad; r0 += 256) rt.push_back(RowTask{p, r0});
+    }
+    fst[K] = (int)ft.size();
+    n_fill = (int)ft.size();
+    n_rows = (int)rt.size();
+    upload(fill_tasks, ft, stream);
+    upload(row_tasks, rt, stream);
+    upload(first, fst, stream);
+  }
+
+  template <typename T>
+  void launch_tasks(const Range& rg, const T* Aar, const T* Bar, T* Car) {
+    if (rg.count == 0) return;
+    hipLaunchKernelGGL((k_gemm_tasks<T>), dim3(rg.count), dim3(256), 0, stream, tasks.p + rg.off,
+                       rg.count, Aar, Bar, Car);
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
+
+  template <typename T>
+  void trmv(const T* M, const double* v, double* outn, double* outt, int strict, int abs_mode) {
+    // outn = tril(M) v (incl diag unless strict) ; outt = tril(M)^T v
+    if (outn)
+      hipLaunchKernelGGL((k_trmv_n_tasks<T>), dim3(n_rows), dim3(256), 0, stream, row_tasks.p,
+                         probs.p, M, v, outn, strict, abs_mode);
+    if (outt)
+      hipLaunchKernelGGL((k_trmv_t_tasks<T>), dim3(n_rows), dim3(256), 0, stream, row_tasks.p,
+                         probs.p, M, v, outt, strict, abs_mode);
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
+
+  template <typename T>
+  void fill(T* Aar) {
+#define GPRX_FILL(DPV, DUAL)                                                                     \
+  hipLaunchKernelGGL((k_fill_tasks<DPV, T, DUAL>), dim3(n_fill), dim3(256), 0, stream,           \
+                     fill_tasks.p, probs.p, progs.p, xar.p, Aar, K64.p, info.p)
+    const bool dual = mixed();
+    switch (dp) {
+      case 1: if (dual) GPRX_FILL(1, true); else GPRX_FILL(1, false); break;
+      case 2: if (dual) GPRX_FILL(2, true); else GPRX_FILL(2, false); break;
+      case 4: if (dual) GPRX_FILL(4, true); else GPRX_FILL(4, false); break;
+      case 8: if (dual) GPRX_FILL(8, true); else GPRX_FILL(8, false); break;
+      case 16: if (dual) GPRX_FILL(16, true); else GPRX_FILL(16, false); break;
+      default: if (dual) GPRX_FILL(32, true); else GPRX_FILL(32, false); break;
+    }
+#undef GPRX_FILL
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
+
+  template <typename T>
+  void contract(const T* Ainv) {
+#define GPRX_CON(DPV)                                                                           \
+  hipLaunchKernelGGL((k_contract_tasks<DPV, T>), dim3(n_fill), dim3(256), 0, stream,            \
+                     fill_tasks.p, probs.p, progs.p, xar.p, Ainv, alpha.p, partial.p)
+    switch (dp) {
+      case 1: GPRX_CON(1); break;
+      case 2: GPRX_CON(2); break;
+      case 4: GPRX_CON(4); break;
+      case 8: GPRX_CON(8); break;
+      case 16: GPRX_CON(16); break;
+      default: GPRX_CON(32); break;
+    }
+#undef GPRX_CON
+    hipLaunchKernelGGL(k_reduce_by_problem, dim3(K * kMaxAcc), dim3(256), 0, stream, partial.p,
+                       first.p, accs.p);
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
+
+  // Factor + inverse of every problem (arenas of type T).
+  template <typename T>
+  void factor(T* Aar, T* Par) {
+    for (int k = 0; k < maxnt; ++k) {
+      const Range& dg = step_diag[k];
+      potrf_diag_list<T>(Aar, Par, d_aoff.p + dg.off, d_xoff.p + dg.off, d_ld.p + dg.off,
+                         d_logoff.p + dg.off, logd.p, info.p, d_infoidx.p + dg.off, dg.count, stream);
+      launch_tasks<T>(step_pa[k], Aar, Par, Aar);
+      launch_tasks<T>(step_pb[k], Par, Par, Par);
+      launch_tasks<T>(step_ta[k], Aar, Aar, Aar);
+      launch_tasks<T>(step_tb[k], Aar, Par, Par);
+    }
+  }
+
+
+
+  // The device part of one FP64 evaluation (capturable: kernels and
+  // device copies only).
+  void record_fp64() {
+    fill<double>(A64.p);
+    factor<double>(A64.p, P64.p);
+    trmv<double>(P64.p, yar.p, tv.p, nullptr, 0, 0);
+    trmv<double>(P64.p, tv.p, nullptr, alpha.p, 0, 0);
+    launch_tasks<double>(lauum, P64.p, P64.p, A64.p);
+    hipLaunchKernelGGL((k_problem_scalars<double>), dim3(K), dim3(256), 0, stream, probs.p, yar.p,
+                       alpha.p, logd.p, A64.p, scal.p);
+    contract<double>(A64.p);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precision,
+                          gprx_hip_model** out, gprx_error* err) {
+  if (out) *out = nullptr;
+  return guarded(err, [&]() -> int {
+    if (!problems || !out || k == 0) {
+      set_error(err, GPRX_EMPTY_INPUT, "no problems");
+      return GPRX_EMPTY_INPUT;
+    }
+    if (precision != GPRX_FP64 && precision != GPRX_MIXED) {
+      set_error(err, GPRX_UNSUPPORTED, "unknown precision");
+      return GPRX_UNSUPPORTED;
+    }
+    auto mo = std::make_unique<BatchModel>();
+    BatchModel& B = *mo;
+    B.K = (int)k;
+    B.precision = precision;
+    B.hp.resize(k);
+    int maxd = 1;
+    for (size_t p = 0; p < k; ++p) {
+      const gprx_problem& pr = problems[p];
+      if (!pr.x || !pr.y || pr.n == 0 || pr.d == 0) {
+        set_error(err, GPRX_EMPTY_INPUT, "empty problem " + std::to_string(p));
+        return GPRX_EMPTY_INPUT;
+      }
+      if (pr.d > (size_t)kMaxDim || pr.n > 65536) {
+        set_error(err, GPRX_UNSUPPORTED, "batched problems take d <= 32 and n <= 65536");
+        return GPRX_UNSUPPORTED;
+      }
+      if (!all_finite(pr.x, pr.n * pr.d) || !all_finite(pr.y, pr.n)) {
+        set_error(err, GPRX_NON_FINITE_INPUT, "non-finite data in problem " + std::to_string(p));
+        return GPRX_NON_FINITE_INPUT;
+      }
+      HostProb& h = B.hp[p];
+      std::string why;
+      int rc = parse_program(pr.kernel, (int)pr.d, &h.hp, &why);
+      if (rc != GPRX_OK) {
+        set_error(err, rc, why + " (problem " + std::to_string(p) + ")");
+        return rc;
+      }
+      if (!pr.log_theta) {
+        set_error(err, GPRX_LENGTH_MISMATCH, "missing parameters");
+        return GPRX_LENGTH_MISMATCH;
+      }
+      rc = exact_validate_theta(h.hp, pr.log_theta, pr.p, err);
+      if (rc != GPRX_OK) return rc;
+      h.n = (int)pr.n;
+      h.d = (int)pr.d;
+      h.npad = (int)round_up(pr.n, kTile);
+      h.nt = h.npad / kTile;
+      h.theta.assign(pr.log_theta, pr.log_theta + pr.p);
+      maxd = std::max(maxd, h.d);
+    }
+    B.dp = padded_dim(maxd);
+    for (auto& h : B.hp) {
+      h.xoff = B.xtot;
+      h.voff = B.vtot;
+      h.moff = B.mtot;
+      B.xtot += (long)h.npad * B.dp;
+      B.vtot += h.npad;
+      B.mtot += (long)h.npad * h.npad;
+      B.maxnt = std::max(B.maxnt, h.nt);
+    }
+    GPRX_HIP_CHECK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    for (auto& e : B.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
+    hipStream_t s = B.stream;
+    {
+      std::vector<double> xh(B.xtot, 0.0), yh(B.vtot, 0.0);
+      std::vector<ProbDesc> pdv(k);
+      for (size_t p = 0; p < k; ++p) {
+        const gprx_problem& pr = problems[p];
+        const HostProb& h = B.hp[p];
+        for (size_t q = 0; q < pr.d; ++q)
+          for (size_t i = 0; i < pr.n; ++i) xh[h.xoff + q * h.npad + i] = pr.x[q * pr.n + i];
+        for (size_t i = 0; i < pr.n; ++i) yh[h.voff + i] = pr.y[i];
+        pdv[p] = ProbDesc{h.xoff, h.voff, h.moff, h.n, h.npad};
+      }
+      upload(B.xar, xh, s);
+      upload(B.yar, yh, s);
+      upload(B.probs, pdv, s);
+      GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    for (DevBuf<double>* b : {&B.logd, &B.alpha, &B.tv, &B.r, &B.dlt, &B.p1, &B.p2, &B.q1, &B.q2}) {
+      b->alloc(B.vtot);
+      GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * B.vtot, s));
+    }
+    if (precision == GPRX_MIXED) {
+      B.A32.alloc(B.mtot);
+      B.P32.alloc(B.mtot);
+      B.K64.alloc(B.mtot);
+      GPRX_HIP_CHECK(hipMemsetAsync(B.A32.p, 0, sizeof(float) * B.mtot, s));
+      GPRX_HIP_CHECK(hipMemsetAsync(B.P32.p, 0, sizeof(float) * B.mtot, s));
+      GPRX_HIP_CHECK(hipMemsetAsync(B.K64.p, 0, sizeof(double) * B.mtot, s));
+    } else {
+      B.A64.alloc(B.mtot);
+      B.P64.alloc(B.mtot);
+      GPRX_HIP_CHECK(hipMemsetAsync(B.A64.p, 0, sizeof(double) * B.mtot, s));
+      GPRX_HIP_CHECK(hipMemsetAsync(B.P64.p, 0, sizeof(double) * B.mtot, s));
+    }
+    B.progs.alloc(k);
+    B.hprogs.alloc(k);
+    B.info.alloc(k);
+    B.active.alloc(k);
+    B.hinfo.alloc(k);
+    B.hactive.alloc(k);
+    B.scal.alloc(8 * k);
+    B.hscal.alloc(8 * k);
+    B.accs.alloc(k * kMaxAcc);
+    B.hacc.alloc(k * kMaxAcc);
+    B.build_tasks();
+    prepare_dense_kernels();
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    B.partial.alloc((size_t)B.n_fill * kMaxAcc);
+    *out = mo.release();
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* log_theta,
+                                  double* nlml, double* const* grad, gprx_error* errs) {
+  gprx_error* err0 = errs;
+  return guarded(err0, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kBatch || !log_theta || !nlml) {
+      set_error(err0, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* B = static_cast<BatchModel*>(model);
+    hipStream_t s = B->stream;
+    std::vector<int> ok(B->K, 1);
+    for (int p = 0; p < B->K; ++p) {
+      HostProb& h = B->hp[p];
+      gprx_error* e = errs ? errs + p : nullptr;
+      if (e) std::memset(e, 0, sizeof(*e));
+      if (exact_validate_theta(h.hp, log_theta[p], h.theta.size(), e) != GPRX_OK) {
+        ok[p] = 0;
+        make_dev_program(h.hp, h.theta.data(), std::exp(h.theta.back()), B->hprogs.p + p);
+      } else {
+        make_dev_program(h.hp, log_theta[p], std::exp(log_theta[p][h.hp.n_params]), B->hprogs.p + p);
+      }
+    }
+    GPRX_HIP_CHECK(hipMemcpyAsync(B->progs.p, B->hprogs.p, sizeof(DevProgram) * B->K, hipMemcpyHostToDevice, s));
+    GPRX_HIP_CHECK(hipMemsetAsync(B->info.p, 0, sizeof(int) * B->K, s));
+    GPRX_HIP_CHECK(hipEventRecord(B->ev[0], s));
+    if (B->precision == GPRX_FP64) {
+      if (!B->graph_exec) {
+        // Capture one evaluation; replays reuse it with new parameters.
+        GPRX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        B->record_fp64();
+        GPRX_HIP_CHECK(hipStreamEndCapture(s, &B->graph));
+        GPRX_HIP_CHECK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+      }
+      GPRX_HIP_CHECK(hipGraphLaunch(B->graph_exec, s));
+    } else {
+      // Mixed: f32 arenas, f64 diagonal blocks, alpha refined with f64 residuals.
+      B->fill<float>(B->A32.p);
+      B->factor<float>(B->A32.p, B->P32.p);
+      B->trmv<float>(B->P32.p, B->yar.p, B->tv.p, nullptr, 0, 0);
+      B->trmv<float>(B->P32.p, B->tv.p, nullptr, B->alpha.p, 0, 0);
+      // row abs sums of A (first residual pass uses them)
+      std::vector<double> ones(B->vtot, 1.0);
+      GPRX_HIP_CHECK(hipMemcpyAsync(B->dlt.p, ones.data(), sizeof(double) * B->vtot, hipMemcpyHostToDevice, s));
+      B->trmv<double>(B->K64.p, B->dlt.p, B->q1.p, nullptr, 0, 1);
+      B->trmv<double>(B->K64.p, B->dlt.p, nullptr, B->q2.p, 1, 1);
+      std::vector<int> act(B->K, 1), streak(B->K, 0), conv(B->K, 0);
+      std::vector<double> prev(B->K, -1.0);
+      for (int it = 0; it < 10; ++it) {
+        B->trmv<double>(B->K64.p, B->alpha.p, B->p1.p, nullptr, 0, 0);
+        B->trmv<double>(B->K64.p, B->alpha.p, nullptr, B->p2.p, 1, 0);
+        hipLaunchKernelGGL(k_problem_residual, dim3(B->K), dim3(256), 0, s, B->probs.p, B->yar.p,
+                           B->p1.p, B->p2.p, B->alpha.p, it == 0 ? B->q1.p : nullptr, B->q2.p,
+                           B->r.p, B->scal.p);
+        GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+        bool any = false;
+        for (int p = 0; p < B->K; ++p) {
+          if (!act[p]) continue;
+          const double* sc = B->hscal.p + 8 * p;
+          const double tol = 10.0 * B->hp[p].n * 2.220446049250313e-16;
+          const double denom = sc[7] * sc[5] + sc[6];
+          if (denom > 0.0 && sc[4] / denom < tol) {
+            act[p] = 0;
+            conv[p] = 1;
+            continue;
+          }
+          if (prev[p] >= 0.0) {
+            const double ratio = prev[p] > 0.0 ? sc[4] / prev[p] : 0.0;
+            if (ratio > 0.9) {
+              if (++streak[p] >= 2) {
+                act[p] = 0;
+                continue;
+              }
+            } else {
+              streak[p] = 0;
+            }
+          }
+          prev[p] = sc[4];
+          any = true;
+        }
+        if (!any) break;
+        for (int p = 0; p < B->K; ++p) B->hactive.p[p] = act[p];
+        GPRX_HIP_CHECK(hipMemcpyAsync(B->active.p, B->hactive.p, sizeof(int) * B->K, hipMemcpyHostToDevice, s));
+        B->trmv<float>(B->P32.p, B->r.p, B->tv.p, nullptr, 0, 0);
+        B->trmv<float>(B->P32.p, B->tv.p, nullptr, B->dlt.p, 0, 0);
+        hipLaunchKernelGGL(k_masked_axpy, dim3((65536 + 255) / 256, B->K), dim3(256), 0, s,
+                           B->probs.p, B->active.p, B->alpha.p, B->dlt.p);
+      }
+      for (int p = 0; p < B->K; ++p)
+        if (!conv[p] && ok[p]) {
+          ok[p] = 0;
+          if (errs)
+            set_error(errs + p, GPRX_CHOLESKY_FAILED, "mixed refinement did not converge (use FP64)");
+        }
+      B->launch_tasks<float>(B->lauum, B->P32.p, B->P32.p, B->A32.p);
+      hipLaunchKernelGGL((k_problem_scalars<float>), dim3(B->K), dim3(256), 0, s, B->probs.p,
+                         B->yar.p, B->alpha.p, B->logd.p, B->A32.p, B->scal.p);
+      B->contract<float>(B->A32.p);
+    }
+    GPRX_HIP_CHECK(hipEventRecord(B->ev[1], s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(B->hacc.p, B->accs.p, sizeof(double) * kMaxAcc * B->K, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(B->hinfo.p, B->info.p, sizeof(int) * B->K, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
+    B->last_eval_ms = ms;
+    for (int p = 0; p < B->K; ++p) {
+      HostProb& h = B->hp[p];
+      gprx_error* e = errs ? errs + p : nullptr;
+      const int P = h.hp.n_params;
+      if (!ok[p]) {
+        nlml[p] = NAN;
+        continue;
+      }
+      if (B->hinfo.p[p]) {
+        set_error(e, (B->hinfo.p[p] & 2) ? GPRX_NON_FINITE_KERNEL_VALUE : GPRX_CHOLESKY_FAILED,
+                  "factor failed", GPRX_STAGE_FIT, 0.0, (size_t)h.n);
+        nlml[p] = NAN;
+        continue;
+      }
+      const double* sc = B->hscal.p + 8 * p;
+      nlml[p] = 0.5 * sc[0] + sc[1] + 0.5 * h.n * kLog2Pi;
+      if (grad && grad[p]) {
+        acc_to_grad(h.hp, log_theta[p], B->hacc.p + (long)p * kMaxAcc, grad[p]);
+        for (int q = 0; q < P; ++q) grad[p][q] *= -0.5;
+        grad[p][P] = -0.5 * std::exp(log_theta[p][P]) * sc[2];
+      }
+      h.theta.assign(log_theta[p], log_theta[p] + P + 1);
+    }
+    return GPRX_OK;
+  });
+}
+
+}  // extern "C"

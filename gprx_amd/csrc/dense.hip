@@ -6,211 +6,39 @@
 // (f64) and v_mfma_f32_16x16x4_f32 (f32); both share one lane layout:
 //   srcA (16x4):  lane l holds X[l % 16][l / 16]
 //   srcB (4x16):  lane l holds Y[l / 16][l % 16]
-//   D (16x16):    lane l holds D[4 * (l / 16) + q][l % 16], q = 0..3
+//   D (16x16):    lane l holds D[r(l, q)][l % 16], q = 0..3, with
+//                 r = 4 * (l / 16) + q for f32 and r = l / 16 + 4 * q for f64
+//                 (measured on gfx950 with an in-bounds probe of both builtins)
 // The tile kernel feeds op(B) as srcA and op(A) as srcB, so D holds C^T and
 // a lane's four results sit in four columns of C at one row: consecutive
 // lanes store consecutive rows, which is the contiguous direction of a
 // column-major C.
 #include "dense.hpp"
+#include "tile_gemm.hpp"
 
 namespace gprx {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef double d2 __attribute__((ext_vector_type(2)));
-
-template <typename T>
-struct Acc;
-template <>
-struct Acc<double> {
-  using V = d4;
-};
-template <>
-struct Acc<float> {
-  using V = f4;
-};
-
-template <typename T>
-struct Vec;
-template <>
-struct Vec<double> {
-  using V = d2;
-};
-template <>
-struct Vec<float> {
-  using V = f4;
-};
-
-__device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-// Tile-kernel configuration: K step and LDS row stride (elements).  The
-// stride is 32 dwords past a multiple of 64 banks, so the four 16-lane groups
-// of a fragment read alternate bank halves.
-template <typename T>
-struct Cfg;
-template <>
-struct Cfg<double> {
-  static constexpr int BK = 16;
-  static constexpr int LS = 144;
-};
-template <>
-struct Cfg<float> {
-  static constexpr int BK = 32;
-  static constexpr int LS = 160;
-};
-
-// Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.
-__device__ __forceinline__ void tri_index(int bid, int& ti, int& tj) {
-  int t = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
-  while ((t + 1) * (t + 2) / 2 <= bid) ++t;
-  while (t * (t + 1) / 2 > bid) --t;
-  ti = t;
-  tj = bid - t * (t + 1) / 2;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Give every XCD one contiguous run of logical tiles so neighbouring
-// tiles, which share A row-panels and B column-panels, share an L2.
-__device__ __forceinline__ int xcd_remap(int bid, int total) {
-  constexpr int kXcd = 8;
-  if (total < kXcd) return bid;
-  const int q = total / kXcd, r = total % kXcd;
-  const int x = bid % kXcd, k = bid / kXcd;
-  return x * q + (x < r ? x : r) + k;
-}
-
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g) {
-  constexpr int BK = Cfg<T>::BK;
-  constexpr int LS = Cfg<T>::LS;
-  constexpr int VEC = 16 / sizeof(T);
-  using V = typename Vec<T>::V;
   using AV = typename Acc<T>::V;
-  __shared__ __attribute__((aligned(16))) T sA[2][BK * LS];
-  __shared__ __attribute__((aligned(16))) T sB[2][BK * LS];
-
+  __shared__ TileSmem<T> sm;
   const int z = blockIdx.y;
   const T* __restrict__ A = g.A + (long)z * g.strideA;
   const T* __restrict__ B = g.B + (long)z * g.strideB;
   T* __restrict__ C = g.C + (long)z * g.strideC;
-
-  const int mt = g.M / kTile, nt = g.N / kTile;
-  const int total = g.lower ? mt * (mt + 1) / 2 : mt * nt;
-  int bid = xcd_remap(blockIdx.x, total);
-  int ti, tj;
-  if (g.lower) {
-    if (g.order & 1) bid = total - 1 - bid;
-    tri_index(bid, ti, tj);
-  } else {
-    if (g.order & 1) bid = total - 1 - bid;
-    if (g.order < 2) {
-      ti = bid % mt;
-      tj = bid / mt;
-    } else {
-      tj = bid % nt;
-      ti = bid / nt;
-    }
-  }
-  int kb = 0, ke = g.K;
-  if (g.krange & kKloRow) kb = max(kb, ti * kTile);
-  if (g.krange & kKloCol) kb = max(kb, tj * kTile);
-  if (g.krange & kKhiRow) ke = min(ke, (ti + 1) * kTile);
-  if (g.krange & kKhiCol) ke = min(ke, (tj + 1) * kTile);
-  const int nk = ke > kb ? (ke - kb) / BK : 0;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const TileCoord tc = tile_coord(g.M, g.N, g.K, g.lower, g.order, g.krange);
+  const int ti = tc.ti, tj = tc.tj;
+  const int nk = tc.ke > tc.kb ? (tc.ke - tc.kb) / Cfg<T>::BK : 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-
   AV acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
-
-  // Global tile origins.
   const T* Ablk = TA ? A + (long)ti * kTile * g.lda : A + (long)ti * kTile;
   const T* Bblk = TB ? B + (long)tj * kTile : B + (long)tj * kTile * g.ldb;
-  V ra[4], rb[4];
-
-  // A "row-contiguous" operand has element (mn, k) at P[mn + k * ld]; a
-  // "k-contiguous" one at P[k + mn * ld].
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int e = tid + 256 * v;
-      if (!TA) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        ra[v] = *reinterpret_cast<const V*>(Ablk + mn + (long)(k0 + k) * g.lda);
-      } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-        ra[v] = *reinterpret_cast<const V*>(Ablk + (k0 + k) + (long)mn * g.lda);
-      }
-      if (TB) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        rb[v] = *reinterpret_cast<const V*>(Bblk + mn + (long)(k0 + k) * g.ldb);
-      } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-        rb[v] = *reinterpret_cast<const V*>(Bblk + (k0 + k) + (long)mn * g.ldb);
-      }
-    }
-  };
-  auto lstore = [&](int buf) {
-    T* a = sA[buf];
-    T* b = sB[buf];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int e = tid + 256 * v;
-      if (!TA) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        *reinterpret_cast<V*>(a + k * LS + mn) = ra[v];
-      } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) a[(k + c) * LS + mn] = ra[v][c];
-      }
-      if (TB) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        *reinterpret_cast<V*>(b + k * LS + mn) = rb[v];
-      } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) b[(k + c) * LS + mn] = rb[v][c];
-      }
-    }
-  };
-
-  if (nk > 0) {
-    gload(kb);
-    lstore(0);
-  }
-  __syncthreads();
-  for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
-    if (it + 1 < nk) gload(kb + (it + 1) * BK);
-    const T* a = sA[cur];
-    const T* b = sB[cur];
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T af[4], bf[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = a[kr * LS + wm * 64 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = b[kr * LS + wn * 64 + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
-    }
-    if (it + 1 < nk) lstore(cur ^ 1);
-    __syncthreads();
-  }
+  tile_mainloop<T, TA, TB>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc);
 
   const bool diag_tile = g.lower && ti == tj;
 #pragma unroll
@@ -220,7 +48,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int r = wm * 64 + i * 16 + (lane & 15);
-        const int c = wn * 64 + j * 16 + 4 * (lane >> 4) + q;
+        const int c = wn * 64 + j * 16 + drow<T>(lane, q);
         if (diag_tile && r < c) continue;
         T* p = C + (long)(tj * kTile + c) * g.ldc + ti * kTile + r;
         double v = g.alpha * (double)acc[i][j][q];
@@ -358,7 +186,7 @@ __device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict
         acc = mfma16(a, b, acc);
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) s[(c0 + (lane & 15)) * DS + bi * 16 + 4 * (lane >> 4) + q] = acc[q];
+      for (int q = 0; q < 4; ++q) s[(c0 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[q];
     }
     __syncthreads();
     // (c) trailing update
@@ -378,7 +206,7 @@ __device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = bi * 16 + 4 * (lane >> 4) + q;
+        const int row = bi * 16 + drow<double>(lane, q);
         const int col = bj * 16 + (lane & 15);
         if (bi != bj || row >= col) s[col * DS + row] -= acc[q];
       }
@@ -400,7 +228,7 @@ __device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict
       acc = mfma16(a, b, acc);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s[(kb * 16 + (lane & 15)) * DS + bi * 16 + 4 * (lane >> 4) + q] = acc[q];
+    for (int q = 0; q < 4; ++q) s[(kb * 16 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[q];
   }
   __syncthreads();
   // X_{bi,bj} = -sum_{k=bj}^{bi-1} Y_{bi,k} X_{k,bj}, by distance bi - bj.
@@ -419,7 +247,7 @@ __device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = bi * 16 + 4 * (lane >> 4) + q;
+        const int r = bi * 16 + drow<double>(lane, q);
         const int c = bj * 16 + (lane & 15);
         s[r * DS + c] = -acc[q];
       }
@@ -471,6 +299,11 @@ static void set_diag_lds_once() {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds));
     done = true;
   }
+}
+
+void prepare_dense_kernels() {
+  set_diag_lds_once<double>();
+  set_diag_lds_once<float>();
 }
 
 template <typename T>

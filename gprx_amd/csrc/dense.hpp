@@ -77,6 +77,9 @@ void col_sumsq(const T* M, long ld, int rows, int cols, double* out, hipStream_t
 template <typename T>
 void zero(T* p, size_t count, hipStream_t stream);
 
+// Sets kernel attributes (large dynamic LDS) once, outside any graph capture.
+void prepare_dense_kernels();
+
 // MFMA layout probe: C(16x16) = A(16x4) B(4x16) through one
 // v_mfma_f64_16x16x4_f64 with the lane layout the kernels assume.
 void mfma_probe(const double* A, const double* B, double* C, hipStream_t stream);

@@ -71,3 +71,25 @@ void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdi
                     hipStream_t s);
 
 }  // namespace gprx
+
+namespace gprx {
+// Dense m x m helpers (f64, column-major, full storage).
+// out = a * A + b * B over rows x cols (out may alias A or B).
+void mat_combine(double* out, long ldo, int rows, int cols, double a, const double* A, long lda,
+                 double b, const double* B, long ldb, hipStream_t s);
+// Copies the strict lower triangle onto the upper one.
+void symmetrize_lower(double* M, long ld, int n, hipStream_t s);
+// out[0] = sum_ij A_ij B_ij over rows x cols (one block, fixed order).
+void frob_dot(const double* A, long lda, const double* B, long ldb, int rows, int cols, double* out,
+              hipStream_t s);
+// W += alpha * u v^T over rows x cols.
+void outer_add(double* W, long ld, int rows, int cols, double alpha, const double* u,
+               const double* v, hipStream_t s);
+// out_i = a * y_i + b * v_i for i < n, zero for n <= i < npad.
+void vec_combine(double* out, int n, int npad, double a, const double* y, double b, const double* v,
+                 hipStream_t s);
+// Sums `parts` lower-triangle partial matrices (storage T, each ld x ld,
+// consecutive) into out (f64), fixed order.
+template <typename T>
+void sum_partials_lower(const T* parts, int nparts, long ld, int n, double* out, hipStream_t s);
+}  // namespace gprx

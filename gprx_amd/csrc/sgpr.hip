@@ -1,0 +1,558 @@
+// Sparse VFE GPR (Sgpr) on one GPU.
+//
+// gprx forms A = L^-1 K_mn, B = A A^T + s2 I and an adjoint over A
+// (src/sgpr/factor/vfe.rs:191-369, derivatives.rs:94-201).  This backend
+// evaluates the same bound and gradient through m x m matrices and one pass
+// of MFMA products over U = K_mn (m x n):
+//   S = U U^T, b = U y                       (split-K SYRK on MFMA, GEMV)
+//   M = s2 Kt + S, Kt = K_mm + j I          (so B = L^-1 M L^-T)
+//   c = M^-1 b
+//   F = 1/2 [n log 2pi + (n - m) log s2 + log|M| - log|Kt| + (|y|^2 - b'c)/s2]
+//       + (sum k_ii - <Kt^-1, S>) / (2 s2)
+// and, with P = M^-1 - Kt^-1 / s2 and r = y - U^T c,
+//   W_mm = 1/2 [s2 M^-1 - Kt^-1 + c c^T + Kt^-1 S Kt^-1 / s2]
+//   W_mn = P U - c r^T / s2                 (fused: MFMA tile + contraction)
+//   dF/dtheta = <W_mm, dK_mm> + <W_mn, dK_mn> + 1/(2 s2) sum dk_ii
+//   dF/dlog s2 = (n - m)/2 + s2/2 <M^-1, Kt> - quad/2 + c'Kt c/2 - trace
+// The m x m algebra and both Cholesky factors are f64; U, S and P U run in
+// f32 on MFMA in the mixed form.
+#include <cstring>
+#include <memory>
+
+#include "kernel_eval.hpp"
+#include "model.hpp"
+#include "tile_gemm.hpp"
+
+namespace gprx {
+
+int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err);
+void exact_upload_points(DevBuf<double>& dst, const double* src, int n, int d, long rows_pad, int dp,
+                         hipStream_t s);
+
+// partial[tile] = contraction of W = (P U)_tile - crs c_i r_j against
+// dK(z_i, x_j), for the (M/128) x (N/128) tiles of P U (P: M x K, U: K x N).
+template <typename T, int DP>
+__global__ __launch_bounds__(256) void k_pu_contract(
+    const DevProgram* __restrict__ Pg, const T* __restrict__ P, long ldp, const T* __restrict__ U,
+    long ldu, int M, int N, int K, const double* __restrict__ z, long ldz, int m,
+    const double* __restrict__ x, long ldx, int n, const double* __restrict__ cv,
+    const double* __restrict__ rv, double crs, double* __restrict__ partial) {
+  using AV = typename Acc<T>::V;
+  const DevProgram& Pr = *Pg;
+  __shared__ TileSmem<T> sm;
+  __shared__ double zr[DP][kTile];
+  __shared__ double xc[DP][kTile];
+  __shared__ double cs[kTile], rs[kTile];
+  __shared__ double red[4 * kMaxAcc];
+  const int mt = M / kTile;
+  const int bid = xcd_remap(blockIdx.x, mt * (N / kTile));
+  const int ti = bid % mt, tj = bid / mt;
+  AV acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
+  tile_mainloop<T, false, false>(P + (long)ti * kTile, ldp, U + (long)tj * kTile * ldu, ldu, 0,
+                                 K / Cfg<T>::BK, sm, acc);
+  load_points<DP>(z, ldz, ti * kTile, zr);
+  load_points<DP>(x, ldx, tj * kTile, xc);
+  if (threadIdx.x < kTile) {
+    cs[threadIdx.x] = cv[ti * kTile + threadIdx.x];
+    rs[threadIdx.x] = rv[tj * kTile + threadIdx.x];
+  }
+  double a2[kMaxLeaves][kAccPerLeaf];
+  double ad[DP];
+#pragma unroll
+  for (int l = 0; l < kMaxLeaves; ++l) a2[l][0] = a2[l][1] = 0.0;
+#pragma unroll
+  for (int q = 0; q < DP; ++q) ad[q] = 0.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // The GEMM's LDS is free after the main loop: stage one 16-row strip of
+  // each wave's 64 x 64 result at a time as f64, [wave][col][row].
+  double* strip = reinterpret_cast<double*>(&sm);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        strip[wave * 1024 + (j * 16 + drow<T>(lane, q)) * 16 + (lane & 15)] = (double)acc[i][j][q];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4096; e += 256) {
+      const int w = e >> 10, rem = e & 1023, cc = rem >> 4, rr = rem & 15;
+      const int r = (w & 1) * 64 + i * 16 + rr;
+      const int c = (w >> 1) * 64 + cc;
+      if (ti * kTile + r >= m || tj * kTile + c >= n) continue;
+      const double wv = strip[e] - crs * cs[r] * rs[c];
+      double dx[DP];
+#pragma unroll
+      for (int q = 0; q < DP; ++q) dx[q] = zr[q][r] - xc[q][c];
+      Leaves L;
+      eval_entry<DP, true>(Pr, dx, L);
+      accumulate_entry<DP>(Pr, dx, L, wv, a2, ad);
+    }
+  }
+  store_partials<DP>(a2, ad, red, partial + (long)blockIdx.x * kMaxAcc);
+}
+
+template <typename T>
+void pu_contract(int dp, const DevProgram* prog, const T* P, long ldp, const T* U, long ldu, int M,
+                 int N, int K, DevPoints z, DevPoints x, const double* c, const double* r,
+                 double crs, double* partial, hipStream_t s) {
+  const int tiles = (M / kTile) * (N / kTile);
+#define GPRX_PU(DPV)                                                                          \
+  hipLaunchKernelGGL((k_pu_contract<T, DPV>), dim3(tiles), dim3(256), 0, s, prog, P, ldp, U, ldu, \
+                     M, N, K, z.p, z.ld, z.rows, x.p, x.ld, x.rows, c, r, crs, partial)
+  switch (dp) {
+    case 1: GPRX_PU(1); break;
+    case 2: GPRX_PU(2); break;
+    case 4: GPRX_PU(4); break;
+    case 8: GPRX_PU(8); break;
+    case 16: GPRX_PU(16); break;
+    case 32: GPRX_PU(32); break;
+    default: throw DeviceError("unsupported padded dimension");
+  }
+#undef GPRX_PU
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+template void pu_contract<double>(int, const DevProgram*, const double*, long, const double*, long,
+                                  int, int, int, DevPoints, DevPoints, const double*, const double*,
+                                  double, double*, hipStream_t);
+template void pu_contract<float>(int, const DevProgram*, const float*, long, const float*, long, int,
+                                 int, int, DevPoints, DevPoints, const double*, const double*,
+                                 double, double*, hipStream_t);
+
+}  // namespace gprx
+
+using namespace gprx;
+
+namespace {
+
+constexpr double kLog2Pi = 1.8378770664093453;
+
+template <typename F>
+int guarded(gprx_error* err, F&& f) {
+  try {
+    return f();
+  } catch (const DeviceError& e) {
+    set_error(err, GPRX_DEVICE_ERROR, e.msg);
+    return GPRX_DEVICE_ERROR;
+  } catch (...) {
+    set_error(err, GPRX_DEVICE_ERROR, "unexpected failure");
+    return GPRX_DEVICE_ERROR;
+  }
+}
+
+struct SgprModel : gprx_hip_model {
+  SgprModel() : gprx_hip_model(kSgpr) {}
+  ~SgprModel() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  HostProgram hp;
+  int n = 0, m = 0, d = 0, dp = 0, mt = 0, nchunks = 1;
+  long npad = 0, mpad = 0, ldt = 0, kc = 0;
+  int precision = GPRX_FP64;
+  std::vector<double> theta;
+  double ynorm2 = 0.0, kmm_jitter = 0.0;
+  bool factored = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  DevBuf<double> x, y, z;
+  DevBuf<DevProgram> prog;
+  HostBuf<DevProgram> hprog;
+  DevBuf<double> Kmm, KA, Xmm, Tm, S, MA, XM, G1, G2, Wmm, P64;
+  DevBuf<double> U64, Sp64;
+  DevBuf<float> U32, Sp32, P32;
+  DevBuf<double> logd_mm, logd_M, b, c, tv, r, ur, kd, gpart, partials, partials2, partials3, accs,
+      accs2, scal;
+  DevBuf<int> info;
+  HostBuf<double> hscal, hacc;
+  HostBuf<int> hinfo;
+
+  DevPoints zp() const { return DevPoints{z.p, mpad, m}; }
+  DevPoints xp() const { return DevPoints{x.p, npad, n}; }
+  bool mixed() const { return precision == GPRX_MIXED; }
+
+  void upload_program(const double* th, double jitter) {
+    make_dev_program(hp, th, jitter, hprog.p);
+    GPRX_HIP_CHECK(hipMemcpyAsync(prog.p, hprog.p, sizeof(DevProgram), hipMemcpyHostToDevice, stream));
+  }
+
+  int sync_info() {
+    GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    GPRX_HIP_CHECK(hipStreamSynchronize(stream));
+    return *hinfo.p;
+  }
+
+  // K_mm + j I, its factor and inverse.  Returns info bits.
+  int factor_kmm(const double* th, double j) {
+    hipStream_t s = stream;
+    upload_program(th, j);
+    GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
+    fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, s);
+    GPRX_HIP_CHECK(hipMemcpyAsync(Kmm.p, KA.p, sizeof(double) * mpad * mpad, hipMemcpyDeviceToDevice, s));
+    factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd_mm.p, info.p, s);
+    return sync_info();
+  }
+
+  template <typename T>
+  void data_pass(T* U, T* Sp) {
+    hipStream_t s = stream;
+    fill_cross<T>(dp, prog.p, zp(), xp(), U, mpad, s);
+    GemmArgs<T> g{};
+    g.A = U;
+    g.lda = mpad;
+    g.B = U;
+    g.ldb = mpad;
+    g.C = Sp;
+    g.ldc = mpad;
+    g.M = g.N = (int)mpad;
+    g.K = (int)kc;
+    g.alpha = 1.0;
+    g.beta = 0.0;
+    g.lower = 1;
+    g.batch = nchunks;
+    g.strideA = g.strideB = kc * mpad;
+    g.strideC = mpad * mpad;
+    gemm<T>(false, true, g, s);
+    sum_partials_lower<T>(Sp, nchunks, mpad, (int)mpad, S.p, s);
+    gemv_n<T>(U, mpad, (int)mpad, (int)npad, y.p, b.p, gpart.p, 0, 0, 0, s);
+  }
+
+  template <typename T>
+  void resid_pass(const T* U) {
+    gemv_t<T>(U, mpad, (int)mpad, (int)npad, c.p, ur.p, 0, 0, 0, 0, stream);
+    vec_combine(r.p, n, (int)npad, 1.0, y.p, -1.0, ur.p, stream);
+  }
+
+  int evaluate(const double* th, bool want_grad, double* value, double* grad, gprx_error* err) {
+    hipStream_t s = stream;
+    const int P = hp.n_params;
+    const double s2 = std::exp(th[P]);
+    // K_mm with the sparse default adaptive jitter (src/policy.rs:222-226).
+    double j = 0.0, next = 1e-8;
+    int remaining = 5;
+    for (;;) {
+      const int bits = factor_kmm(th, j);
+      if (bits == 0) break;
+      factored = false;
+      if (bits & 2) {
+        set_error(err, GPRX_NON_FINITE_KERNEL_VALUE, "K_mm holds a non-finite value");
+        return GPRX_NON_FINITE_KERNEL_VALUE;
+      }
+      if (remaining == 0 || next > 1e-3) {
+        set_error(err, GPRX_CHOLESKY_FAILED, "K_mm Cholesky failed", GPRX_STAGE_FIT, j, (size_t)m);
+        return GPRX_CHOLESKY_FAILED;
+      }
+      --remaining;
+      j = next;
+      next *= 10.0;
+    }
+    kmm_jitter = j;
+    GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
+    if (mixed())
+      data_pass<float>(U32.p, Sp32.p);
+    else
+      data_pass<double>(U64.p, Sp64.p);
+    // M = s2 Kt + S (lower), factor (no jitter retries: src/sgpr/factor/vfe.rs:244-250).
+    mat_combine(MA.p, mpad, (int)mpad, (int)mpad, s2, Kmm.p, mpad, 1.0, S.p, mpad, s);
+    GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
+    factor_inverse<double>(MA.p, XM.p, Tm.p, mpad, ldt, mt, logd_M.p, info.p, s);
+    gemv_n<double>(XM.p, mpad, (int)mpad, (int)mpad, b.p, tv.p, gpart.p, 1, 0, 0, s);
+    gemv_t<double>(XM.p, mpad, (int)mpad, (int)mpad, tv.p, c.p, 1, 0, 0, 0, s);
+    // K_t^-1 into KA, M^-1 into MA (both consumed by the factors), full storage.
+    gemm_call<double>(true, false, Xmm.p, mpad, Xmm.p, mpad, KA.p, mpad, (int)mpad, (int)mpad,
+                      (int)mpad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
+    gemm_call<double>(true, false, XM.p, mpad, XM.p, mpad, MA.p, mpad, (int)mpad, (int)mpad,
+                      (int)mpad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
+    symmetrize_lower(KA.p, mpad, (int)mpad, s);
+    symmetrize_lower(MA.p, mpad, (int)mpad, s);
+    symmetrize_lower(S.p, mpad, (int)mpad, s);
+    symmetrize_lower(Kmm.p, mpad, (int)mpad, s);
+    // scalars: [0] b'c, [1] sum log L_M, [2] sum log L_mm (dot_and_sum pairs),
+    // [4] <Kt^-1, S>, [5] <M^-1, Kt>, [6] c'Kt c, [7] k(x, x)
+    dot_and_sum(b.p, c.p, logd_M.p, m, scal.p, s);
+    dot_and_sum(b.p, c.p, logd_mm.p, m, scal.p + 2, s);
+    frob_dot(KA.p, mpad, S.p, mpad, m, m, scal.p + 4, s);
+    frob_dot(MA.p, mpad, Kmm.p, mpad, m, m, scal.p + 5, s);
+    gemv_n<double>(Kmm.p, mpad, (int)mpad, (int)mpad, c.p, tv.p, gpart.p, 0, 0, 0, s);
+    dot_and_sum(c.p, tv.p, c.p, m, scal.p + 8, s);
+    fill_diag(dp, prog.p, DevPoints{z.p, mpad, 1}, scal.p + 7, s);
+    if (want_grad) {
+      // W_mm = 1/2 [s2 M^-1 - Kt^-1 + c c^T + Kt^-1 S Kt^-1 / s2]
+      gemm_call<double>(false, false, KA.p, mpad, S.p, mpad, G1.p, mpad, (int)mpad, (int)mpad,
+                        (int)mpad, 1.0, 0.0, 0, 0, kOrderRowFast, s);
+      gemm_call<double>(false, false, G1.p, mpad, KA.p, mpad, G2.p, mpad, (int)mpad, (int)mpad,
+                        (int)mpad, 1.0, 0.0, 0, 0, kOrderRowFast, s);
+      mat_combine(Wmm.p, mpad, (int)mpad, (int)mpad, 0.5 * s2, MA.p, mpad, -0.5, KA.p, mpad, s);
+      mat_combine(Wmm.p, mpad, (int)mpad, (int)mpad, 1.0, Wmm.p, mpad, 0.5 / s2, G2.p, mpad, s);
+      outer_add(Wmm.p, mpad, (int)mpad, (int)mpad, 0.5, c.p, c.p, s);
+      // P = M^-1 - Kt^-1 / s2
+      mat_combine(P64.p, mpad, (int)mpad, (int)mpad, 1.0, MA.p, mpad, -1.0 / s2, KA.p, mpad, s);
+      const int tiles1 = (int)(mpad / kTile) * (int)(npad / kTile);
+      if (mixed()) {
+        convert<float, double>(P32.p, P64.p, (size_t)mpad * mpad, s);
+        resid_pass<float>(U32.p);
+        pu_contract<float>(dp, prog.p, P32.p, mpad, U32.p, mpad, (int)mpad, (int)npad, (int)mpad,
+                           zp(), xp(), c.p, r.p, 1.0 / s2, partials.p, s);
+      } else {
+        resid_pass<double>(U64.p);
+        pu_contract<double>(dp, prog.p, P64.p, mpad, U64.p, mpad, (int)mpad, (int)npad, (int)mpad,
+                            zp(), xp(), c.p, r.p, 1.0 / s2, partials.p, s);
+      }
+      reduce_partials(partials.p, tiles1, accs.p, s);
+      contract_cross<double>(dp, prog.p, zp(), zp(), Wmm.p, mpad, partials2.p, s);
+      reduce_partials(partials2.p, cross_tiles(mpad, mpad), accs.p + kMaxAcc, s);
+      contract_diag(dp, prog.p, xp(), 0.5 / s2, accs.p + 2 * kMaxAcc, s);
+    }
+    GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, sizeof(double) * 10, hipMemcpyDeviceToHost, s));
+    if (want_grad)
+      GPRX_HIP_CHECK(hipMemcpyAsync(hacc.p, accs.p, sizeof(double) * 3 * kMaxAcc, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    last_eval_ms = ms;
+    if (*hinfo.p) {
+      factored = false;
+      set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky of s2 K_mm + K_mn K_nm failed", GPRX_STAGE_FIT, 0.0,
+                (size_t)m);
+      return GPRX_CHOLESKY_FAILED;
+    }
+    const double* sc = hscal.p;
+    const double bc = sc[0], logdet_M = 2.0 * sc[1], logdet_K = 2.0 * sc[3];
+    const double trKS = sc[4], trMK = sc[5], kxx = sc[7], cKc = sc[8];
+    const double kdiag_sum = (double)n * kxx;
+    const double quad = (ynorm2 - bc) / s2;
+    const double trace = (kdiag_sum - trKS) / (2.0 * s2);
+    const double nm = (double)n - (double)m;
+    *value = 0.5 * ((double)n * kLog2Pi + nm * std::log(s2) + logdet_M - logdet_K + quad) + trace;
+    if (!std::isfinite(*value)) {
+      factored = false;
+      set_error(err, GPRX_NON_FINITE_KERNEL_VALUE, "non-finite bound");
+      return GPRX_NON_FINITE_KERNEL_VALUE;
+    }
+    if (want_grad) {
+      double acc[kMaxAcc];
+      for (int k = 0; k < kMaxAcc; ++k) acc[k] = hacc.p[k] + hacc.p[kMaxAcc + k] + hacc.p[2 * kMaxAcc + k];
+      acc_to_grad(hp, th, acc, grad);
+      grad[P] = 0.5 * nm + 0.5 * s2 * trMK - 0.5 * quad + 0.5 * cKc - trace;
+    }
+    factored = true;
+    return GPRX_OK;
+  }
+
+  int predict(const double* xs, size_t q, int32_t kind, double* mean, double* var, gprx_error* err) {
+    if (!factored) {
+      double v;
+      int rc = evaluate(theta.data(), false, &v, nullptr, err);
+      if (rc != GPRX_OK) {
+        if (err) err->stage = GPRX_STAGE_PREDICT;
+        return rc;
+      }
+    } else {
+      upload_program(theta.data(), kmm_jitter);
+    }
+    hipStream_t s = stream;
+    const long qpad = (long)round_up(q, kTile);
+    DevBuf<double> xq, Ks, V1, V2, mu, s1, s2v, kss;
+    exact_upload_points(xq, xs, (int)q, d, qpad, dp, s);
+    DevPoints pq{xq.p, qpad, (int)q};
+    Ks.alloc((size_t)mpad * qpad);
+    V1.alloc((size_t)mpad * qpad);
+    V2.alloc((size_t)mpad * qpad);
+    mu.alloc(qpad);
+    s1.alloc(qpad);
+    s2v.alloc(qpad);
+    kss.alloc(qpad);
+    fill_cross<double>(dp, prog.p, zp(), pq, Ks.p, mpad, s);
+    gemv_t<double>(Ks.p, mpad, (int)mpad, (int)qpad, c.p, mu.p, 0, 0, 0, 0, s);
+    gemm_call<double>(false, false, Xmm.p, mpad, Ks.p, mpad, V1.p, mpad, (int)mpad, (int)qpad,
+                      (int)mpad, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s);
+    gemm_call<double>(false, false, XM.p, mpad, Ks.p, mpad, V2.p, mpad, (int)mpad, (int)qpad,
+                      (int)mpad, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s);
+    col_sumsq<double>(V1.p, mpad, (int)mpad, (int)qpad, s1.p, s);
+    col_sumsq<double>(V2.p, mpad, (int)mpad, (int)qpad, s2v.p, s);
+    fill_diag(dp, prog.p, pq, kss.p, s);
+    std::vector<double> h1(q), h2(q), hk(q);
+    GPRX_HIP_CHECK(hipMemcpyAsync(mean, mu.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(h1.data(), s1.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(h2.data(), s2v.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hk.data(), kss.p, sizeof(double) * q, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    if (var) {
+      const double noise = std::exp(theta[hp.n_params]);
+      for (size_t k = 0; k < q; ++k) {
+        double lat = hk[k] - h1[k] + noise * h2[k];
+        if (lat < 0.0) lat = 0.0;
+        var[k] = kind == GPRX_VARIANCE_LATENT ? lat : lat + noise;
+      }
+    }
+    return GPRX_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, const double* z,
+                         size_t m, const gprx_kernel_program* kernel, const double* log_theta,
+                         size_t p, int32_t precision, gprx_hip_model** out, gprx_error* err) {
+  if (out) *out = nullptr;
+  return guarded(err, [&]() -> int {
+    if (!x || !y || !z || !out || n == 0 || d == 0 || m == 0) {
+      set_error(err, GPRX_EMPTY_INPUT, "empty data or inducing set");
+      return GPRX_EMPTY_INPUT;
+    }
+    if (d > (size_t)kMaxDim) {
+      set_error(err, GPRX_UNSUPPORTED, "input dimension above 32");
+      return GPRX_UNSUPPORTED;
+    }
+    if (m > 16384 || n > ((size_t)1 << 26)) {
+      set_error(err, GPRX_UNSUPPORTED, "sparse sizes above m = 16384 or n = 2^26");
+      return GPRX_UNSUPPORTED;
+    }
+    if (!all_finite(x, n * d) || !all_finite(y, n) || !all_finite(z, m * d)) {
+      set_error(err, GPRX_NON_FINITE_INPUT, "non-finite data");
+      return GPRX_NON_FINITE_INPUT;
+    }
+    if (precision != GPRX_FP64 && precision != GPRX_MIXED) {
+      set_error(err, GPRX_UNSUPPORTED, "unknown precision");
+      return GPRX_UNSUPPORTED;
+    }
+    auto mo = std::make_unique<SgprModel>();
+    std::string why;
+    int rc = parse_program(kernel, (int)d, &mo->hp, &why);
+    if (rc != GPRX_OK) {
+      set_error(err, rc, why);
+      return rc;
+    }
+    if (!log_theta) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "missing parameters");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    rc = exact_validate_theta(mo->hp, log_theta, p, err);
+    if (rc != GPRX_OK) return rc;
+    SgprModel& M = *mo;
+    M.n = (int)n;
+    M.m = (int)m;
+    M.d = (int)d;
+    M.dp = padded_dim((int)d);
+    M.mpad = (long)round_up(m, kTile);
+    M.mt = (int)(M.mpad / kTile);
+    M.ldt = (long)((M.mt + 1) / 2) * kTile;
+    // Split the n-long reduction of U U^T into chunks so the launch holds
+    // about 2048 workgroups; pad n to a whole number of chunks.
+    const long ntiles = (long)round_up(n, kTile) / kTile;
+    const long lower = (long)M.mt * (M.mt + 1) / 2;
+    long want = std::max<long>(1, std::min<long>(ntiles, (2048 + lower - 1) / lower));
+    M.kc = ((ntiles + want - 1) / want) * kTile;
+    M.npad = (long)round_up(n, (size_t)M.kc);
+    M.nchunks = (int)(M.npad / M.kc);
+    M.precision = precision;
+    M.theta.assign(log_theta, log_theta + p);
+    M.ynorm2 = 0.0;
+    for (size_t i = 0; i < n; ++i) M.ynorm2 += y[i] * y[i];
+    GPRX_HIP_CHECK(hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking));
+    for (auto& e : M.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
+    hipStream_t s = M.stream;
+    exact_upload_points(M.x, x, (int)n, (int)d, M.npad, M.dp, s);
+    exact_upload_points(M.z, z, (int)m, (int)d, M.mpad, M.dp, s);
+    {
+      std::vector<double> yh(M.npad, 0.0);
+      std::memcpy(yh.data(), y, sizeof(double) * n);
+      M.y.alloc(M.npad);
+      GPRX_HIP_CHECK(hipMemcpyAsync(M.y.p, yh.data(), sizeof(double) * M.npad, hipMemcpyHostToDevice, s));
+    }
+    const size_t mm = (size_t)M.mpad * M.mpad;
+    for (DevBuf<double>* b : {&M.Kmm, &M.KA, &M.Xmm, &M.S, &M.MA, &M.XM, &M.G1, &M.G2, &M.Wmm, &M.P64}) {
+      b->alloc(mm);
+      GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * mm, s));
+    }
+    M.Tm.alloc((size_t)M.ldt * (M.mpad / 2 + kTile));
+    const size_t un = (size_t)M.mpad * M.npad;
+    if (precision == GPRX_MIXED) {
+      M.U32.alloc(un);
+      M.Sp32.alloc(mm * M.nchunks);
+      M.P32.alloc(mm);
+    } else {
+      M.U64.alloc(un);
+      M.Sp64.alloc(mm * M.nchunks);
+    }
+    M.prog.alloc(1);
+    M.hprog.alloc(1);
+    for (DevBuf<double>* b : {&M.logd_mm, &M.logd_M, &M.b, &M.c, &M.tv}) {
+      b->alloc(M.mpad);
+      GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * M.mpad, s));
+    }
+    M.r.alloc(M.npad);
+    M.ur.alloc(M.npad);
+    M.kd.alloc(M.mpad);
+    const long gp = std::max<long>((M.npad + 255) / 256 * M.mpad, (M.mpad + 255) / 256 * M.mpad);
+    M.gpart.alloc(gp);
+    M.partials.alloc((size_t)(M.mpad / kTile) * (M.npad / kTile) * kMaxAcc);
+    M.partials2.alloc((size_t)cross_tiles(M.mpad, M.mpad) * kMaxAcc);
+    M.accs.alloc(3 * kMaxAcc);
+    M.scal.alloc(16);
+    M.info.alloc(1);
+    M.hscal.alloc(16);
+    M.hacc.alloc(3 * kMaxAcc);
+    M.hinfo.alloc(1);
+    GPRX_HIP_CHECK(hipMemsetAsync(M.scal.p, 0, sizeof(double) * 16, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    *out = mo.release();
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_sgpr_value_and_grad(gprx_hip_model* model, const double* log_theta, size_t p,
+                                 double* value, double* grad, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kSgpr || !log_theta || !value) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* M = static_cast<SgprModel*>(model);
+    int rc = exact_validate_theta(M->hp, log_theta, p, err);
+    if (rc != GPRX_OK) return rc;
+    std::vector<double> g(p, 0.0);
+    double v = 0.0;
+    rc = M->evaluate(log_theta, grad != nullptr, &v, g.data(), err);
+    if (rc != GPRX_OK) return rc;
+    M->theta.assign(log_theta, log_theta + p);
+    *value = v;
+    if (grad) std::memcpy(grad, g.data(), sizeof(double) * p);
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_sgpr_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                          int32_t variance_kind, double* mean, double* var, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kSgpr || !mean) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* M = static_cast<SgprModel*>(model);
+    if (d != (size_t)M->d) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "query dimension differs from training");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    if (q == 0) return GPRX_OK;
+    if (!xs || !all_finite(xs, q * d)) {
+      set_error(err, GPRX_NON_FINITE_INPUT, "non-finite query");
+      return GPRX_NON_FINITE_INPUT;
+    }
+    return M->predict(xs, q, variance_kind, mean, var, err);
+  });
+}
+
+}  // extern "C"

@@ -186,3 +186,121 @@ template void factor_inverse<float>(float*, float*, float*, long, long, int, dou
                                     hipStream_t);
 
 }  // namespace gprx
+
+namespace gprx {
+
+__global__ void k_mat_combine(double* out, long ldo, int rows, int cols, double a, const double* A,
+                              long lda, double b, const double* B, long ldb) {
+  const long total = (long)rows * cols;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    const double va = A ? A[i + j * lda] : 0.0;
+    const double vb = B ? B[i + j * ldb] : 0.0;
+    out[i + j * ldo] = a * va + b * vb;
+  }
+}
+
+void mat_combine(double* out, long ldo, int rows, int cols, double a, const double* A, long lda,
+                 double b, const double* B, long ldb, hipStream_t s) {
+  const long total = (long)rows * cols;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_mat_combine, dim3(blocks), dim3(256), 0, s, out, ldo, rows, cols, a, A, lda,
+                     b, B, ldb);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_symmetrize(double* M, long ld, int n) {
+  const long total = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i < j) M[i + j * ld] = M[j + i * ld];
+  }
+}
+
+void symmetrize_lower(double* M, long ld, int n, hipStream_t s) {
+  const long total = (long)n * n;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_symmetrize, dim3(blocks), dim3(256), 0, s, M, ld, n);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(1024) void k_frob_dot(const double* A, long lda, const double* B,
+                                                   long ldb, int rows, int cols, double* out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  const long total = (long)rows * cols;
+  for (long e = threadIdx.x; e < total; e += 1024) {
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    acc += A[i + j * lda] * B[i + j * ldb];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+void frob_dot(const double* A, long lda, const double* B, long ldb, int rows, int cols, double* out,
+              hipStream_t s) {
+  hipLaunchKernelGGL(k_frob_dot, dim3(1), dim3(1024), 0, s, A, lda, B, ldb, rows, cols, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_outer_add(double* W, long ld, int rows, int cols, double alpha, const double* u,
+                            const double* v) {
+  const long total = (long)rows * cols;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    W[i + j * ld] += alpha * u[i] * v[j];
+  }
+}
+
+void outer_add(double* W, long ld, int rows, int cols, double alpha, const double* u,
+               const double* v, hipStream_t s) {
+  const long total = (long)rows * cols;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_outer_add, dim3(blocks), dim3(256), 0, s, W, ld, rows, cols, alpha, u, v);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_vec_combine(double* out, int n, int npad, double a, const double* y, double b,
+                              const double* v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= npad) return;
+  out[i] = i < n ? a * y[i] + b * v[i] : 0.0;
+}
+
+void vec_combine(double* out, int n, int npad, double a, const double* y, double b, const double* v,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_combine, dim3((npad + 255) / 256), dim3(256), 0, s, out, n, npad, a, y,
+                     b, v);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_sum_partials_lower(const T* parts, int nparts, long ld, int n, double* out) {
+  const long total = (long)n * n;
+  const long stride = ld * ld;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i < j) continue;
+    double acc = 0.0;
+    for (int p = 0; p < nparts; ++p) acc += (double)parts[p * stride + i + j * ld];
+    out[i + j * ld] = acc;
+  }
+}
+
+template <typename T>
+void sum_partials_lower(const T* parts, int nparts, long ld, int n, double* out, hipStream_t s) {
+  const long total = (long)n * n;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL((k_sum_partials_lower<T>), dim3(blocks), dim3(256), 0, s, parts, nparts, ld,
+                     n, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+template void sum_partials_lower<double>(const double*, int, long, int, double*, hipStream_t);
+template void sum_partials_lower<float>(const float*, int, long, int, double*, hipStream_t);
+
+}  // namespace gprx

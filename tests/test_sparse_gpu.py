@@ -1,0 +1,141 @@
+"""Sgpr (VFE) and Svgp on the GPU against the reference goldens and the oracle.
+
+FP64: bound / -ELBO, every gradient component and predictions within 1e-9
+(|a - e| / max(|e|, 1)) of the oracle (the GPU uses the M = s2 K_mm + K_mn K_nm
+formulation, the oracle the reference's A = L^-1 K_mn one) and within the
+reference's 1e-8 of its GPyTorch goldens.  Mixed: within 1e-4.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gprx_amd as g
+import oracle
+from conftest import assert_close, load_golden, rel_err
+from test_oracle_sparse import _sparse_problem, _spec, pack_lower, prior_packed
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["sgpr_forrester", "sgpr_sphere"])
+def test_sgpr_goldens(gpu, name):
+    gd = load_golden(name)
+    n, d, m, q = gd["n_rows"], gd["n_cols"], gd["m"], gd["xs_n_rows"]
+    for k in gd["kernels"]:
+        spec = _spec(k)
+        if spec is None:
+            continue
+        print(f"[sgpr golden] {name} {k['name']}", flush=True)
+        th = np.concatenate([spec.get_params(), [math.log(gd["noise_variance"])]])
+        f = g.Sgpr(spec, g.GaussianLikelihood(gd["noise_variance"])).prepare(gd["x"], n, d, gd["y"], gd["z"], m)
+        grad = np.empty(th.size)
+        val = f.value_and_gradient_into(th, grad)
+        assert_close(val, k["neg_log_marginal_likelihood"], 1e-8, "bound")
+        ov, og = oracle.vfe_value_grad(spec.program(), th, gd["x"], n, d, gd["y"], gd["z"], m)
+        assert_close(val, ov, 1e-9, "bound vs oracle")
+        assert_close(grad, og, 1e-9, "grad vs oracle")
+        p = f.predict(gd["xs"], q, d)
+        lat = f.predict_with(gd["xs"], q, d, g.VarianceKind.LATENT)
+        assert_close(p.mean, k["mean"], 1e-8, "mean")
+        assert_close(p.variance, k["observation_variance"], 1e-8, "obs")
+        assert_close(lat.variance, k["latent_variance"], 1e-8, "latent")
+
+
+@pytest.mark.parametrize("n,m,d", [(60, 9, 2), (1000, 50, 2), (3000, 200, 2)])
+def test_sgpr_composite_vs_oracle(gpu, n, m, d):
+    print(f"[sgpr] n={n} m={m} d={d}", flush=True)
+    x, y, z, spec, th, n, m, d = _sparse_problem(n=n, m=m, d=d, seed=n)
+    for prec, tol in ((g.Precision.FP64, 1e-9), (g.Precision.MIXED, 1e-4)):
+        f = g.Sgpr(spec, g.GaussianLikelihood(0.02)).with_precision(prec).prepare(x, n, d, y, z, m)
+        grad = np.empty(th.size)
+        val = f.value_and_gradient_into(th, grad)
+        ov, og = oracle.vfe_value_grad(spec.program(), th, x, n, d, y, z, m)
+        print(f"  prec={prec} value err {rel_err(val, ov):.2e} grad err {rel_err(grad, og):.2e}")
+        assert_close(val, ov, tol, "bound")
+        assert_close(grad, og, tol, "grad")
+
+
+@pytest.mark.parametrize("name", ["svgp_forrester", "svgp_sphere"])
+def test_svgp_prior_goldens(gpu, name):
+    gd = load_golden(name)
+    n, d, m, q = gd["n_rows"], gd["n_cols"], gd["m"], gd["xs_n_rows"]
+    for k in gd["kernels"]:
+        spec = _spec(k)
+        if spec is None:
+            continue
+        print(f"[svgp golden] {name} {k['name']}", flush=True)
+        th = np.concatenate([spec.get_params(), [math.log(gd["noise_variance"])]])
+        params = np.concatenate([th, np.zeros(m), prior_packed(m)])
+        f = g.Svgp(spec, g.GaussianLikelihood(gd["noise_variance"])).prepare(gd["x"], n, d, gd["y"], gd["z"], m)
+        grad = np.empty(params.size)
+        val = f.value_and_gradient_into(params, grad)
+        assert_close(val, k["neg_elbo"], 1e-8, "-elbo")
+        ov, og = oracle.svgp_value_grad(spec.program(), params, gd["x"], n, d, gd["y"], gd["z"], m, np.arange(n))
+        assert_close(grad, og, 1e-9, "grad vs oracle")
+        p = f.predict_with(gd["xs"], q, d, params)
+        assert_close(p.mean, k["mean"], 1e-8, "mean")
+        assert_close(p.variance, k["observation_variance"], 1e-8, "obs")
+
+
+@pytest.mark.parametrize("n,m,b", [(40, 6, 17), (2000, 100, 256), (5000, 300, 1000)])
+def test_svgp_random_q_vs_oracle(gpu, n, m, b):
+    print(f"[svgp] n={n} m={m} b={b}", flush=True)
+    x, y, z, spec, th, n, m, d = _sparse_problem(n=n, m=m, seed=m)
+    rng = np.random.default_rng(9)
+    Lq = np.tril(0.1 * rng.standard_normal((m, m))) + np.diag(0.5 + rng.random(m))
+    params = np.concatenate([th, 0.4 * rng.standard_normal(m), pack_lower(Lq)])
+    batch = rng.permutation(n)[:b]
+    ov, og = oracle.svgp_value_grad(spec.program(), params, x, n, d, y, z, m, batch)
+    for prec, tol in ((g.Precision.FP64, 1e-9), (g.Precision.MIXED, 1e-4)):
+        f = g.Svgp(spec, g.GaussianLikelihood(0.02)).with_precision(prec).prepare(x, n, d, y, z, m)
+        grad = np.empty(params.size)
+        val = f.value_and_gradient_into(params, grad, batch)
+        print(f"  prec={prec} value err {rel_err(val, ov):.2e} grad err {rel_err(grad, og):.2e}")
+        assert_close(val, ov, tol, "-elbo")
+        assert_close(grad, og, tol, "grad")
+
+
+def test_svgp_adam_matches_oracle_loop(gpu):
+    """Three Adam steps with the reference's shuffle against the oracle-driven loop."""
+    from gprx_amd.rng import SeededRng
+
+    x, y, z, spec, th, n, m, d = _sparse_problem(n=50, m=5, seed=7)
+    f = g.Svgp(spec, g.GaussianLikelihood(0.02)).prepare(x, n, d, y, z, m)
+    p0 = f.params.copy()
+    steps, bs, lr = 5, 16, 0.01
+    pg, vals = f.adam(p0, steps=steps, batch_size=bs, lr=lr, seed=3)
+    # host restatement of run_adam_fit with the oracle's value/gradient
+    ln_lo, ln_hi = math.log(1e-5), math.log(1e5)
+    span, sc = ln_hi - ln_lo, (ln_hi - ln_lo) / 4
+    nth = th.size
+    zt = np.log(((p0[:nth] - ln_lo) / span) / (1 - (p0[:nth] - ln_lo) / span)) * sc
+    q = p0[nth:].copy()
+    diag_idx = []
+    k = m
+    for j in range(m):
+        diag_idx.append(k)
+        k += m - j
+    q[diag_idx] = np.log(q[diag_idx])
+    zz = np.concatenate([zt, q])
+    m1, m2 = np.zeros_like(zz), np.zeros_like(zz)
+    order = np.arange(n, dtype=np.uint64)
+    rng = SeededRng(3)
+    start, t, ref_vals = n, 0, []
+    for _ in range(steps):
+        if start >= n:
+            rng.shuffle(order)
+            start = 0
+        batch = order[start:start + bs]
+        user = zz.copy()
+        s = 1 / (1 + np.exp(-zz[:nth] / sc))
+        user[:nth] = ln_lo + s * span
+        user[nth + np.array(diag_idx)] = np.exp(zz[nth + np.array(diag_idx)])
+        v, gu = oracle.svgp_value_grad(spec.program(), user, x, n, d, y, z, m, batch)
+        ref_vals.append(v)
+        gz = gu.copy()
+        gz[:nth] *= span * s * (1 - s) / sc
+        gz[nth + np.array(diag_idx)] *= user[nth + np.array(diag_idx)]
+        t = oracle.adam_step(zz, gz, m1, m2, t, lr=lr)
+        start += bs
+    assert_close(vals, ref_vals, 1e-8, "trajectory")

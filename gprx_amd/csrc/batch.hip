@@ -77,7 +77,7 @@ __device__ __forceinline__ void run_task(const TileTask& t, const T* Aar, const 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_tasks(const TileTask* __restrict__ tasks, int count,
+__global__ __launch_bounds__(256, kTileOccupancy) void k_gemm_tasks(const TileTask* __restrict__ tasks, int count,
                                                     const T* Aar, const T* Bar, T* Car) {
   __shared__ TileSmem<T> sm;
   const int id = xcd_remap(blockIdx.x, count);

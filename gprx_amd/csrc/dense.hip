@@ -19,7 +19,7 @@
 namespace gprx {
 
 template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g) {
+__global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
   using AV = typename Acc<T>::V;
   __shared__ TileSmem<T> sm;
   const int z = blockIdx.y;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
 }
 
 template <typename T>
-__device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
+__device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
                                 double* __restrict__ logdiag, int* info, double* s, double* xd,
                                 int* sbad) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -163,14 +163,20 @@ __device__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict
         }
       }
       if (lane < 16) {
+        // Every register index is a compile-time constant: a lane-dependent
+        // index (d[r]) would move the arrays to scratch memory.
+        double drr = 0.0, xrr = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
+        for (int k = 0; k < 16; ++k) {
           if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
-#pragma unroll
-        for (int c = 0; c < 16; ++c)
-          if (c < r) s[(c0 + r) * DS + c0 + c] = xr[c];
-        xd[c0 + r] = xr[r];
-        logdiag[c0 + r] = log(d[r]);
+          if (k < r) s[(c0 + r) * DS + c0 + k] = xr[k];
+          if (k == r) {
+            drr = d[k];
+            xrr = xr[k];
+          }
+        }
+        xd[c0 + r] = xrr;
+        logdiag[c0 + r] = log(drr);
       }
     }
     __syncthreads();

@@ -93,3 +93,15 @@ void vec_combine(double* out, int n, int npad, double a, const double* y, double
 template <typename T>
 void sum_partials_lower(const T* parts, int nparts, long ld, int n, double* out, hipStream_t s);
 }  // namespace gprx
+
+namespace gprx {
+// M_ii += v for i < n.
+void add_diag(double* M, long ld, int n, double v, hipStream_t s);
+// out[0] = sum_{i<n} M_ii (one block).
+void diag_sum(const double* M, long ld, int n, double* out, hipStream_t s);
+// H = tril_half(W): strict lower of W, half its diagonal, zero above (W of storage T).
+template <typename T>
+void tril_half(const T* W, long ld, int n, double* H, hipStream_t s);
+// W = -1/2 (T2 + T2^T) over n x n.
+void neg_sym(const double* T2, long ld, int n, double* W, hipStream_t s);
+}  // namespace gprx

@@ -1,23 +1,24 @@
-// Sparse VFE GPR (Sgpr) on one GPU.
-//
-// gprx forms A = L^-1 K_mn, B = A A^T + s2 I and an adjoint over A
-// (src/sgpr/factor/vfe.rs:191-369, derivatives.rs:94-201).  This backend
-// evaluates the same bound and gradient through m x m matrices and one pass
-// of MFMA products over U = K_mn (m x n):
-//   S = U U^T, b = U y                       (split-K SYRK on MFMA, GEMV)
-//   M = s2 Kt + S, Kt = K_mm + j I          (so B = L^-1 M L^-T)
-//   c = M^-1 b
-//   F = 1/2 [n log 2pi + (n - m) log s2 + log|M| - log|Kt| + (|y|^2 - b'c)/s2]
-//       + (sum k_ii - <Kt^-1, S>) / (2 s2)
-// and, with P = M^-1 - Kt^-1 / s2 and r = y - U^T c,
-//   W_mm = 1/2 [s2 M^-1 - Kt^-1 + c c^T + Kt^-1 S Kt^-1 / s2]
-//   W_mn = P U - c r^T / s2                 (fused: MFMA tile + contraction)
-//   dF/dtheta = <W_mm, dK_mm> + <W_mn, dK_mn> + 1/(2 s2) sum dk_ii
-//   dF/dlog s2 = (n - m)/2 + s2/2 <M^-1, Kt> - quad/2 + c'Kt c/2 - trace
-// The m x m algebra and both Cholesky factors are f64; U, S and P U run in
-// f32 on MFMA in the mixed form.
+// Sparse VFE GPR (Sgpr) on one GPU, in gprx's whitened formulation
+// (src/sgpr/factor/vfe.rs:191-369, src/sgpr/factor/derivatives.rs:94-201):
+//   L = chol(K_mm + j I), X = L^-1, U = K_mn           (m x n)
+//   A = X U                                           (TRMM on MFMA)
+//   B = A A^T + s2 I                                  (split-K SYRK on MFMA)
+//   w = B^-1 A y, r = y - A^T w
+//   F = 1/2 [n log 2pi + (n - m) log s2 + log|B| + (|y|^2 - (A y)'w)/s2]
+//       + (sum k_ii - |A|_F^2) / (2 s2)
+// The adjoint G = B^-1 A - (w r^T + A)/s2 is never formed: with
+// Q = X^T (B^-1 - I/s2) and v = X^T w,
+//   w_mn = X^T G = Q A - v r^T / s2                 (fused: MFMA tile + contraction)
+//   G A^T = I - s2 B^-1 - (B - s2 I)/s2 - w (A r)^T / s2 (+ I)   (m x m only)
+//   w_mm = -sym(X^T tril_half(G A^T) X)
+//   dF/dtheta = <w_mm, dK_mm> + <w_mn, dK_mn> + 1/(2 s2) sum dk_ii
+//   dF/dlog s2 = 1/2 ((n - m) + s2 tr B^-1 + |w|^2 - quad) - trace
+// so the data-sized products are 4 m^2 n (A, A A^T, Q A) against gprx's
+// 7 m^2 n.  The m x m algebra and both Cholesky factors are f64; U, A, A A^T
+// and Q A run in f32 on MFMA in the mixed form.
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 #include "kernel_eval.hpp"
 #include "model.hpp"
@@ -68,16 +69,26 @@ __global__ __launch_bounds__(256) void k_pu_contract(
   for (int q = 0; q < DP; ++q) ad[q] = 0.0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // The GEMM's LDS is free after the main loop: stage one 16-row strip of
-  // each wave's 64 x 64 result at a time as f64, [wave][col][row].
+  // each wave's 64 x 64 result at a time as f64, [wave][col][row].  The
+  // strip index goes through a switch so every accumulator index is a
+  // compile-time constant.
   double* strip = reinterpret_cast<double*>(&sm);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    __syncthreads();
+  auto stage = [&](auto I) {
+    constexpr int i = decltype(I)::value;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         strip[wave * 1024 + (j * 16 + drow<T>(lane, q)) * 16 + (lane & 15)] = (double)acc[i][j][q];
+  };
+  for (int i = 0; i < 4; ++i) {
+    __syncthreads();
+    switch (i) {
+      case 0: stage(std::integral_constant<int, 0>{}); break;
+      case 1: stage(std::integral_constant<int, 1>{}); break;
+      case 2: stage(std::integral_constant<int, 2>{}); break;
+      default: stage(std::integral_constant<int, 3>{}); break;
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < 4096; e += 256) {
       const int w = e >> 10, rem = e & 1023, cc = rem >> 4, rr = rem & 15;
@@ -166,11 +177,12 @@ struct SgprModel : gprx_hip_model {
   DevBuf<double> x, y, z;
   DevBuf<DevProgram> prog;
   HostBuf<DevProgram> hprog;
-  DevBuf<double> Kmm, KA, Xmm, Tm, S, MA, XM, G1, G2, Wmm, P64;
-  DevBuf<double> U64, Sp64;
-  DevBuf<float> U32, Sp32, P32;
-  DevBuf<double> logd_mm, logd_M, b, c, tv, r, ur, kd, gpart, partials, partials2, partials3, accs,
-      accs2, scal;
+  // m x m, f64
+  DevBuf<double> KA, Xmm, Tm, S, BA, XB, Dm, Q64, H, T1, T2, Wmm;
+  // m x n and split-K partials, storage type
+  DevBuf<double> U64, A64, Sp64, Xc64;
+  DevBuf<float> U32, A32, Sp32, Xc32, Q32;
+  DevBuf<double> logd_mm, logd_B, b, w, v, tv, r, ur, Ar, gpart, partials, partials2, accs, scal;
   DevBuf<int> info;
   HostBuf<double> hscal, hacc;
   HostBuf<int> hinfo;
@@ -196,23 +208,30 @@ struct SgprModel : gprx_hip_model {
     upload_program(th, j);
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
     fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, s);
-    GPRX_HIP_CHECK(hipMemcpyAsync(Kmm.p, KA.p, sizeof(double) * mpad * mpad, hipMemcpyDeviceToDevice, s));
     factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd_mm.p, info.p, s);
     return sync_info();
   }
 
+  // U = K(Z, X), A = X U, S = A A^T (lower), b = A y.
   template <typename T>
-  void data_pass(T* U, T* Sp) {
+  void data_pass(T* U, T* A, T* Xc, T* Sp) {
     hipStream_t s = stream;
+    const int M = (int)mpad, N = (int)npad;
     fill_cross<T>(dp, prog.p, zp(), xp(), U, mpad, s);
+    if (std::is_same<T, double>::value)
+      GPRX_HIP_CHECK(hipMemcpyAsync(Xc, Xmm.p, sizeof(double) * mpad * mpad, hipMemcpyDeviceToDevice, s));
+    else
+      convert<float, double>(reinterpret_cast<float*>(Xc), Xmm.p, (size_t)mpad * mpad, s);
+    gemm_call<T>(false, false, Xc, mpad, U, mpad, A, mpad, M, N, M, 1.0, 0.0, kKhiRow, 0,
+                 kOrderColFastRev, s);
     GemmArgs<T> g{};
-    g.A = U;
+    g.A = A;
     g.lda = mpad;
-    g.B = U;
+    g.B = A;
     g.ldb = mpad;
     g.C = Sp;
     g.ldc = mpad;
-    g.M = g.N = (int)mpad;
+    g.M = g.N = M;
     g.K = (int)kc;
     g.alpha = 1.0;
     g.beta = 0.0;
@@ -221,20 +240,23 @@ struct SgprModel : gprx_hip_model {
     g.strideA = g.strideB = kc * mpad;
     g.strideC = mpad * mpad;
     gemm<T>(false, true, g, s);
-    sum_partials_lower<T>(Sp, nchunks, mpad, (int)mpad, S.p, s);
-    gemv_n<T>(U, mpad, (int)mpad, (int)npad, y.p, b.p, gpart.p, 0, 0, 0, s);
+    sum_partials_lower<T>(Sp, nchunks, mpad, M, S.p, s);
+    gemv_n<T>(A, mpad, M, N, y.p, b.p, gpart.p, 0, 0, 0, s);
   }
 
+  // r = y - A^T w, Ar = A r.
   template <typename T>
-  void resid_pass(const T* U) {
-    gemv_t<T>(U, mpad, (int)mpad, (int)npad, c.p, ur.p, 0, 0, 0, 0, stream);
+  void resid_pass(const T* A) {
+    gemv_t<T>(A, mpad, (int)mpad, (int)npad, w.p, ur.p, 0, 0, 0, 0, stream);
     vec_combine(r.p, n, (int)npad, 1.0, y.p, -1.0, ur.p, stream);
+    gemv_n<T>(A, mpad, (int)mpad, (int)npad, r.p, Ar.p, gpart.p, 0, 0, 0, stream);
   }
 
   int evaluate(const double* th, bool want_grad, double* value, double* grad, gprx_error* err) {
     hipStream_t s = stream;
     const int P = hp.n_params;
     const double s2 = std::exp(th[P]);
+    const int M = (int)mpad;
     // K_mm with the sparse default adaptive jitter (src/policy.rs:222-226).
     double j = 0.0, next = 1e-8;
     int remaining = 5;
@@ -257,62 +279,65 @@ struct SgprModel : gprx_hip_model {
     kmm_jitter = j;
     GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
     if (mixed())
-      data_pass<float>(U32.p, Sp32.p);
+      data_pass<float>(U32.p, A32.p, Xc32.p, Sp32.p);
     else
-      data_pass<double>(U64.p, Sp64.p);
-    // M = s2 Kt + S (lower), factor (no jitter retries: src/sgpr/factor/vfe.rs:244-250).
-    mat_combine(MA.p, mpad, (int)mpad, (int)mpad, s2, Kmm.p, mpad, 1.0, S.p, mpad, s);
+      data_pass<double>(U64.p, A64.p, Xc64.p, Sp64.p);
+    // B = A A^T + s2 I; no jitter retries (src/sgpr/factor/vfe.rs:244-250).
+    symmetrize_lower(S.p, mpad, M, s);
+    mat_combine(BA.p, mpad, M, M, 1.0, S.p, mpad, 0.0, nullptr, 0, s);
+    add_diag(BA.p, mpad, M, s2, s);
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
-    factor_inverse<double>(MA.p, XM.p, Tm.p, mpad, ldt, mt, logd_M.p, info.p, s);
-    gemv_n<double>(XM.p, mpad, (int)mpad, (int)mpad, b.p, tv.p, gpart.p, 1, 0, 0, s);
-    gemv_t<double>(XM.p, mpad, (int)mpad, (int)mpad, tv.p, c.p, 1, 0, 0, 0, s);
-    // K_t^-1 into KA, M^-1 into MA (both consumed by the factors), full storage.
-    gemm_call<double>(true, false, Xmm.p, mpad, Xmm.p, mpad, KA.p, mpad, (int)mpad, (int)mpad,
-                      (int)mpad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
-    gemm_call<double>(true, false, XM.p, mpad, XM.p, mpad, MA.p, mpad, (int)mpad, (int)mpad,
-                      (int)mpad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
-    symmetrize_lower(KA.p, mpad, (int)mpad, s);
-    symmetrize_lower(MA.p, mpad, (int)mpad, s);
-    symmetrize_lower(S.p, mpad, (int)mpad, s);
-    symmetrize_lower(Kmm.p, mpad, (int)mpad, s);
-    // scalars: [0] b'c, [1] sum log L_M, [2] sum log L_mm (dot_and_sum pairs),
-    // [4] <Kt^-1, S>, [5] <M^-1, Kt>, [6] c'Kt c, [7] k(x, x)
-    dot_and_sum(b.p, c.p, logd_M.p, m, scal.p, s);
-    dot_and_sum(b.p, c.p, logd_mm.p, m, scal.p + 2, s);
-    frob_dot(KA.p, mpad, S.p, mpad, m, m, scal.p + 4, s);
-    frob_dot(MA.p, mpad, Kmm.p, mpad, m, m, scal.p + 5, s);
-    gemv_n<double>(Kmm.p, mpad, (int)mpad, (int)mpad, c.p, tv.p, gpart.p, 0, 0, 0, s);
-    dot_and_sum(c.p, tv.p, c.p, m, scal.p + 8, s);
+    factor_inverse<double>(BA.p, XB.p, Tm.p, mpad, ldt, mt, logd_B.p, info.p, s);
+    gemv_n<double>(XB.p, mpad, M, M, b.p, tv.p, gpart.p, 1, 0, 0, s);
+    gemv_t<double>(XB.p, mpad, M, M, tv.p, w.p, 1, 0, 0, 0, s);
+    // B^-1 into BA (consumed by the factor), full storage.
+    gemm_call<double>(true, false, XB.p, mpad, XB.p, mpad, BA.p, mpad, M, M, M, 1.0, 0.0, kKloRow,
+                      1, kOrderRowFast, s);
+    symmetrize_lower(BA.p, mpad, M, s);
+    if (mixed())
+      resid_pass<float>(A32.p);
+    else
+      resid_pass<double>(A64.p);
+    // scalars: [0] (Ay)'w, [1] sum log L_B, [2] |w|^2, [3] -, [4] |A|_F^2,
+    // [5] tr B^-1, [7] k(x, x)
+    dot_and_sum(b.p, w.p, logd_B.p, m, scal.p, s);
+    dot_and_sum(w.p, w.p, w.p, m, scal.p + 2, s);
+    diag_sum(S.p, mpad, m, scal.p + 4, s);
+    diag_sum(BA.p, mpad, m, scal.p + 5, s);
     fill_diag(dp, prog.p, DevPoints{z.p, mpad, 1}, scal.p + 7, s);
     if (want_grad) {
-      // W_mm = 1/2 [s2 M^-1 - Kt^-1 + c c^T + Kt^-1 S Kt^-1 / s2]
-      gemm_call<double>(false, false, KA.p, mpad, S.p, mpad, G1.p, mpad, (int)mpad, (int)mpad,
-                        (int)mpad, 1.0, 0.0, 0, 0, kOrderRowFast, s);
-      gemm_call<double>(false, false, G1.p, mpad, KA.p, mpad, G2.p, mpad, (int)mpad, (int)mpad,
-                        (int)mpad, 1.0, 0.0, 0, 0, kOrderRowFast, s);
-      mat_combine(Wmm.p, mpad, (int)mpad, (int)mpad, 0.5 * s2, MA.p, mpad, -0.5, KA.p, mpad, s);
-      mat_combine(Wmm.p, mpad, (int)mpad, (int)mpad, 1.0, Wmm.p, mpad, 0.5 / s2, G2.p, mpad, s);
-      outer_add(Wmm.p, mpad, (int)mpad, (int)mpad, 0.5, c.p, c.p, s);
-      // P = M^-1 - Kt^-1 / s2
-      mat_combine(P64.p, mpad, (int)mpad, (int)mpad, 1.0, MA.p, mpad, -1.0 / s2, KA.p, mpad, s);
+      // Q = X^T (B^-1 - I/s2), v = X^T w
+      mat_combine(Dm.p, mpad, M, M, 1.0, BA.p, mpad, 0.0, nullptr, 0, s);
+      add_diag(Dm.p, mpad, M, -1.0 / s2, s);
+      gemm_call<double>(true, false, Xmm.p, mpad, Dm.p, mpad, Q64.p, mpad, M, M, M, 1.0, 0.0,
+                        kKloRow, 0, kOrderRowFast, s);
+      gemv_t<double>(Xmm.p, mpad, M, M, w.p, v.p, 1, 0, 0, 0, s);
       const int tiles1 = (int)(mpad / kTile) * (int)(npad / kTile);
       if (mixed()) {
-        convert<float, double>(P32.p, P64.p, (size_t)mpad * mpad, s);
-        resid_pass<float>(U32.p);
-        pu_contract<float>(dp, prog.p, P32.p, mpad, U32.p, mpad, (int)mpad, (int)npad, (int)mpad,
-                           zp(), xp(), c.p, r.p, 1.0 / s2, partials.p, s);
+        convert<float, double>(Q32.p, Q64.p, (size_t)mpad * mpad, s);
+        pu_contract<float>(dp, prog.p, Q32.p, mpad, A32.p, mpad, M, (int)npad, M, zp(), xp(), v.p,
+                           r.p, 1.0 / s2, partials.p, s);
       } else {
-        resid_pass<double>(U64.p);
-        pu_contract<double>(dp, prog.p, P64.p, mpad, U64.p, mpad, (int)mpad, (int)npad, (int)mpad,
-                            zp(), xp(), c.p, r.p, 1.0 / s2, partials.p, s);
+        pu_contract<double>(dp, prog.p, Q64.p, mpad, A64.p, mpad, M, (int)npad, M, zp(), xp(), v.p,
+                            r.p, 1.0 / s2, partials.p, s);
       }
       reduce_partials(partials.p, tiles1, accs.p, s);
+      // G A^T = I - s2 B^-1 - S/s2 - w (A r)^T / s2 ; w_mm = -sym(X^T tril_half(.) X)
+      mat_combine(H.p, mpad, M, M, -s2, BA.p, mpad, -1.0 / s2, S.p, mpad, s);
+      add_diag(H.p, mpad, M, 1.0, s);
+      outer_add(H.p, mpad, M, M, -1.0 / s2, w.p, Ar.p, s);
+      tril_half<double>(H.p, mpad, M, T2.p, s);
+      gemm_call<double>(true, false, Xmm.p, mpad, T2.p, mpad, T1.p, mpad, M, M, M, 1.0, 0.0,
+                        kKloRow, 0, kOrderRowFast, s);
+      gemm_call<double>(false, false, T1.p, mpad, Xmm.p, mpad, T2.p, mpad, M, M, M, 1.0, 0.0,
+                        kKloCol, 0, kOrderRowFast, s);
+      neg_sym(T2.p, mpad, M, Wmm.p, s);
       contract_cross<double>(dp, prog.p, zp(), zp(), Wmm.p, mpad, partials2.p, s);
       reduce_partials(partials2.p, cross_tiles(mpad, mpad), accs.p + kMaxAcc, s);
       contract_diag(dp, prog.p, xp(), 0.5 / s2, accs.p + 2 * kMaxAcc, s);
     }
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
-    GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, sizeof(double) * 10, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
     if (want_grad)
       GPRX_HIP_CHECK(hipMemcpyAsync(hacc.p, accs.p, sizeof(double) * 3 * kMaxAcc, hipMemcpyDeviceToHost, s));
     GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -322,18 +347,17 @@ struct SgprModel : gprx_hip_model {
     last_eval_ms = ms;
     if (*hinfo.p) {
       factored = false;
-      set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky of s2 K_mm + K_mn K_nm failed", GPRX_STAGE_FIT, 0.0,
+      set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky of A A^T + s2 I failed", GPRX_STAGE_FIT, 0.0,
                 (size_t)m);
       return GPRX_CHOLESKY_FAILED;
     }
     const double* sc = hscal.p;
-    const double bc = sc[0], logdet_M = 2.0 * sc[1], logdet_K = 2.0 * sc[3];
-    const double trKS = sc[4], trMK = sc[5], kxx = sc[7], cKc = sc[8];
-    const double kdiag_sum = (double)n * kxx;
-    const double quad = (ynorm2 - bc) / s2;
-    const double trace = (kdiag_sum - trKS) / (2.0 * s2);
+    const double ayw = sc[0], logdet_B = 2.0 * sc[1], w2 = sc[2], af2 = sc[4], trBinv = sc[5];
+    const double kdiag_sum = (double)n * sc[7];
+    const double quad = (ynorm2 - ayw) / s2;
+    const double trace = (kdiag_sum - af2) / (2.0 * s2);
     const double nm = (double)n - (double)m;
-    *value = 0.5 * ((double)n * kLog2Pi + nm * std::log(s2) + logdet_M - logdet_K + quad) + trace;
+    *value = 0.5 * ((double)n * kLog2Pi + nm * std::log(s2) + logdet_B + quad) + trace;
     if (!std::isfinite(*value)) {
       factored = false;
       set_error(err, GPRX_NON_FINITE_KERNEL_VALUE, "non-finite bound");
@@ -343,16 +367,18 @@ struct SgprModel : gprx_hip_model {
       double acc[kMaxAcc];
       for (int k = 0; k < kMaxAcc; ++k) acc[k] = hacc.p[k] + hacc.p[kMaxAcc + k] + hacc.p[2 * kMaxAcc + k];
       acc_to_grad(hp, th, acc, grad);
-      grad[P] = 0.5 * nm + 0.5 * s2 * trMK - 0.5 * quad + 0.5 * cKc - trace;
+      grad[P] = 0.5 * (nm + s2 * trBinv + w2 - quad) - trace;
     }
     factored = true;
     return GPRX_OK;
   }
 
+  // src/sgpr/factor/predict.rs:102-260: a = X k_m*, mean = a'w,
+  // latent = k** - |a|^2 + s2 |L_B^-1 a|^2.
   int predict(const double* xs, size_t q, int32_t kind, double* mean, double* var, gprx_error* err) {
     if (!factored) {
-      double v;
-      int rc = evaluate(theta.data(), false, &v, nullptr, err);
+      double vv;
+      int rc = evaluate(theta.data(), false, &vv, nullptr, err);
       if (rc != GPRX_OK) {
         if (err) err->stage = GPRX_STAGE_PREDICT;
         return rc;
@@ -373,10 +399,10 @@ struct SgprModel : gprx_hip_model {
     s2v.alloc(qpad);
     kss.alloc(qpad);
     fill_cross<double>(dp, prog.p, zp(), pq, Ks.p, mpad, s);
-    gemv_t<double>(Ks.p, mpad, (int)mpad, (int)qpad, c.p, mu.p, 0, 0, 0, 0, s);
     gemm_call<double>(false, false, Xmm.p, mpad, Ks.p, mpad, V1.p, mpad, (int)mpad, (int)qpad,
                       (int)mpad, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s);
-    gemm_call<double>(false, false, XM.p, mpad, Ks.p, mpad, V2.p, mpad, (int)mpad, (int)qpad,
+    gemv_t<double>(V1.p, mpad, (int)mpad, (int)qpad, w.p, mu.p, 0, 0, 0, 0, s);
+    gemm_call<double>(false, false, XB.p, mpad, V1.p, mpad, V2.p, mpad, (int)mpad, (int)qpad,
                       (int)mpad, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s);
     col_sumsq<double>(V1.p, mpad, (int)mpad, (int)qpad, s1.p, s);
     col_sumsq<double>(V2.p, mpad, (int)mpad, (int)qpad, s2v.p, s);
@@ -473,7 +499,7 @@ int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, c
       GPRX_HIP_CHECK(hipMemcpyAsync(M.y.p, yh.data(), sizeof(double) * M.npad, hipMemcpyHostToDevice, s));
     }
     const size_t mm = (size_t)M.mpad * M.mpad;
-    for (DevBuf<double>* b : {&M.Kmm, &M.KA, &M.Xmm, &M.S, &M.MA, &M.XM, &M.G1, &M.G2, &M.Wmm, &M.P64}) {
+    for (DevBuf<double>* b : {&M.KA, &M.Xmm, &M.S, &M.BA, &M.XB, &M.Dm, &M.Q64, &M.H, &M.T1, &M.T2, &M.Wmm}) {
       b->alloc(mm);
       GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * mm, s));
     }
@@ -481,21 +507,24 @@ int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, c
     const size_t un = (size_t)M.mpad * M.npad;
     if (precision == GPRX_MIXED) {
       M.U32.alloc(un);
+      M.A32.alloc(un);
       M.Sp32.alloc(mm * M.nchunks);
-      M.P32.alloc(mm);
+      M.Xc32.alloc(mm);
+      M.Q32.alloc(mm);
     } else {
       M.U64.alloc(un);
+      M.A64.alloc(un);
       M.Sp64.alloc(mm * M.nchunks);
+      M.Xc64.alloc(mm);
     }
     M.prog.alloc(1);
     M.hprog.alloc(1);
-    for (DevBuf<double>* b : {&M.logd_mm, &M.logd_M, &M.b, &M.c, &M.tv}) {
+    for (DevBuf<double>* b : {&M.logd_mm, &M.logd_B, &M.b, &M.w, &M.v, &M.tv, &M.Ar}) {
       b->alloc(M.mpad);
       GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * M.mpad, s));
     }
     M.r.alloc(M.npad);
     M.ur.alloc(M.npad);
-    M.kd.alloc(M.mpad);
     const long gp = std::max<long>((M.npad + 255) / 256 * M.mpad, (M.mpad + 255) / 256 * M.mpad);
     M.gpart.alloc(gp);
     M.partials.alloc((size_t)(M.mpad / kTile) * (M.npad / kTile) * kMaxAcc);

@@ -1,3 +1,6 @@
+// Workgroups per CU the tile kernels are compiled for.
+constexpr int kTileOccupancy = 2;
+
 // The MFMA tile main loop shared by the dense kernels (see dense.hip for
 // the lane layouts).  A 256-thread workgroup computes one 128 x 128 tile of
 // op(A) op(B) with four waves of 64 x 64, staging BK-deep slices of both
@@ -59,14 +62,16 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
 // of a fragment read alternate bank halves.
 template <typename T>
 struct Cfg;
+// BK 8 (f64) / 16 (f32) with two workgroups per CU measured fastest on
+// gfx950 among BK in {8, 16, 32} and 1-2 workgroups per CU.
 template <>
 struct Cfg<double> {
-  static constexpr int BK = 16;
+  static constexpr int BK = 8;
   static constexpr int LS = 144;
 };
 template <>
 struct Cfg<float> {
-  static constexpr int BK = 32;
+  static constexpr int BK = 16;
   static constexpr int LS = 160;
 };
 
@@ -108,13 +113,14 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   constexpr int LS = Cfg<T>::LS;
   constexpr int VEC = 16 / sizeof(T);
   using V = typename Vec<T>::V;
+  constexpr int NV = kTile * BK / VEC / 256;  // 16-byte vectors per thread per operand
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  V ra[4], rb[4];
+  V ra[NV], rb[NV];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int e = tid + 256 * v;
       if (!TA) {
         const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
@@ -136,7 +142,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
     T* a = sm.a[buf];
     T* b = sm.b[buf];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int e = tid + 256 * v;
       if (!TA) {
         const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;

@@ -304,3 +304,70 @@ template void sum_partials_lower<double>(const double*, int, long, int, double*,
 template void sum_partials_lower<float>(const float*, int, long, int, double*, hipStream_t);
 
 }  // namespace gprx
+
+namespace gprx {
+
+__global__ void k_add_diag(double* M, long ld, int n, double v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) M[(long)i * (ld + 1)] += v;
+}
+
+void add_diag(double* M, long ld, int n, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_diag, dim3((n + 255) / 256), dim3(256), 0, s, M, ld, n, v);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(1024) void k_diag_sum(const double* M, long ld, int n, double* out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) acc += M[(long)i * (ld + 1)];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+void diag_sum(const double* M, long ld, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_sum, dim3(1), dim3(1024), 0, s, M, ld, n, out);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_tril_half(const T* W, long ld, int n, double* H) {
+  const long total = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const double v = (double)W[i + j * ld];
+    H[i + j * ld] = i > j ? v : (i == j ? 0.5 * v : 0.0);
+  }
+}
+
+template <typename T>
+void tril_half(const T* W, long ld, int n, double* H, hipStream_t s) {
+  const long total = (long)n * n;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL((k_tril_half<T>), dim3(blocks), dim3(256), 0, s, W, ld, n, H);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+template void tril_half<double>(const double*, long, int, double*, hipStream_t);
+template void tril_half<float>(const float*, long, int, double*, hipStream_t);
+
+__global__ void k_neg_sym(const double* T2, long ld, int n, double* W) {
+  const long total = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    W[i + j * ld] = -0.5 * (T2[i + j * ld] + T2[j + i * ld]);
+  }
+}
+
+void neg_sym(const double* T2, long ld, int n, double* W, hipStream_t s) {
+  const long total = (long)n * n;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_neg_sym, dim3(blocks), dim3(256), 0, s, T2, ld, n, W);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace gprx

@@ -13,65 +13,219 @@
 // a lane's four results sit in four columns of C at one row: consecutive
 // lanes store consecutive rows, which is the contiguous direction of a
 // column-major C.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "dense.hpp"
 #include "tile_gemm.hpp"
 
 namespace gprx {
 
-template <typename T, bool TA, bool TB>
+template <typename T, bool TA, bool TB, int TT>
 __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
   using AV = typename Acc<T>::V;
-  __shared__ TileSmem<T> sm;
+  constexpr int FB = TT / 32, WT = TT / 2;
+  __shared__ TileSmem<T, TT> sm;
   const int z = blockIdx.y;
   const T* __restrict__ A = g.A + (long)z * g.strideA;
   const T* __restrict__ B = g.B + (long)z * g.strideB;
   T* __restrict__ C = g.C + (long)z * g.strideC;
-  const TileCoord tc = tile_coord(g.M, g.N, g.K, g.lower, g.order, g.krange);
+  TileCoord tc;
+  if (g.sched) {
+    if (!tile_from_schedule(g.sched, g.K, g.krange, tc, TT)) return;
+  } else {
+    tc = tile_coord(g.M, g.N, g.K, g.lower, g.order, g.krange, TT);
+  }
   const int ti = tc.ti, tj = tc.tj;
-  const int nk = tc.ke > tc.kb ? (tc.ke - tc.kb) / Cfg<T>::BK : 0;
+  const int nk = tc.ke > tc.kb ? (tc.ke - tc.kb) / Cfg<T, TT>::BK : 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  AV acc[4][4];
+  AV acc[FB][FB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
-  const T* Ablk = TA ? A + (long)ti * kTile * g.lda : A + (long)ti * kTile;
-  const T* Bblk = TB ? B + (long)tj * kTile : B + (long)tj * kTile * g.ldb;
-  tile_mainloop<T, TA, TB>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc);
+    for (int j = 0; j < FB; ++j) acc[i][j] = AV{};
+  const T* Ablk = TA ? A + (long)ti * TT * g.lda : A + (long)ti * TT;
+  const T* Bblk = TB ? B + (long)tj * TT : B + (long)tj * TT * g.ldb;
+  tile_mainloop<T, TA, TB, TT>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc);
 
   const bool diag_tile = g.lower && ti == tj;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < FB; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = wm * 64 + i * 16 + (lane & 15);
-        const int c = wn * 64 + j * 16 + drow<T>(lane, q);
+        const int r = wm * WT + i * 16 + (lane & 15);
+        const int c = wn * WT + j * 16 + drow<T>(lane, q);
         if (diag_tile && r < c) continue;
-        T* p = C + (long)(tj * kTile + c) * g.ldc + ti * kTile + r;
+        T* p = C + (long)(tj * TT + c) * g.ldc + ti * TT + r;
         double v = g.alpha * (double)acc[i][j][q];
         if (g.beta != 0.0) v += g.beta * (double)(*p);
         *p = (T)v;
       }
 }
 
+// ---------------------------------------------------------------------------
+// Tile schedules.  The hardware hands workgroup b to XCD b % 8, each XCD
+// with its own L2, and within an XCD starts workgroups in order as slots
+// free up.  A launch whose operands are triangular gives its tiles K-ranges
+// that differ by up to the whole K, so a fixed blockIdx -> tile map that
+// keeps neighbours together (for L2 reuse) puts most of the work on a few
+// XCDs.  The schedule built here:
+//   - groups tiles into S x S super-blocks (S = 8: one XCD's 64 slots at two
+//     workgroups per CU; smaller when the launch has few tiles), so tiles
+//     running together on an XCD share 2S operand panels;
+//   - deals super-blocks, heaviest first, each to the least-loaded XCD
+//     (longest-processing-time), which balances the XCDs' total work and
+//     starts the longest tiles first;
+//   - writes XCD x's k-th tile to slot 8k + x, padding with kNoTile.
+// Work of a tile is its K-range in 128-blocks plus one for the epilogue.
+// Schedules are cached per device and launch shape for the process.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kXcds = 8;
+
+struct SchedKey {
+  int dev, mt, nt, lower, krange, kt;
+  bool operator<(const SchedKey& o) const {
+    return std::tie(dev, mt, nt, lower, krange, kt) <
+           std::tie(o.dev, o.mt, o.nt, o.lower, o.krange, o.kt);
+  }
+};
+
+struct Sched {
+  unsigned* dev = nullptr;
+  int slots = 0;
+};
+
+std::mutex g_sched_mu;
+std::map<SchedKey, Sched> g_sched;
+
+int tile_work(int ti, int tj, int kt, int krange) {
+  int kb = 0, ke = kt;
+  if (krange & kKloRow) kb = std::max(kb, ti);
+  if (krange & kKloCol) kb = std::max(kb, tj);
+  if (krange & kKhiRow) ke = std::min(ke, ti + 1);
+  if (krange & kKhiCol) ke = std::min(ke, tj + 1);
+  return std::max(0, ke - kb) + 1;
+}
+
+}  // namespace
+
+static std::vector<unsigned> build_tile_schedule(int mt, int nt, int lower, int krange, int kt) {
+  struct Block {
+    long work = 0;
+    int order = 0;
+    std::vector<unsigned> tiles;
+  };
+  auto make_blocks = [&](int S) {
+    std::vector<Block> blocks;
+    const int bm = (mt + S - 1) / S, bn = (nt + S - 1) / S;
+    for (int I = 0; I < bm; ++I)
+      for (int J = 0; J < bn; ++J) {
+        if (lower && J > I) continue;
+        Block b;
+        b.order = (int)blocks.size();
+        for (int tj = J * S; tj < std::min(nt, (J + 1) * S); ++tj)
+          for (int ti = I * S; ti < std::min(mt, (I + 1) * S); ++ti) {
+            if (lower && tj > ti) continue;
+            b.tiles.push_back((unsigned)ti << 16 | (unsigned)tj);
+            b.work += tile_work(ti, tj, kt, krange);
+          }
+        if (!b.tiles.empty()) blocks.push_back(std::move(b));
+      }
+    return blocks;
+  };
+  int S = 8;
+  std::vector<Block> blocks = make_blocks(S);
+  while (S > 1 && (int)blocks.size() < 4 * kXcds) {
+    S /= 2;
+    blocks = make_blocks(S);
+  }
+  std::stable_sort(blocks.begin(), blocks.end(),
+                   [](const Block& a, const Block& b) { return a.work > b.work; });
+  std::vector<std::vector<unsigned>> lists(kXcds);
+  long load[kXcds] = {0};
+  for (const Block& b : blocks) {
+    int x = 0;
+    for (int i = 1; i < kXcds; ++i)
+      if (load[i] < load[x]) x = i;
+    load[x] += b.work;
+    lists[x].insert(lists[x].end(), b.tiles.begin(), b.tiles.end());
+  }
+  size_t len = 0;
+  for (auto& l : lists) len = std::max(len, l.size());
+  std::vector<unsigned> out(len * kXcds, kNoTile);
+  for (int x = 0; x < kXcds; ++x)
+    for (size_t k = 0; k < lists[x].size(); ++k) out[k * kXcds + x] = lists[x][k];
+  return out;
+}
+
+// Device copy of the schedule, or nullptr (the kernel then maps blockIdx.x
+// itself) when the stream is being captured and the schedule is not cached:
+// the upload is a synchronous copy.
+static const unsigned* tile_schedule(int mt, int nt, int lower, int krange, int kt,
+                                     hipStream_t stream, int* slots) {
+  int dev = 0;
+  GPRX_HIP_CHECK(hipGetDevice(&dev));
+  const SchedKey key{dev, mt, nt, lower, krange, kt};
+  std::lock_guard<std::mutex> lock(g_sched_mu);
+  auto it = g_sched.find(key);
+  if (it != g_sched.end()) {
+    *slots = it->second.slots;
+    return it->second.dev;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  GPRX_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return nullptr;
+  const std::vector<unsigned> h = build_tile_schedule(mt, nt, lower, krange, kt);
+  Sched sc;
+  sc.slots = (int)h.size();
+  GPRX_HIP_CHECK(hipMalloc(&sc.dev, sizeof(unsigned) * h.size()));
+  GPRX_HIP_CHECK(hipMemcpy(sc.dev, h.data(), sizeof(unsigned) * h.size(), hipMemcpyHostToDevice));
+  g_sched[key] = sc;
+  *slots = sc.slots;
+  return sc.dev;
+}
+
+template <typename T, int TT>
+static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream) {
+  const int mt = g.M / TT, nt = g.N / TT;
+  const int total = g.lower ? mt * (mt + 1) / 2 : mt * nt;
+  int slots = total;
+  g.sched = tile_schedule(mt, nt, g.lower, g.krange, g.K / TT, stream, &slots);
+  if (!g.sched) slots = total;
+  dim3 grid(slots, g.batch > 0 ? g.batch : 1);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((k_gemm<T, false, false, TT>), grid, dim3(256), 0, stream, g);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((k_gemm<T, false, true, TT>), grid, dim3(256), 0, stream, g);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((k_gemm<T, true, false, TT>), grid, dim3(256), 0, stream, g);
+  else
+    hipLaunchKernelGGL((k_gemm<T, true, true, TT>), grid, dim3(256), 0, stream, g);
+  GPRX_HIP_CHECK(hipGetLastError());
+}
+
+// 128-tiles while they give every CU at least one workgroup, 64-tiles (four
+// times the workgroups, a quarter of the MFMA work each) below that: a
+// small product is bound by how many CUs share it, not by per-CU reuse.
+constexpr int kSmallTileBelow = 256;
+
 template <typename T>
 void gemm(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream) {
   const int mt = g.M / kTile, nt = g.N / kTile;
-  const int total = g.lower ? mt * (mt + 1) / 2 : mt * nt;
+  const long total = (g.lower ? (long)mt * (mt + 1) / 2 : (long)mt * nt) * (g.batch > 0 ? g.batch : 1);
   if (total == 0) return;
-  dim3 grid(total, g.batch > 0 ? g.batch : 1);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((k_gemm<T, false, false>), grid, dim3(256), 0, stream, g);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((k_gemm<T, false, true>), grid, dim3(256), 0, stream, g);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((k_gemm<T, true, false>), grid, dim3(256), 0, stream, g);
+  if (total < kSmallTileBelow)
+    launch_gemm<T, 64>(ta, tb, g, stream);
   else
-    hipLaunchKernelGGL((k_gemm<T, true, true>), grid, dim3(256), 0, stream, g);
-  GPRX_HIP_CHECK(hipGetLastError());
+    launch_gemm<T, 128>(ta, tb, g, stream);
 }
 
 template void gemm<double>(bool, bool, const GemmArgs<double>&, hipStream_t);
@@ -84,17 +238,26 @@ template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 // inverse X = L^-1 stored transposed above the diagonal, X[r][c] (r > c) at
 // s[r * DS + c]; X[r][r] in xd[r].  Steps, per 16-column panel:
 //   (a) wave 0 factors the 16 x 16 diagonal sub-block in registers (lane r
-//       owns row r; pivots and columns move by cross-lane shuffles) and
-//       inverts it by forward substitution;
+//       owns row r; pivots and columns are broadcast with v_readlane) and
+//       forms its inverse in the same sweep;
 //   (b) panel rows below: L_i = A_i X_D^T on MFMA;
 //   (c) trailing update A_ij -= L_i L_j^T (lower) on MFMA.
-// Then Y_ik = X_D[i] L_ik in place, and X_ij = -sum_k Y_ik X_kj by
-// diagonal distance, all on MFMA.
+// Then the off-diagonal blocks of X by doubling, X21 = -X22 L21 X11 over
+// runs of 16, 32 and 64 rows, on MFMA.
 // ---------------------------------------------------------------------------
 constexpr int DS = 129;
 
 __device__ __forceinline__ double x_at(const double* s, const double* xd, int r, int c) {
   return r > c ? s[r * DS + c] : (r == c ? xd[r] : 0.0);
+}
+
+// Value of v in lane `src` of the wave, src uniform (a constant once the
+// loops around it are unrolled): two v_readlane_b32, no LDS round trip.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
@@ -103,80 +266,161 @@ __device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
   v = t - u * (u + 1) / 2;
 }
 
+// p^-1/2 from the hardware estimate and two Newton steps
+// y <- y + y/2 (1 - p y^2), each squaring the relative error.
+__device__ __forceinline__ double rsqrt_nr(double p) {
+  double y = __builtin_amdgcn_rsq(p);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-p * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+// One doubling level of the diagonal-block inverse: for each pair of
+// adjacent diagonal runs of HB 16-blocks at block offset o,
+//   T   = L21 X11   (X11 lower: k >= column)  -> stored over L21
+//   X21 = -X22 T    (X22 lower: k <= row)     -> X's transposed storage
+// L21 is read by other waves while T is formed, so T is held in registers
+// until a barrier; X21's slots are read by no one in the second product.
+template <int HB>
+__device__ __forceinline__ void inverse_level(double* s, const double* xd) {
+  constexpr int kPairs = 8 / (2 * HB), kPer = HB * HB;
+  constexpr int kTasks = kPairs * kPer, kMine = (kTasks + 3) / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  d4 acc[kMine];
+#pragma unroll
+  for (int i = 0; i < kMine; ++i) {
+    acc[i] = d4{};
+    const int t = wave + 4 * i;
+    if (t >= kTasks) continue;
+    const int o = (t / kPer) * 2 * HB, u = t % kPer;
+    const int bi = o + HB + u / HB, bj = o + u % HB;
+    for (int kb = bj; kb < o + HB; ++kb) {
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const int k = kb * 16 + kk + (lane >> 4);
+        const double a = s[k * DS + bi * 16 + (lane & 15)];
+        const double b = x_at(s, xd, k, bj * 16 + (lane & 15));
+        acc[i] = mfma16(a, b, acc[i]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kMine; ++i) {
+    const int t = wave + 4 * i;
+    if (t >= kTasks) continue;
+    const int o = (t / kPer) * 2 * HB, u = t % kPer;
+    const int bi = o + HB + u / HB, bj = o + u % HB;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      s[(bj * 16 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[i][q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kMine; ++i) {
+    const int t = wave + 4 * i;
+    if (t >= kTasks) continue;
+    const int o = (t / kPer) * 2 * HB, u = t % kPer;
+    const int bi = o + HB + u / HB, bj = o + u % HB;
+    d4 x = d4{};
+    for (int kb = o + HB; kb <= bi; ++kb) {
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const int k = kb * 16 + kk + (lane >> 4);
+        const double a = x_at(s, xd, bi * 16 + (lane & 15), k);
+        const double b = s[(bj * 16 + (lane & 15)) * DS + k];
+        x = mfma16(a, b, x);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      s[(bi * 16 + drow<double>(lane, q)) * DS + bj * 16 + (lane & 15)] = -x[q];
+  }
+  __syncthreads();
+}
+
 template <typename T>
 __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
                                 double* __restrict__ logdiag, int* info, double* s, double* xd,
                                 int* sbad) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) *sbad = 0;
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    const int r = e & 127, c = e >> 7;
-    s[c * DS + r] = (r >= c) ? (double)A[r + (long)c * lda] : 0.0;
+  // 16 loads in flight per thread: the whole block lies inside the matrix,
+  // so the upper part is read and dropped rather than branched around.
+  constexpr int kBatch = 16;
+#pragma unroll
+  for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int e = tid + 256 * (b0 + i);
+      v[i] = (double)A[(e & 127) + (long)(e >> 7) * lda];
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int e = tid + 256 * (b0 + i);
+      const int r = e & 127, c = e >> 7;
+      s[c * DS + r] = (r >= c) ? v[i] : 0.0;
+    }
   }
   __syncthreads();
 
   for (int jb = 0; jb < 8; ++jb) {
     const int c0 = jb * 16;
     if (wave == 0) {
+      // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k], xr[k] =
+      // X[r][k].  Entries right of the diagonal (k > r) start at zero and
+      // collect values no lower entry ever reads, so every update runs
+      // unconditionally.  Step j: pivot p = A[j][j], rs = p^-1/2; column j
+      // of L is d[j] * rs (lane j: p rs = sqrt(p)); rank-1 update of the
+      // columns right of j; row j of X = L^-1 is
+      //   X[j][c] = -rs * sum_{k<j} L[j][k] X[k][c],  X[j][j] = rs,
+      // whose sums (acc) gain column j of L as soon as it is final.
       const int r = lane & 15;
-      double d[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) d[k] = (k <= r) ? s[(c0 + k) * DS + c0 + r] : 0.0;
-      bool bad = false;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        double pj = __shfl(d[j], j);
-        if (!(pj > 0.0) || !isfinite(pj)) {
-          bad = true;
-          pj = 1.0;
-        }
-        const double ljj = sqrt(pj);
-        if (r == j)
-          d[j] = ljj;
-        else if (r > j)
-          d[j] /= ljj;
-#pragma unroll
-        for (int c = j + 1; c < 16; ++c) {
-          const double lcj = __shfl(d[j], c);
-          if (r >= c) d[c] -= d[j] * lcj;
-        }
-      }
-      if (bad && lane == 0) *sbad = 1;
-      double xr[16], acc[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        xr[c] = 0.0;
-        acc[c] = 0.0;
-      }
+      double d[16], xr[16], acc[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        if (r == k) {
-          const double inv = 1.0 / d[k];
-#pragma unroll
-          for (int c = 0; c < k; ++c) xr[c] = -acc[c] * inv;
-          xr[k] = inv;
-        }
-#pragma unroll
-        for (int c = 0; c <= k; ++c) {
-          const double xkc = __shfl(xr[c], k);
-          if (r > k) acc[c] += d[k] * xkc;
-        }
+        d[k] = s[(c0 + k) * DS + c0 + r];
+        xr[k] = 0.0;
+        acc[k] = 0.0;
       }
+      bool bad = false;
+      double lrr = 1.0, xrr = 1.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        double p = lane_bcast(d[j], j);
+        if (!(p > 0.0) || !isfinite(p)) {
+          bad = true;
+          p = 1.0;
+        }
+        const double rs = rsqrt_nr(p);
+        d[j] *= rs;
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
+        if (r == j) {
+#pragma unroll
+          for (int c = 0; c < j; ++c) xr[c] = -acc[c] * rs;
+          xr[j] = rs;
+          lrr = d[j];
+          xrr = rs;
+        }
+#pragma unroll
+        for (int c = 0; c <= j; ++c) acc[c] = fma(d[j], lane_bcast(xr[c], j), acc[c]);
+      }
+      if (bad && lane == 0) *sbad = 1;
       if (lane < 16) {
         // Every register index is a compile-time constant: a lane-dependent
         // index (d[r]) would move the arrays to scratch memory.
-        double drr = 0.0, xrr = 0.0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
           if (k < r) s[(c0 + r) * DS + c0 + k] = xr[k];
-          if (k == r) {
-            drr = d[k];
-            xrr = xr[k];
-          }
         }
         xd[c0 + r] = xrr;
-        logdiag[c0 + r] = log(drr);
+        logdiag[c0 + r] = log(lrr);
       }
     }
     __syncthreads();
@@ -220,46 +464,12 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
     __syncthreads();
   }
 
-  // Y_{bi,kb} = X_D[bi] L_{bi,kb}, in place over L (28 strictly-lower blocks).
-  for (int t = wave; t < 28; t += 4) {
-    int u, v;
-    tri_index_small(t, u, v);
-    const int bi = u + 1, kb = v;
-    d4 acc = d4{};
-#pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      const double a = x_at(s, xd, bi * 16 + (lane & 15), bi * 16 + kr);
-      const double b = s[(kb * 16 + (lane & 15)) * DS + bi * 16 + kr];
-      acc = mfma16(a, b, acc);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) s[(kb * 16 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[q];
-  }
-  __syncthreads();
-  // X_{bi,bj} = -sum_{k=bj}^{bi-1} Y_{bi,k} X_{k,bj}, by distance bi - bj.
-  for (int dist = 1; dist < 8; ++dist) {
-    for (int bj = wave; bj < 8 - dist; bj += 4) {
-      const int bi = bj + dist;
-      d4 acc = d4{};
-      for (int k = bj; k < bi; ++k) {
-#pragma unroll
-        for (int kk = 0; kk < 16; kk += 4) {
-          const int kr = kk + (lane >> 4);
-          const double a = s[(k * 16 + kr) * DS + bi * 16 + (lane & 15)];
-          const double b = x_at(s, xd, k * 16 + kr, bj * 16 + (lane & 15));
-          acc = mfma16(a, b, acc);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = bi * 16 + drow<double>(lane, q);
-        const int c = bj * 16 + (lane & 15);
-        s[r * DS + c] = -acc[q];
-      }
-    }
-    __syncthreads();
-  }
+  // X = L^-1 by doubling: with X of the 16 x 16 diagonal blocks known, for
+  // halves of HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
+  inverse_level<1>(s, xd);
+  inverse_level<2>(s, xd);
+  inverse_level<4>(s, xd);
+#pragma unroll 16
   for (int e = tid; e < kTile * kTile; e += 256) {
     const int r = e & 127, c = e >> 7;
     X[r + (long)c * ldx] = (T)x_at(s, xd, r, c);

@@ -33,7 +33,11 @@ struct GemmArgs {
   int order;   // kOrder*
   int batch;   // independent problems along blockIdx.y (>= 1)
   long strideA, strideB, strideC;
+  // Filled by gemm(): blockIdx.x -> (ti << 16 | tj), kNoTile for padding.
+  const unsigned* sched;
 };
+
+constexpr unsigned kNoTile = 0xffffffffu;
 
 // C = beta * C + alpha * op(A) op(B) over each tile's K-range.
 // ta: A is stored K x M (op = transpose); tb: B is stored N x K.

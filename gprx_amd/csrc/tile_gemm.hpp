@@ -1,16 +1,16 @@
-// Workgroups per CU the tile kernels are compiled for.
-constexpr int kTileOccupancy = 2;
-
 // The MFMA tile main loop shared by the dense kernels (see dense.hip for
-// the lane layouts).  A 256-thread workgroup computes one 128 x 128 tile of
-// op(A) op(B) with four waves of 64 x 64, staging BK-deep slices of both
-// operands through double-buffered LDS with a register prefetch of the next
-// slice.
+// the lane layouts).  A 256-thread workgroup computes one TT x TT tile
+// (TT = 128, or 64 for launches too small to fill the chip) of op(A) op(B)
+// with four waves, staging BK-deep slices of both operands through
+// double-buffered LDS fed by a register prefetch kPrefetch slices deep.
 #pragma once
 
 #include "common.hpp"
 
 namespace gprx {
+
+// Workgroups per CU the tile kernels are compiled for.
+constexpr int kTileOccupancy = 2;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -57,22 +57,33 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Tile-kernel configuration: K step and LDS row stride (elements).  The
-// stride is 32 dwords past a multiple of 64 banks, so the four 16-lane groups
-// of a fragment read alternate bank halves.
-template <typename T>
+// Tile-kernel configuration for a TT x TT tile: K step and LDS row stride
+// (elements).  The stride is 32 dwords past a multiple of 64 banks, so the
+// four 16-lane groups of a fragment read alternate bank halves.  BK keeps
+// two 16-byte global loads per thread per operand and slice.
+template <typename T, int TT = 128>
 struct Cfg;
 // BK 8 (f64) / 16 (f32) with two workgroups per CU measured fastest on
 // gfx950 among BK in {8, 16, 32} and 1-2 workgroups per CU.
 template <>
-struct Cfg<double> {
+struct Cfg<double, 128> {
   static constexpr int BK = 8;
   static constexpr int LS = 144;
 };
 template <>
-struct Cfg<float> {
+struct Cfg<float, 128> {
   static constexpr int BK = 16;
   static constexpr int LS = 160;
+};
+template <>
+struct Cfg<double, 64> {
+  static constexpr int BK = 16;
+  static constexpr int LS = 80;
+};
+template <>
+struct Cfg<float, 64> {
+  static constexpr int BK = 32;
+  static constexpr int LS = 96;
 };
 
 // Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.
@@ -96,97 +107,118 @@ __device__ __forceinline__ int xcd_remap(int bid, int total) {
 }
 
 
-template <typename T>
+template <typename T, int TT = 128>
 struct TileSmem {
-  __attribute__((aligned(16))) T a[2][Cfg<T>::BK * Cfg<T>::LS];
-  __attribute__((aligned(16))) T b[2][Cfg<T>::BK * Cfg<T>::LS];
+  __attribute__((aligned(16))) T a[2][Cfg<T, TT>::BK * Cfg<T, TT>::LS];
+  __attribute__((aligned(16))) T b[2][Cfg<T, TT>::BK * Cfg<T, TT>::LS];
 };
+
+// Global-load stages in flight per thread: a slice leaves HBM kPrefetch
+// iterations before its MFMAs run, which hides the load latency of a launch
+// too small to have a second workgroup per CU to switch to.
+constexpr int kPrefetch = 3;
 
 // acc += op(A)[ti tile, kb..kb+nk*BK) op(B)[.., tj tile], with Ablk / Bblk
 // the tile origins: A "row-contiguous" operand has element (mn, k) at
-// P[mn + k * ld], a "k-contiguous" one at P[k + mn * ld].
-template <typename T, bool TA, bool TB>
+// P[mn + k * ld], a "k-contiguous" one at P[k + mn * ld].  Four waves in a
+// 2 x 2 grid each own a (TT/2)^2 quarter as FB x FB 16 x 16 MFMA blocks.
+// Slice i sits in register slot i % kPrefetch until it is copied to LDS
+// buffer i % 2; the iteration loop is unrolled by kPrefetch so every slot
+// index is static.
+template <typename T, bool TA, bool TB, int TT = 128>
 __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long lda,
                                               const T* __restrict__ Bblk, long ldb, int kb, int nk,
-                                              TileSmem<T>& sm, typename Acc<T>::V (&acc)[4][4]) {
-  constexpr int BK = Cfg<T>::BK;
-  constexpr int LS = Cfg<T>::LS;
+                                              TileSmem<T, TT>& sm,
+                                              typename Acc<T>::V (&acc)[TT / 32][TT / 32]) {
+  constexpr int BK = Cfg<T, TT>::BK;
+  constexpr int LS = Cfg<T, TT>::LS;
   constexpr int VEC = 16 / sizeof(T);
+  constexpr int P = kPrefetch;
+  constexpr int FB = TT / 32, WT = TT / 2;
   using V = typename Vec<T>::V;
-  constexpr int NV = kTile * BK / VEC / 256;  // 16-byte vectors per thread per operand
+  constexpr int NV = TT * BK / VEC / 256;  // 16-byte vectors per thread per operand
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  V ra[NV], rb[NV];
-  auto gload = [&](int k0) {
+  V ra[P][NV], rb[P][NV];
+  auto gload = [&](int k0, V(&xa)[NV], V(&xb)[NV]) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int e = tid + 256 * v;
       if (!TA) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        ra[v] = *reinterpret_cast<const V*>(Ablk + mn + (long)(k0 + k) * lda);
+        const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
+        xa[v] = *reinterpret_cast<const V*>(Ablk + mn + (long)(k0 + k) * lda);
       } else {
         const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-        ra[v] = *reinterpret_cast<const V*>(Ablk + (k0 + k) + (long)mn * lda);
+        xa[v] = *reinterpret_cast<const V*>(Ablk + (k0 + k) + (long)mn * lda);
       }
       if (TB) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        rb[v] = *reinterpret_cast<const V*>(Bblk + mn + (long)(k0 + k) * ldb);
+        const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
+        xb[v] = *reinterpret_cast<const V*>(Bblk + mn + (long)(k0 + k) * ldb);
       } else {
         const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-        rb[v] = *reinterpret_cast<const V*>(Bblk + (k0 + k) + (long)mn * ldb);
+        xb[v] = *reinterpret_cast<const V*>(Bblk + (k0 + k) + (long)mn * ldb);
       }
     }
   };
-  auto lstore = [&](int buf) {
+  auto lstore = [&](int buf, const V(&xa)[NV], const V(&xb)[NV]) {
     T* a = sm.a[buf];
     T* b = sm.b[buf];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int e = tid + 256 * v;
       if (!TA) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        *reinterpret_cast<V*>(a + k * LS + mn) = ra[v];
+        const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
+        *reinterpret_cast<V*>(a + k * LS + mn) = xa[v];
       } else {
         const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
 #pragma unroll
-        for (int c = 0; c < VEC; ++c) a[(k + c) * LS + mn] = ra[v][c];
+        for (int c = 0; c < VEC; ++c) a[(k + c) * LS + mn] = xa[v][c];
       }
       if (TB) {
-        const int k = e / (kTile / VEC), mn = (e % (kTile / VEC)) * VEC;
-        *reinterpret_cast<V*>(b + k * LS + mn) = rb[v];
+        const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
+        *reinterpret_cast<V*>(b + k * LS + mn) = xb[v];
       } else {
         const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
 #pragma unroll
-        for (int c = 0; c < VEC; ++c) b[(k + c) * LS + mn] = rb[v][c];
+        for (int c = 0; c < VEC; ++c) b[(k + c) * LS + mn] = xb[v][c];
       }
     }
   };
-  if (nk > 0) {
-    gload(kb);
-    lstore(0);
-  }
-  __syncthreads();
-  for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
-    if (it + 1 < nk) gload(kb + (it + 1) * BK);
-    const T* a = sm.a[cur];
-    const T* b = sm.b[cur];
+  auto compute = [&](int buf) {
+    const T* a = sm.a[buf];
+    const T* b = sm.b[buf];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      T af[4], bf[4];
+      T af[FB], bf[FB];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = a[kr * LS + wm * 64 + i * 16 + (lane & 15)];
+      for (int i = 0; i < FB; ++i) af[i] = a[kr * LS + wm * WT + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = b[kr * LS + wn * 64 + j * 16 + (lane & 15)];
+      for (int j = 0; j < FB; ++j) bf[j] = b[kr * LS + wn * WT + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < FB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
+        for (int j = 0; j < FB; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
     }
-    if (it + 1 < nk) lstore(cur ^ 1);
-    __syncthreads();
+  };
+  if (nk <= 0) return;
+#pragma unroll
+  for (int u = 0; u < P; ++u)
+    if (u < nk) gload(kb + u * BK, ra[u], rb[u]);
+  lstore(0, ra[0], rb[0]);
+  __syncthreads();
+  for (int it = 0; it < nk; it += P) {
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      const int i = it + u;
+      if (i >= nk) break;
+      // slot u (slice i) is in LDS already: refill it with slice i + P
+      if (i + P < nk) gload(kb + (i + P) * BK, ra[u], rb[u]);
+      compute(i & 1);
+      if (i + 1 < nk) lstore((i + 1) & 1, ra[(u + 1) % P], rb[(u + 1) % P]);
+      __syncthreads();
+    }
   }
 }
 
@@ -195,9 +227,30 @@ struct TileCoord {
   int ti, tj, kb, ke;
 };
 
+__device__ __forceinline__ void tile_krange(int K, int krange, TileCoord& t, int tt = kTile) {
+  t.kb = 0;
+  t.ke = K;
+  if (krange & 1) t.kb = max(t.kb, t.ti * tt);
+  if (krange & 2) t.kb = max(t.kb, t.tj * tt);
+  if (krange & 4) t.ke = min(t.ke, (t.ti + 1) * tt);
+  if (krange & 8) t.ke = min(t.ke, (t.tj + 1) * tt);
+}
+
+// Tile of this workgroup from a host-built schedule (see tile_schedule in
+// dense.hip); false for a padding slot.
+__device__ __forceinline__ bool tile_from_schedule(const unsigned* sched, int K, int krange,
+                                                   TileCoord& t, int tt = kTile) {
+  const unsigned v = sched[blockIdx.x];
+  if (v == 0xffffffffu) return false;
+  t.ti = (int)(v >> 16);
+  t.tj = (int)(v & 0xffffu);
+  tile_krange(K, krange, t, tt);
+  return true;
+}
+
 __device__ __forceinline__ TileCoord tile_coord(int M, int N, int K, int lower, int order,
-                                                int krange) {
-  const int mt = M / kTile, nt = N / kTile;
+                                                int krange, int tt = kTile) {
+  const int mt = M / tt, nt = N / tt;
   const int total = lower ? mt * (mt + 1) / 2 : mt * nt;
   int bid = xcd_remap(blockIdx.x, total);
   TileCoord t;
@@ -211,12 +264,7 @@ __device__ __forceinline__ TileCoord tile_coord(int M, int N, int K, int lower, 
     t.tj = bid % nt;
     t.ti = bid / nt;
   }
-  t.kb = 0;
-  t.ke = K;
-  if (krange & 1) t.kb = max(t.kb, t.ti * kTile);
-  if (krange & 2) t.kb = max(t.kb, t.tj * kTile);
-  if (krange & 4) t.ke = min(t.ke, (t.ti + 1) * kTile);
-  if (krange & 8) t.ke = min(t.ke, (t.tj + 1) * kTile);
+  tile_krange(K, krange, t, tt);
   return t;
 }
 

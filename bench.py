@@ -12,6 +12,11 @@ Every library call returns after a stream synchronisation, so the host clock
 around K calls brackets exactly K evaluations.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1]
+
+Other workloads of the same path, each printing one JSON line of its own:
+    --workload sgpr    Sgpr (VFE) bound + gradient, n=2^20 d=8 m=1024
+    --workload svgp    Svgp Adam steps, n=2^22 d=16 m=2048 b=4096
+    --workload batch   256 exact problems of uneven n, batched vs looped
 """
 from __future__ import annotations
 
@@ -41,6 +46,21 @@ def exact_flops(n: int, d: int) -> float:
 def inverse_product_flops(n: int) -> float:
     """X^T X over the lower triangle: sum_i (i+1) 2 (n-i) = n(n+1)(n+2)/3."""
     return n * (n + 1) * (n + 2) / 3
+
+
+def sgpr_flops(n: int, m: int) -> float:
+    """Algorithmic FLOP of one VFE bound + gradient: the three m x m x n
+    contractions a minimal formulation needs (K_mn K_mn^T or A A^T, the
+    whitening product and the weight product), 3 m^2 n; the m^3 terms are
+    below 0.5% at n = 1024 m."""
+    return 3.0 * m * m * n
+
+
+def svgp_flops(b: int, m: int) -> float:
+    """Algorithmic FLOP of one SVGP step: 6 m^2 b for the b-long products
+    (A, U = Lq^T A, A U^T) and 7 m^3 / 3 for K_mm's factor and the m x m
+    algebra."""
+    return 6.0 * m * m * b + 7.0 * m ** 3 / 3
 
 
 def rel(a, e):
@@ -92,22 +112,7 @@ def cpu_baseline(wl, cpu_n):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--cpu-n", type=int, default=4096)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-predict", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this path runs on one GPU; exact GPR at n=16384 does not shard "
-              "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
-        sys.exit(2)
-
-    import gprx_amd as gx
+def bench_exact(args, gx):
     from gprx_amd.configs import exact_ard
 
     wl = exact_ard(n=args.n)
@@ -122,7 +127,8 @@ def main():
     evals32 = args.steps / wall32
     F = exact_flops(wl.n, wl.d)
     lauum = inverse_product_flops(wl.n)
-    # Dominant kernel: the X^T X tile-GEMM launch (one launch per evaluation).
+    # Dominant kernel: the X^T X tile-GEMM launch (one launch per evaluation),
+    # timed with HIP events on the model's stream.
     ach32 = lauum / (ph32[3] * 1e-3) / 1e12
     ach64 = lauum / (ph64[3] * 1e-3) / 1e12
 
@@ -163,17 +169,181 @@ def main():
                      "mixed": [round(v, 3) for v in ph32], "fp64": [round(v, 3) for v in ph64]},
         "eval_tflops": {"mixed": F / (wall32 / args.steps) / 1e12, "fp64": F / (wall64 / args.steps) / 1e12,
                         "algorithmic_flop_per_eval": F},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm<float,T,N> (X^T X, one launch per eval)",
+        "roofline": {"bound": "mfma", "kernel": "k_gemm<float,T,N,128> (X^T X, one launch per eval)",
                      "achieved": ach32, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
                      "traffic": None},
-        "roofline_fp64": {"bound": "mfma", "kernel": "k_gemm<double,T,N> (X^T X)", "achieved": ach64,
+        "roofline_fp64": {"bound": "mfma", "kernel": "k_gemm<double,T,N,128> (X^T X)", "achieved": ach64,
                           "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
         **extra,
     }
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline (oracle, n={args.cpu_n})", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_n)
-    print(json.dumps(out))
+    return out
+
+
+def _timed(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = fn()
+    return r, time.perf_counter() - t0
+
+
+def bench_sgpr(args, gx):
+    from gprx_amd.configs import sgpr_vfe
+
+    wl = sgpr_vfe()
+    th = wl.theta()
+    res = {}
+    for name, prec in (("fp64", gx.Precision.FP64), ("mixed", gx.Precision.MIXED)):
+        print(f"[bench] sgpr n={wl.n} m={wl.m}: {name}", file=sys.stderr, flush=True)
+        f = gx.Sgpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).with_precision(prec).prepare(
+            wl.x, wl.n, wl.d, wl.y, wl.z, wl.m)
+        grad = np.empty(th.size)
+        val, wall = _timed(lambda: f.value_and_gradient_into(th, grad), args.steps, args.warmup)
+        res[name] = (val, grad.copy(), wall)
+        f.close()
+    F = sgpr_flops(wl.n, wl.m)
+    v64, g64, w64 = res["fp64"]
+    v32, g32, w32 = res["mixed"]
+    ach = F / (w32 / args.steps) / 1e12
+    ach64 = F / (w64 / args.steps) / 1e12
+    return {
+        "metric": "VFE bound+grad evals/s, Sgpr n=2^20 d=8 m=1024 Constant x RbfArd + Periodic",
+        "value": args.steps / w32, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": w32 / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed (f64 m x m algebra)",
+        "data": "synthetic (seeded SeededRng, seed 3)",
+        "config": {"workload": "Sgpr VFE bound+grad", "n": wl.n, "d": wl.d, "m": wl.m,
+                   "params": int(th.size), "parallelism": "single GPU"},
+        "fp64_evals_per_s": args.steps / w64, "mixed_vs_fp64": w64 / w32,
+        "mixed_rel_err_vs_fp64": {"bound": rel(v32, v64), "grad_max": rel(g32, g64)},
+        "roofline": {"bound": "mfma", "kernel": "whole evaluation (algorithmic 3 m^2 n)",
+                     "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                     "traffic": None},
+        "roofline_fp64": {"bound": "mfma", "kernel": "whole evaluation", "achieved": ach64,
+                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
+    }
+
+
+def bench_svgp(args, gx):
+    from gprx_amd.configs import svgp_minibatch
+
+    wl = svgp_minibatch()
+    th = wl.theta()
+    b, lr = 4096, 0.01
+    steps = args.steps
+    res = {}
+    for name, prec in (("fp64", gx.Precision.FP64), ("mixed", gx.Precision.MIXED)):
+        print(f"[bench] svgp n={wl.n} m={wl.m} b={b}: {name}, {steps} steps", file=sys.stderr, flush=True)
+        f = gx.Svgp(wl.kernel(), gx.GaussianLikelihood(wl.noise)).with_precision(prec).prepare(
+            wl.x, wl.n, wl.d, wl.y, wl.z, wl.m)
+        p0 = f.params.copy()
+        if args.warmup:
+            f.adam(p0, steps=args.warmup, batch_size=b, lr=lr, seed=0)
+        t0 = time.perf_counter()
+        p, vals = f.adam(p0, steps=steps, batch_size=b, lr=lr, seed=0)
+        res[name] = (vals, time.perf_counter() - t0)
+        f.close()
+    (v64, w64), (v32, w32) = res["fp64"], res["mixed"]
+    F = svgp_flops(b, wl.m)
+    ach = F / (w32 / steps) / 1e12
+    ach64 = F / (w64 / steps) / 1e12
+    traj = np.abs(v32 - v64) / np.maximum(np.abs(v64), 1e-300)
+    return {
+        "metric": "Adam steps/s, Svgp n=2^22 d=16 m=2048 b=4096 ARD-RBF",
+        "value": steps / w32, "unit": "steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": w32 / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed (f64 m x m algebra)",
+        "data": "synthetic (seeded SeededRng, seed 4)",
+        "config": {"workload": "Svgp minibatch Adam, lr 0.01", "n": wl.n, "d": wl.d, "m": wl.m,
+                   "batch": b, "params": int(len(gx.prior_params(th, wl.m))),
+                   "parallelism": "single GPU"},
+        "fp64_steps_per_s": steps / w64, "mixed_vs_fp64": w64 / w32,
+        "neg_elbo": {"fp64_first": float(v64[0]), "fp64_last": float(v64[-1]),
+                     "mixed_last": float(v32[-1])},
+        "mixed_elbo_traj_rel_err": {"max": float(traj.max()), "last": float(traj[-1])},
+        "roofline": {"bound": "mfma", "kernel": "whole step (algorithmic 6 m^2 b + 7 m^3 / 3)",
+                     "achieved": ach, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                     "traffic": None},
+        "roofline_fp64": {"bound": "mfma", "kernel": "whole step", "achieved": ach64,
+                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
+    }
+
+
+def bench_batch(args, gx):
+    from gprx_amd.configs import batched_problems
+
+    raw = batched_problems(k=256)
+    probs = [(gx.RbfArdKernel(list(ls)), gx.GaussianLikelihood(noise), x, n, d, y)
+             for (n, d, x, y, ls, noise) in raw]
+    thetas = [np.concatenate([np.log(ls), [np.log(noise)]]) for (n, d, x, y, ls, noise) in raw]
+    F = sum(exact_flops(n, d) for (n, d, *_r) in raw)
+    out_forms = {}
+    for name, prec in (("fp64", gx.Precision.FP64), ("mixed", gx.Precision.MIXED)):
+        print(f"[bench] batch of {len(probs)}: {name} batched", file=sys.stderr, flush=True)
+        bt = gx.GprBatch(probs, precision=prec)
+        (nl, gr, er), wall_b = _timed(lambda: bt.value_and_gradient(thetas), args.steps, args.warmup)
+        bt.close()
+        print(f"[bench] batch of {len(probs)}: {name} looped single-problem path", file=sys.stderr,
+              flush=True)
+        models = [gx.Gpr(k, lik).with_precision(prec).prepare(x, n, d, y) for (k, lik, x, n, d, y) in probs]
+        grads = [np.empty(t.size) for t in thetas]
+
+        def loop():
+            return [mdl.value_and_gradient_into(t, g_) for mdl, t, g_ in zip(models, thetas, grads)]
+
+        vals, wall_l = _timed(loop, args.steps, args.warmup)
+        for mdl in models:
+            mdl.close()
+        err_nl = max(abs(a - b) / max(abs(b), 1.0) for a, b in zip(nl, vals))
+        err_g = max(rel(a, b) for a, b in zip(gr, grads))
+        out_forms[name] = {"batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
+                           "speedup": wall_l / wall_b, "failed": sum(e is not None for e in er),
+                           "batched_vs_looped_rel_err": {"nlml": err_nl, "grad": err_g}}
+    fb, fm = out_forms["fp64"], out_forms["mixed"]
+    ach = F / (fm["batched_ms"] * 1e-3) / 1e12
+    ach64 = F / (fb["batched_ms"] * 1e-3) / 1e12
+    return {
+        "metric": "batch evals/s, 256 exact ARD-RBF problems n in [256, 4096], batched vs looped",
+        "value": 1e3 / fm["batched_ms"], "unit": "batch evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fm["batched_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed",
+        "data": "synthetic (seeded SeededRng, seed 5)",
+        "config": {"workload": "batched exact GPR NLML+grad", "problems": len(probs),
+                   "sum_n": int(sum(p[3] for p in probs)), "parallelism": "single GPU"},
+        "fp64": fb, "mixed": fm, "algorithmic_flop_per_batch": F,
+        "roofline": {"bound": "mfma", "kernel": "whole batched evaluation", "achieved": ach,
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None},
+        "roofline_fp64": {"bound": "mfma", "kernel": "whole batched evaluation", "achieved": ach64,
+                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch"], default="exact")
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-predict", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this path runs on one GPU; exact GPR at n=16384 does not shard "
+              "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3}[args.workload]
+
+    import gprx_amd as gx
+
+    fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch}[args.workload]
+    print(json.dumps(fn(args, gx)))
 
 
 if __name__ == "__main__":

@@ -15,6 +15,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libgprx_hip.so"
+RNG_SRC = HERE / "host" / "seeded_rng.cpp"
+RNG_LIB = HERE / "libgprx_rng.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -60,7 +62,19 @@ def build(force: bool = False, jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_rng()
     return LIB
+
+
+def build_rng() -> Path:
+    """Host-only SeededRng used to draw the synthetic workloads (g++)."""
+    tmp = RNG_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-o", str(tmp), str(RNG_SRC)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed on {RNG_SRC.name}:\n{res.stderr}")
+    os.replace(tmp, RNG_LIB)
+    return RNG_LIB
 
 
 if __name__ == "__main__":

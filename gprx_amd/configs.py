@@ -80,3 +80,78 @@ def batched_problems(k: int = 256, seed: int = 5, n_lo: int = 256, n_hi: int = 4
         y = f + math.sqrt(noise) * r.normals(n)
         out.append((n, d, x, y, ls, noise))
     return out
+
+
+@dataclass
+class SparseWorkload:
+    n: int
+    d: int
+    m: int
+    x: np.ndarray  # column-major n x d
+    y: np.ndarray
+    z: np.ndarray  # column-major m x d, rows of x
+    kernel_fn: object
+    theta0: np.ndarray  # kernel log-parameters then log noise
+
+    def kernel(self):
+        return self.kernel_fn()
+
+    def theta(self) -> np.ndarray:
+        return self.theta0.copy()
+
+    @property
+    def noise(self) -> float:
+        return math.exp(self.theta0[-1])
+
+
+def _subset_rows(r: SeededRng, x: np.ndarray, n: int, d: int, m: int) -> np.ndarray:
+    """m distinct rows of x by a partial Fisher-Yates draw from the stream."""
+    idx = np.arange(n)
+    for i in range(m):
+        j = i + r.up_to(n - 1 - i)
+        idx[i], idx[j] = idx[j], idx[i]
+    X = x.reshape(d, n)
+    return np.ascontiguousarray(X[:, idx[:m]]).reshape(-1)
+
+
+def sgpr_vfe(n: int = 1 << 20, d: int = 8, m: int = 1024, seed: int = 3) -> SparseWorkload:
+    """Sgpr (VFE), Constant x RbfArd + Periodic: x ~ N(0, I_d), w ~ N(0, I/d),
+    y = tanh(w'x) + 0.3 sin(4 x0) + 0.1 N(0,1); Z = m seeded rows of x.
+    Initial parameters c = 1, l_k = 1, periodic l = 1 with period 20 (a
+    shorter period makes the periodic Gram of 8-D Euclidean distances
+    indefinite), sigma^2 = 0.01; kappa(K_mm) is about 1.7e4."""
+    from .kernels import ConstantKernel, PeriodicKernel, RbfArdKernel
+
+    r = SeededRng(seed)
+    x = r.normals(n * d)
+    w = r.normals(d) / math.sqrt(d)
+    eps = r.normals(n)
+    X = x.reshape(d, n)
+    y = np.tanh(w @ X) + 0.3 * np.sin(4 * X[0]) + 0.1 * eps
+    z = _subset_rows(r, x, n, d, m)
+
+    def kern():
+        return ConstantKernel(1.0) * RbfArdKernel([1.0] * d) + PeriodicKernel(1.0, 20.0)
+
+    th = np.concatenate([kern().get_params(), [math.log(0.01)]])
+    return SparseWorkload(n, d, m, x, y, z, kern, th)
+
+
+def svgp_minibatch(n: int = 1 << 22, d: int = 16, m: int = 2048, seed: int = 4) -> SparseWorkload:
+    """Svgp, RbfArd: x ~ U[-1,1]^d, y = sum_k x_k^2 / d + 0.2 cos(pi x0) +
+    0.1 N(0,1); Z = m seeded rows of x.  Initial l_k = 1.5 (kappa(K_mm) about
+    9.5e3 at m = 2048), sigma^2 = 0.01."""
+    from .kernels import RbfArdKernel
+
+    r = SeededRng(seed)
+    x = -1.0 + 2.0 * r.units(n * d)
+    eps = r.normals(n)
+    X = x.reshape(d, n)
+    y = (X ** 2).sum(axis=0) / d + 0.2 * np.cos(math.pi * X[0]) + 0.1 * eps
+    z = _subset_rows(r, x, n, d, m)
+
+    def kern():
+        return RbfArdKernel([1.5] * d)
+
+    th = np.concatenate([kern().get_params(), [math.log(0.01)]])
+    return SparseWorkload(n, d, m, x, y, z, kern, th)

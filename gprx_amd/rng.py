@@ -2,15 +2,35 @@
 
 Used to draw the synthetic benchmark inputs exactly as the reference's
 seeded configurations define them.  The first outputs for seeds 0 and 42 are
-pinned by src/rng.rs:105-116 and tested in tests/test_rng.py.
+pinned by src/rng.rs:105-116 and tested in tests/test_rng.py.  Bulk draws
+(units, normals) go through the same generator in libgprx_rng.so when it is
+built, which yields the identical stream at native speed.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
+from pathlib import Path
 
 import numpy as np
 
 _M64 = (1 << 64) - 1
+_RNG_LIB_PATH = Path(__file__).resolve().parent / "libgprx_rng.so"
+_rng_lib = None
+
+
+def _native():
+    global _rng_lib
+    if _rng_lib is None and _RNG_LIB_PATH.exists():
+        lib = C.CDLL(str(_RNG_LIB_PATH))
+        u64p = C.POINTER(C.c_uint64)
+        dp = C.POINTER(C.c_double)
+        lib.gprx_rng_units.argtypes = [u64p, dp, C.c_size_t]
+        lib.gprx_rng_units.restype = None
+        lib.gprx_rng_normals.argtypes = [u64p, dp, C.c_size_t]
+        lib.gprx_rng_normals.restype = None
+        _rng_lib = lib
+    return _rng_lib
 
 
 def _rotl(x: int, k: int) -> int:
@@ -45,7 +65,22 @@ class SeededRng:
     def unit(self) -> float:
         return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
 
-    def units(self, n: int) -> np.ndarray:
+    def _bulk(self, fn_name: str, n: int):
+        lib = _native()
+        if lib is None:
+            return None
+        st = np.array(self.s, dtype=np.uint64)
+        out = np.empty(n, dtype=np.float64)
+        getattr(lib, fn_name)(st.ctypes.data_as(C.POINTER(C.c_uint64)),
+                              out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        self.s = [int(v) for v in st]
+        return out
+
+    def units(self, n: int, native: bool = True) -> np.ndarray:
+        if native and n > 64:
+            out = self._bulk("gprx_rng_units", n)
+            if out is not None:
+                return out
         return np.fromiter((self.unit() for _ in range(n)), dtype=np.float64, count=n)
 
     def open_unit(self) -> float:
@@ -58,7 +93,11 @@ class SeededRng:
         u2 = self.open_unit()
         return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
 
-    def normals(self, n: int) -> np.ndarray:
+    def normals(self, n: int, native: bool = True) -> np.ndarray:
+        if native and n > 64:
+            out = self._bulk("gprx_rng_normals", n)
+            if out is not None:
+                return out
         return np.fromiter((self.unit_normal() for _ in range(n)), dtype=np.float64, count=n)
 
     def up_to(self, mx: int) -> int:

@@ -25,3 +25,19 @@ def test_matches_oracle_stream():
     a.shuffle(i1)
     b.shuffle(i2)
     assert np.array_equal(i1, i2)
+
+
+def test_native_bulk_draws_match_python_stream():
+    """libgprx_rng.so continues the same stream as the Python generator:
+    interleaved bulk and scalar draws agree bit for bit."""
+    import gprx_amd.rng as grng
+
+    if grng._native() is None:
+        import pytest
+
+        pytest.skip("libgprx_rng.so not built")
+    a, b = SeededRng(3), SeededRng(3)
+    for n in (1000, 77, 4096):
+        assert np.array_equal(a.units(n), b.units(n, native=False))
+        assert np.array_equal(a.normals(n), b.normals(n, native=False))
+        assert a.next_u64() == b.next_u64()

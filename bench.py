@@ -17,6 +17,8 @@ Other workloads of the same path, each printing one JSON line of its own:
     --workload sgpr    Sgpr (VFE) bound + gradient, n=2^20 d=8 m=1024
     --workload svgp    Svgp Adam steps, n=2^22 d=16 m=2048 b=4096
     --workload batch   256 exact problems of uneven n, batched vs looped
+    --workload copy    a plain 1 GiB device copy (calibrates the HBM byte
+                       counters of rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
 """
 from __future__ import annotations
 
@@ -35,6 +37,11 @@ sys.path.insert(0, HERE)
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 PEAK_HBM = 8000.0  # GB/s
+
+# HBM-side bytes per X^T X launch at n = 16384: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on that launch, each scaled by what the counter reads for a 1 GiB
+# device copy (x2.0 and x1.0); the run is summarised under profiles/.
+XTX_TRAFFIC_N16384 = {"mixed": 4.222e10, "fp64": 8.548e10}
 
 
 def exact_flops(n: int, d: int) -> float:
@@ -171,9 +178,10 @@ def bench_exact(args, gx):
                         "algorithmic_flop_per_eval": F},
         "roofline": {"bound": "mfma", "kernel": "k_gemm<float,T,N,128> (X^T X, one launch per eval)",
                      "achieved": ach32, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
-                     "traffic": None},
+                     "traffic": XTX_TRAFFIC_N16384["mixed"] if wl.n == 16384 else None},
         "roofline_fp64": {"bound": "mfma", "kernel": "k_gemm<double,T,N,128> (X^T X)", "achieved": ach64,
-                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
+                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64,
+                          "traffic": XTX_TRAFFIC_N16384["fp64"] if wl.n == 16384 else None},
         **extra,
     }
     if not args.no_cpu_baseline:
@@ -282,6 +290,12 @@ def bench_batch(args, gx):
     thetas = [np.concatenate([np.log(ls), [np.log(noise)]]) for (n, d, x, y, ls, noise) in raw]
     F = sum(exact_flops(n, d) for (n, d, *_r) in raw)
     out_forms = {}
+    ref = None  # FP64 batched results: the reference the mixed form is held to
+
+    def err1(a, e):
+        a, e = np.asarray(a, float), np.asarray(e, float)
+        return float(np.max(np.abs(a - e) / np.maximum(np.abs(e), 1.0)))
+
     for name, prec in (("fp64", gx.Precision.FP64), ("mixed", gx.Precision.MIXED)):
         print(f"[bench] batch of {len(probs)}: {name} batched", file=sys.stderr, flush=True)
         bt = gx.GprBatch(probs, precision=prec)
@@ -296,13 +310,22 @@ def bench_batch(args, gx):
             return [mdl.value_and_gradient_into(t, g_) for mdl, t, g_ in zip(models, thetas, grads)]
 
         vals, wall_l = _timed(loop, args.steps, args.warmup)
+        fallbacks = sum(mdl.last_fallback() for mdl in models)
         for mdl in models:
             mdl.close()
-        err_nl = max(abs(a - b) / max(abs(b), 1.0) for a, b in zip(nl, vals))
-        err_g = max(rel(a, b) for a, b in zip(gr, grads))
-        out_forms[name] = {"batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
-                           "speedup": wall_l / wall_b, "failed": sum(e is not None for e in er),
-                           "batched_vs_looped_rel_err": {"nlml": err_nl, "grad": err_g}}
+        form = {"batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
+                "speedup": wall_l / wall_b, "failed": sum(e is not None for e in er),
+                "looped_fp64_fallbacks": int(fallbacks),
+                "batched_vs_looped_err": {"nlml": err1(nl, vals),
+                                          "grad": max(err1(a, b) for a, b in zip(gr, grads))}}
+        if ref is None:
+            ref = (np.asarray(nl), [g_.copy() for g_ in gr])
+        else:
+            form["batched_vs_fp64_err"] = {"nlml": err1(nl, ref[0]),
+                                           "grad": max(err1(a, b) for a, b in zip(gr, ref[1]))}
+            form["looped_vs_fp64_err"] = {"nlml": err1(vals, ref[0]),
+                                          "grad": max(err1(a, b) for a, b in zip(grads, ref[1]))}
+        out_forms[name] = form
     fb, fm = out_forms["fp64"], out_forms["mixed"]
     ach = F / (fm["batched_ms"] * 1e-3) / 1e12
     ach64 = F / (fb["batched_ms"] * 1e-3) / 1e12
@@ -322,12 +345,36 @@ def bench_batch(args, gx):
     }
 
 
+def bench_copy(args, gx):
+    """One 1 GiB device-to-device copy per step (torch's copy kernel): known
+    HBM traffic of 1 GiB read + 1 GiB written per launch."""
+    import torch
+
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gbs = 2 * n * 8 * args.steps / wall / 1e9
+    return {"metric": "device copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "copy", "bytes_read_per_launch": n * 8,
+                                            "bytes_written_per_launch": n * 8}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch"], default="exact")
+    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "copy"], default="exact")
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--cpu-n", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -338,11 +385,12 @@ def main():
               "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3}[args.workload]
+        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "copy": 5}[args.workload]
 
     import gprx_amd as gx
 
-    fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch}[args.workload]
+    fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch,
+          "copy": bench_copy}[args.workload]
     print(json.dumps(fn(args, gx)))
 
 

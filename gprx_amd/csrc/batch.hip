@@ -178,10 +178,10 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
   double xi[DP];
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
-  double acc[kMaxLeaves][kAccPerLeaf];
+  double acc[kMaxSlots][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   const T* M = Ainv + pd.moff;

@@ -18,44 +18,54 @@ namespace gprx {
 // Kernel program in sum-of-products form.
 //
 // The C ABI takes gprx's KernelSpec tree (src/kernel/spec.rs:106-132) as a
-// postfix list.  The host expands it into a sum of products of leaves, which
-// the fill and contraction kernels evaluate per matrix entry with every index
-// known at compile time (registers only, no scratch):
-//     K(a, b) = sum_t prod_{l in term t} v_l(a, b)
-// A leaf may appear in several terms after expansion ((k1 + k2) * k3); its
-// parameters still live once in theta, so gradients add up across terms.
+// postfix list.  The host expands it into a sum of products of leaves,
+//     K(a, b) = sum_t prod_{l in term t} v_l(a, b),
+// and lays the factors out term by term as "slots": slot s holds one leaf
+// occurrence, the slots of a term are contiguous, and bitmasks mark where
+// terms open and close.  A leaf shared by several terms ((k1 + k2) * k3)
+// occupies one slot per term.  Per matrix entry the kernels then need one
+// forward sweep (running product of the open term: K) and, for gradients,
+// one backward sweep (each slot's full term product, d v_s / d log theta_s =
+// v_s f_s, so d K / d log theta_s = term product x f_s), every index known
+// at compile time (registers only, no scratch).  Gradients of a leaf add up
+// over its slots on the host.
 // ---------------------------------------------------------------------------
 constexpr int kMaxLeaves = 8;
 constexpr int kMaxTerms = 8;
+constexpr int kMaxSlots = 8;
 constexpr int kMaxDim = 32;
 constexpr int kTile = 128;  // tile edge of every dense kernel
 
 enum LeafKind : int32_t { kLeafConst = 0, kLeafRbf = 1, kLeafArd = 2, kLeafPeriodic = 3 };
 
-// Per-leaf slots of the contraction accumulators: two per leaf plus one per
-// ARD dimension (the single ARD leaf of a program).
+// Contraction accumulators: two per slot plus one per ARD dimension (the
+// single ARD leaf of a program, summed over its slots).
 constexpr int kAccPerLeaf = 2;
-constexpr int kMaxAcc = kMaxLeaves * kAccPerLeaf + kMaxDim;
+constexpr int kMaxAcc = kMaxSlots * kAccPerLeaf + kMaxDim;
 
 struct DevProgram {
-  int32_t nl;        // leaves
-  int32_t nt;        // terms
-  int32_t d;         // true input dimension (<= padded dimension)
-  int32_t ard_leaf;  // the ARD leaf, or -1
-  int32_t kind[kMaxLeaves];
-  uint32_t mask[kMaxTerms];  // bit l set: leaf l is a factor of term t
-  double a[kMaxLeaves];      // const: c; rbf: 1/l^2; periodic: pi/p
-  double b[kMaxLeaves];      // periodic: 1/l^2
-  double ard_w[kMaxDim];     // 1/l_d^2 of the ARD leaf, 0 beyond d
-  double noise;              // sigma^2 + jitter on the training diagonal
+  int32_t ns;       // slots
+  int32_t d;        // true input dimension (<= padded dimension)
+  uint32_t start;   // bit s: slot s opens a term
+  uint32_t end;     // bit s: slot s closes a term
+  uint32_t ard;     // bit s: slot s holds the ARD leaf
+  int32_t kind[kMaxSlots];
+  double a[kMaxSlots];     // const: c; rbf: 1/l^2; periodic: pi/p
+  double b[kMaxSlots];     // periodic: 1/l^2
+  double ard_w[kMaxDim];   // 1/l_d^2 of the ARD leaf, 0 beyond d
+  double noise;            // sigma^2 + jitter on the training diagonal
 };
 
-// Host-side description of a program: leaves in depth-first order, terms.
+// Host-side description of a program: leaves in depth-first order, terms,
+// and the slot layout.
 struct HostProgram {
   int nl = 0, nt = 0, d = 0, ard_leaf = -1;
   int kind[kMaxLeaves];
   int param_off[kMaxLeaves];  // first theta index of each leaf
   std::vector<uint32_t> terms;
+  int ns = 0;
+  int slot_leaf[kMaxSlots];
+  uint32_t start = 0, end = 0;
   int n_params = 0;  // kernel parameters (theta without log sigma^2)
 };
 
@@ -67,8 +77,8 @@ int parse_program(const gprx_kernel_program* prog, int d, HostProgram* out, std:
 void make_dev_program(const HostProgram& hp, const double* log_theta, double diag_noise,
                       DevProgram* out);
 
-// Number of contraction accumulators used by a program (2 per leaf + d).
-inline int n_acc(const HostProgram& hp) { return hp.nl * kAccPerLeaf + (hp.ard_leaf >= 0 ? hp.d : 0); }
+// Number of contraction accumulators used by a program (2 per slot + d).
+inline int n_acc(const HostProgram& hp) { return hp.ns * kAccPerLeaf + (hp.ard_leaf >= 0 ? hp.d : 0); }
 
 // Maps contraction accumulators (sums of G_l * f) into d<sum W dK>/d(log theta),
 // kernel parameters only.  acc has n_acc(hp) entries.

@@ -112,33 +112,52 @@ int parse_program(const gprx_kernel_program* prog, int d, HostProgram* out, std:
   hp.terms = stack[0].terms;
   hp.nt = (int)hp.terms.size();
   hp.n_params = poff;
+  // Slots: the factors of each term, term after term.
+  for (uint32_t mask : hp.terms) {
+    bool first = true;
+    for (int l = 0; l < hp.nl; ++l) {
+      if (!((mask >> l) & 1u)) continue;
+      if (hp.ns >= kMaxSlots) {
+        *why = "kernel program expands to more than 8 leaf factors over its product terms";
+        return GPRX_UNSUPPORTED;
+      }
+      if (first) hp.start |= 1u << hp.ns;
+      first = false;
+      hp.slot_leaf[hp.ns++] = l;
+    }
+    hp.end |= 1u << (hp.ns - 1);
+  }
   *out = hp;
   return GPRX_OK;
 }
 
 void make_dev_program(const HostProgram& hp, const double* th, double diag_noise, DevProgram* P) {
   std::memset(P, 0, sizeof(*P));
-  P->nl = hp.nl;
-  P->nt = hp.nt;
+  P->ns = hp.ns;
   P->d = hp.d;
-  P->ard_leaf = hp.ard_leaf;
-  for (int t = 0; t < hp.nt; ++t) P->mask[t] = hp.terms[t];
-  for (int l = 0; l < hp.nl; ++l) {
-    P->kind[l] = hp.kind[l];
+  P->start = hp.start;
+  P->end = hp.end;
+  if (hp.ard_leaf >= 0) {
+    const double* q = th + hp.param_off[hp.ard_leaf];
+    for (int k = 0; k < hp.d; ++k) P->ard_w[k] = std::exp(-2.0 * q[k]);
+  }
+  for (int s = 0; s < hp.ns; ++s) {
+    const int l = hp.slot_leaf[s];
+    P->kind[s] = hp.kind[l];
     const double* q = th + hp.param_off[l];
     switch (hp.kind[l]) {
       case kLeafConst:
-        P->a[l] = std::exp(q[0]);
+        P->a[s] = std::exp(q[0]);
         break;
       case kLeafRbf:
-        P->a[l] = std::exp(-2.0 * q[0]);
+        P->a[s] = std::exp(-2.0 * q[0]);
         break;
       case kLeafArd:
-        for (int k = 0; k < hp.d; ++k) P->ard_w[k] = std::exp(-2.0 * q[k]);
+        P->ard |= 1u << s;
         break;
       default:
-        P->a[l] = M_PI * std::exp(-q[1]);
-        P->b[l] = std::exp(-2.0 * q[0]);
+        P->a[s] = M_PI * std::exp(-q[1]);
+        P->b[s] = std::exp(-2.0 * q[0]);
         break;
     }
   }
@@ -146,27 +165,32 @@ void make_dev_program(const HostProgram& hp, const double* th, double diag_noise
 }
 
 void acc_to_grad(const HostProgram& hp, const double* th, const double* acc, double* grad) {
-  for (int l = 0; l < hp.nl; ++l) {
+  for (int k = 0; k < hp.n_params; ++k) grad[k] = 0.0;
+  for (int s = 0; s < hp.ns; ++s) {
+    const int l = hp.slot_leaf[s];
     const int o = hp.param_off[l];
-    const double a0 = acc[kAccPerLeaf * l], a1 = acc[kAccPerLeaf * l + 1];
+    const double a0 = acc[kAccPerLeaf * s], a1 = acc[kAccPerLeaf * s + 1];
     switch (hp.kind[l]) {
       case kLeafConst:
-        grad[o] = a0;
+        grad[o] += a0;
         break;
       case kLeafRbf:
-        grad[o] = a0 * std::exp(-2.0 * th[o]);
+        grad[o] += a0 * std::exp(-2.0 * th[o]);
         break;
       case kLeafArd:
-        for (int k = 0; k < hp.d; ++k)
-          grad[o + k] = acc[kMaxLeaves * kAccPerLeaf + k] * std::exp(-2.0 * th[o + k]);
-        break;
+        break;  // summed over its slots on the device
       default: {
         const double b4 = 4.0 * std::exp(-2.0 * th[o]);
-        grad[o] = a0 * b4;
-        grad[o + 1] = a1 * b4;
+        grad[o] += a0 * b4;
+        grad[o + 1] += a1 * b4;
         break;
       }
     }
+  }
+  if (hp.ard_leaf >= 0) {
+    const int o = hp.param_off[hp.ard_leaf];
+    for (int k = 0; k < hp.d; ++k)
+      grad[o + k] = acc[kMaxSlots * kAccPerLeaf + k] * std::exp(-2.0 * th[o + k]);
   }
 }
 
@@ -294,10 +318,10 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
   const double ai = ar[i];
-  double acc[kMaxLeaves][kAccPerLeaf];
+  double acc[kMaxSlots][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   if (gi < n) {
@@ -339,10 +363,10 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
   double xi[DP];
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
-  double acc[kMaxLeaves][kAccPerLeaf];
+  double acc[kMaxSlots][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   if (gi < na) {
@@ -367,10 +391,10 @@ __global__ __launch_bounds__(256) void k_contract_diag(const DevProgram* __restr
                                                        double w, double* __restrict__ partial) {
   const DevProgram& P = *Pg;
   __shared__ double red[4 * kMaxAcc];
-  double acc[kMaxLeaves][kAccPerLeaf];
+  double acc[kMaxSlots][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   // k(x, x) does not depend on x for these stationary leaves: one entry

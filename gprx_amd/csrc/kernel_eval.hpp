@@ -1,5 +1,5 @@
 // Device-side evaluation of a kernel program at one matrix entry, and the
-// per-leaf contraction accumulators shared by the fill, contraction and
+// per-slot contraction accumulators shared by the fill, contraction and
 // fused GEMM-contraction kernels.
 #pragma once
 
@@ -11,11 +11,13 @@ namespace gprx {
 // Device: one entry of the program.
 // ---------------------------------------------------------------------------
 struct Leaves {
-  double v[kMaxLeaves];
-  double f0[kMaxLeaves];
-  double f1[kMaxLeaves];
+  double f0[kMaxSlots];  // d v_s / d log theta = v_s * (feature) per slot
+  double f1[kMaxSlots];
+  double tp[kMaxSlots];  // product of the slot's term up to and including it
 };
 
+// K at one entry (dx = a - b over the padded dimensions): leaf values slot by
+// slot, the running product of the open term, and the sum over closed terms.
 template <int DP, bool GRAD>
 __device__ __forceinline__ double eval_entry(const DevProgram& P, const double (&dx)[DP], Leaves& L) {
   double s = 0.0, r2 = 0.0;
@@ -25,78 +27,60 @@ __device__ __forceinline__ double eval_entry(const DevProgram& P, const double (
     s += t;
     r2 += t * P.ard_w[q];
   }
+  double k = 0.0, rp = 1.0;
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) {
-    L.v[l] = 1.0;
+  for (int sl = 0; sl < kMaxSlots; ++sl) {
+    if (sl >= P.ns) break;
+    const int kd = P.kind[sl];
+    double v, f0 = 0.0, f1 = 0.0;
+    if (kd == kLeafConst) {
+      v = P.a[sl];
+      f0 = 1.0;
+    } else if (kd == kLeafRbf) {
+      v = exp(-0.5 * s * P.a[sl]);
+      f0 = s;
+    } else if (kd == kLeafArd) {
+      v = exp(-0.5 * r2);
+    } else {
+      const double r = sqrt(s);
+      const double al = r * P.a[sl];
+      double sn, cs;
+      sincos(al, &sn, &cs);
+      v = exp(-2.0 * sn * sn * P.b[sl]);
+      f0 = sn * sn;
+      f1 = sn * cs * al;
+    }
+    rp = ((P.start >> sl) & 1u) ? v : rp * v;
+    L.tp[sl] = rp;
     if (GRAD) {
-      L.f0[l] = 0.0;
-      L.f1[l] = 0.0;
+      L.f0[sl] = f0;
+      L.f1[sl] = f1;
     }
-    if (l < P.nl) {
-      const int kd = P.kind[l];
-      if (kd == kLeafConst) {
-        L.v[l] = P.a[l];
-        if (GRAD) L.f0[l] = 1.0;
-      } else if (kd == kLeafRbf) {
-        L.v[l] = exp(-0.5 * s * P.a[l]);
-        if (GRAD) L.f0[l] = s;
-      } else if (kd == kLeafArd) {
-        L.v[l] = exp(-0.5 * r2);
-      } else {
-        const double r = sqrt(s);
-        const double al = r * P.a[l];
-        double sn, cs;
-        sincos(al, &sn, &cs);
-        L.v[l] = exp(-2.0 * sn * sn * P.b[l]);
-        if (GRAD) {
-          L.f0[l] = sn * sn;
-          L.f1[l] = sn * cs * al;
-        }
-      }
-    }
-  }
-  double k = 0.0;
-#pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
-    if (t < P.nt) {
-      double pr = 1.0;
-#pragma unroll
-      for (int l = 0; l < kMaxLeaves; ++l)
-        if ((P.mask[t] >> l) & 1u) pr *= L.v[l];
-      k += pr;
-    }
+    if ((P.end >> sl) & 1u) k += rp;
   }
   return k;
 }
 
-// Adds w * dK/d(leaf features) of one entry to the accumulators.
+// Adds w * dK/d(slot features) of one entry to the accumulators: sweeping
+// the slots backwards, each term's full product (held by its last slot) is
+// carried to the slots before it.
 template <int DP>
 __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const double (&dx)[DP],
                                                  const Leaves& L, double w,
-                                                 double (&acc)[kMaxLeaves][kAccPerLeaf],
+                                                 double (&acc)[kMaxSlots][kAccPerLeaf],
                                                  double (&accd)[DP]) {
-  double gard = 0.0;
+  double gard = 0.0, term = 0.0;
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) {
-    if (l < P.nl) {
-      double cl = 0.0;
-#pragma unroll
-      for (int t = 0; t < kMaxTerms; ++t) {
-        if (t < P.nt && ((P.mask[t] >> l) & 1u)) {
-          double pr = 1.0;
-#pragma unroll
-          for (int l2 = 0; l2 < kMaxLeaves; ++l2)
-            if (l2 != l && ((P.mask[t] >> l2) & 1u)) pr *= L.v[l2];
-          cl += pr;
-        }
-      }
-      const double g = w * cl * L.v[l];
-      acc[l][0] += g * L.f0[l];
-      acc[l][1] += g * L.f1[l];
-      if (l == P.ard_leaf) gard = g;
+  for (int sl = kMaxSlots - 1; sl >= 0; --sl) {
+    if (sl < P.ns) {
+      if ((P.end >> sl) & 1u) term = L.tp[sl];
+      const double g = w * term;
+      acc[sl][0] += g * L.f0[sl];
+      acc[sl][1] += g * L.f1[sl];
+      if ((P.ard >> sl) & 1u) gard += g;
     }
   }
-  if (P.ard_leaf >= 0) {
+  if (P.ard) {
 #pragma unroll
     for (int q = 0; q < DP; ++q) accd[q] += gard * dx[q] * dx[q];
   }
@@ -110,12 +94,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Block reduction of the accumulators into partial[tile * kMaxAcc + k].
 template <int DP>
-__device__ __forceinline__ void store_partials(double (&acc)[kMaxLeaves][kAccPerLeaf],
+__device__ __forceinline__ void store_partials(double (&acc)[kMaxSlots][kAccPerLeaf],
                                                double (&accd)[DP], double* red /* [4][kMaxAcc] */,
                                                double* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l)
+  for (int l = 0; l < kMaxSlots; ++l)
 #pragma unroll
     for (int c = 0; c < kAccPerLeaf; ++c) {
       const double v = wave_sum(acc[l][c]);
@@ -125,7 +109,7 @@ __device__ __forceinline__ void store_partials(double (&acc)[kMaxLeaves][kAccPer
   for (int q = 0; q < kMaxDim; ++q) {
     double v = 0.0;
     if (q < DP) v = wave_sum(accd[q < DP ? q : 0]);
-    if (lane == 0) red[wave * kMaxAcc + kMaxLeaves * kAccPerLeaf + q] = v;
+    if (lane == 0) red[wave * kMaxAcc + kMaxSlots * kAccPerLeaf + q] = v;
   }
   __syncthreads();
   if (threadIdx.x < kMaxAcc) {

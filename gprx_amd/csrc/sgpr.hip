@@ -61,10 +61,10 @@ __global__ __launch_bounds__(256) void k_pu_contract(
     cs[threadIdx.x] = cv[ti * kTile + threadIdx.x];
     rs[threadIdx.x] = rv[tj * kTile + threadIdx.x];
   }
-  double a2[kMaxLeaves][kAccPerLeaf];
+  double a2[kMaxSlots][kAccPerLeaf];
   double ad[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxLeaves; ++l) a2[l][0] = a2[l][1] = 0.0;
+  for (int l = 0; l < kMaxSlots; ++l) a2[l][0] = a2[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) ad[q] = 0.0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

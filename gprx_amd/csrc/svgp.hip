@@ -170,51 +170,54 @@ __global__ void k_neg_sym(const double* T2, long ld, int n, double* W) {
 }
 
 // Variational-parameter gradient (user space) and the KL terms:
-// g_mu = mu - c Ares; g_L = [L_ij or L_ii - 1/L_ii] + c M_ij.
-// out[0] = sum L^2, out[1] = sum log L_ii, out[2] = |mu|^2 (one block).
+// g_mu = mu - c Ares; g_L = [L_ij or L_ii - 1/L_ii] + c M_ij.  Workgroup j
+// takes column j of the packed lower triangle (rows i >= j, contiguous in
+// both the packed vector and M) and mu_j; part[3 j + {0, 1, 2}] receive its
+// sum of L^2, log L_jj and mu_j^2, summed by k_q_sums.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_q_grad(const double* mu, const double* packed, int m,
-                                                 const double* Ares, const T* Mq, long ld, double c,
-                                                 double* gmu, double* gL, double* out) {
-  __shared__ double r0[1024], r1[1024], r2[1024];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  const long plen = (long)m * (m + 1) / 2;
-  for (long k = threadIdx.x; k < plen; k += 1024) {
-    // column j of the packed lower triangle holding k
-    long j = (long)((2.0 * m + 1.0 - sqrt((2.0 * m + 1.0) * (2.0 * m + 1.0) - 8.0 * (double)k)) * 0.5);
-    while (j > 0 && j * m - j * (j - 1) / 2 > k) --j;
-    while ((j + 1) * m - (j + 1) * j / 2 <= k) ++j;
-    const long i = j + (k - (j * m - j * (j - 1) / 2));
+__global__ __launch_bounds__(256) void k_q_grad(const double* mu, const double* packed, int m,
+                                                const double* Ares, const T* Mq, long ld, double c,
+                                                double* gmu, double* gL, double* part) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  const long col = (long)j * m - (long)j * (j - 1) / 2;  // packed index of (j, j)
+  double s0 = 0.0;
+  for (int i = j + (int)threadIdx.x; i < m; i += 256) {
+    const long k = col + (i - j);
     const double v = packed[k];
     s0 += v * v;
-    double g = v;
-    if (i == j) {
-      s1 += log(v);
-      g = v - 1.0 / v;
-    }
-    gL[k] = g + c * (double)Mq[i + j * ld];
+    const double g = i == j ? v - 1.0 / v : v;
+    gL[k] = g + c * (double)Mq[i + (long)j * ld];
   }
-  for (int k = threadIdx.x; k < m; k += 1024) {
-    s2 += mu[k] * mu[k];
-    gmu[k] = mu[k] - c * Ares[k];
-  }
-  r0[threadIdx.x] = s0;
-  r1[threadIdx.x] = s1;
-  r2[threadIdx.x] = s2;
+  red[threadIdx.x] = s0;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      r0[threadIdx.x] += r0[threadIdx.x + s];
-      r1[threadIdx.x] += r1[threadIdx.x + s];
-      r2[threadIdx.x] += r2[threadIdx.x + s];
-    }
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = r0[0];
-    out[1] = r1[0];
-    out[2] = r2[0];
+    part[3 * j] = red[0];
+    part[3 * j + 1] = log(packed[col]);
+    part[3 * j + 2] = mu[j] * mu[j];
+    gmu[j] = mu[j] - c * Ares[j];
   }
+}
+
+// out[q] = sum_j part[3 j + q], q = 0, 1, 2 (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void k_q_sums(const double* part, int m, double* out) {
+  __shared__ double red[3][256];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < m; j += 256)
+    for (int q = 0; q < 3; ++q) s[q] += part[3 * j + q];
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) out[q] = red[q][0];
 }
 
 // Adam over the variational block: z holds mu (m) then the packed Lq in the
@@ -302,7 +305,7 @@ struct SvgpModel : gprx_hip_model {
   // storage-typed
   DevBuf<double> X64c, L64c, A64, U64, G64, LU64, Wmn64, Kzx64;
   DevBuf<float> X32, L32, A32, U32, G32, LU32, Wmn32, Kzx32, Mq32, Wpre32;
-  DevBuf<double> xb, yb, mu_a, a2, u2, resid, Ares, logd, scal, partials, partials2, accs, gmu, gL;
+  DevBuf<double> xb, yb, mu_a, a2, u2, resid, Ares, logd, scal, partials, partials2, accs, gmu, gL, qpart;
   DevBuf<double> params_d;  // user-space params (device)
   DevBuf<double> zq, m1q, m2q, userq;  // Adam state of the variational block
   DevBuf<uint32_t> perm, bidx;
@@ -411,8 +414,9 @@ struct SvgpModel : gprx_hip_model {
     gemv_n<T>(A, mpad, M, B, resid.p, Ares.p, partials.p, 0, 0, 0, s);
     // Mq = A U^T (lower used)
     gemm_call<T>(false, true, A, mpad, U, mpad, Mq, mpad, M, M, B, 1.0, 0.0, 0, 1, kOrderRowFast, s);
-    hipLaunchKernelGGL((k_q_grad<T>), dim3(1), dim3(1024), 0, s, pu + n_theta, pu + n_theta + m, m,
-                       Ares.p, Mq, mpad, scale / s2, gmu.p, gL.p, scal.p + 1);
+    hipLaunchKernelGGL((k_q_grad<T>), dim3(m), dim3(256), 0, s, pu + n_theta, pu + n_theta + m, m,
+                       Ares.p, Mq, mpad, scale / s2, gmu.p, gL.p, qpart.p);
+    hipLaunchKernelGGL(k_q_sums, dim3(1), dim3(256), 0, s, qpart.p, m, scal.p + 1);
     // G = (mu resid^T + A - Lq U) / s2
     gemm_call<T>(false, false, Lc, mpad, U, mpad, LU, mpad, M, B, M, 1.0, 0.0, kKhiRow, 0,
                  kOrderColFastRev, s);
@@ -545,6 +549,7 @@ int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, c
     M.Ares.alloc(M.mpad);
     M.gmu.alloc(m);
     M.gL.alloc(M.plen);
+    M.qpart.alloc(3 * (size_t)m);
     M.params_d.alloc(M.ptot);
     M.scal.alloc(16);
     M.accs.alloc(3 * kMaxAcc);

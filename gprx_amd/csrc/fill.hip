@@ -348,7 +348,9 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
                                                         int na, const double* __restrict__ b,
                                                         long ldb, int nb, const T* __restrict__ W,
                                                         long ldw, int mt,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial,
+                                                        const double* __restrict__ cv,
+                                                        const double* __restrict__ rv, double crs) {
   const DevProgram& P = *Pg;
   __shared__ double xr[DP][kTile];
   __shared__ double xc[DP][kTile];
@@ -370,11 +372,13 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   if (gi < na) {
+    const double ci = cv ? crs * cv[gi] : 0.0;
     for (int jj = 0; jj < 64; ++jj) {
       const int j = jh + 2 * jj;
       const int gj = c0 + j;
       if (gj >= nb) continue;
-      const double w = (double)W[gi + (long)gj * ldw];
+      double w = (double)W[gi + (long)gj * ldw];
+      if (cv) w -= ci * rv[gj];
       double dx[DP];
 #pragma unroll
       for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
@@ -501,12 +505,13 @@ void contract_exact(int dp, const DevProgram* prog, DevPoints x, const T* Ainv, 
 
 template <typename T>
 void contract_cross(int dp, const DevProgram* prog, DevPoints a, DevPoints b, const T* W, long ldw,
-                    double* partial, hipStream_t stream) {
+                    double* partial, hipStream_t stream, const double* cv, const double* rv,
+                    double crs) {
   const int mt = (int)(a.ld / kTile);
   const int tiles = cross_tiles(a.ld, b.ld);
   GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_cross<DP, T>), dim3(tiles), dim3(256), 0,
                                            stream, prog, a.p, a.ld, a.rows, b.p, b.ld, b.rows, W,
-                                           ldw, mt, partial));
+                                           ldw, mt, partial, cv, rv, crs));
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
@@ -535,8 +540,10 @@ template void contract_exact<double>(int, const DevProgram*, DevPoints, const do
 template void contract_exact<float>(int, const DevProgram*, DevPoints, const float*, long,
                                     const double*, double*, hipStream_t);
 template void contract_cross<double>(int, const DevProgram*, DevPoints, DevPoints, const double*,
-                                     long, double*, hipStream_t);
+                                     long, double*, hipStream_t, const double*, const double*,
+                                     double);
 template void contract_cross<float>(int, const DevProgram*, DevPoints, DevPoints, const float*,
-                                    long, double*, hipStream_t);
+                                    long, double*, hipStream_t, const double*, const double*,
+                                    double);
 
 }  // namespace gprx

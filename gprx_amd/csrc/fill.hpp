@@ -39,10 +39,11 @@ void contract_exact(int dim_pad, const DevProgram* prog, DevPoints x, const T* A
 
 // Contraction of a dense weight matrix W (a.rows x b.rows, f64 or f32, ld
 // ldw) against dK(a, b): partial per tile as above.  Tiles are full
-// (a.ld / 128) x (b.ld / 128).
+// (a.ld / 128) x (b.ld / 128).  With cv, the weight is W_ij - crs cv_i rv_j.
 template <typename T>
 void contract_cross(int dim_pad, const DevProgram* prog, DevPoints a, DevPoints b, const T* W,
-                    long ldw, double* partial, hipStream_t stream);
+                    long ldw, double* partial, hipStream_t stream, const double* cv = nullptr,
+                    const double* rv = nullptr, double crs = 0.0);
 
 // Contraction of a constant weight w against dk(a_i, a_i): one partial row.
 void contract_diag(int dim_pad, const DevProgram* prog, DevPoints a, double w, double* partial,

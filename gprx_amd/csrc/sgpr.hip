@@ -8,7 +8,7 @@
 //       + (sum k_ii - |A|_F^2) / (2 s2)
 // The adjoint G = B^-1 A - (w r^T + A)/s2 is never formed: with
 // Q = X^T (B^-1 - I/s2) and v = X^T w,
-//   w_mn = X^T G = Q A - v r^T / s2                 (fused: MFMA tile + contraction)
+//   w_mn = X^T G = Q A - v r^T / s2      (Q A on MFMA, rank-1 part in the contraction)
 //   G A^T = I - s2 B^-1 - (B - s2 I)/s2 - w (A r)^T / s2 (+ I)   (m x m only)
 //   w_mm = -sym(X^T tril_half(G A^T) X)
 //   dF/dtheta = <w_mm, dK_mm> + <w_mn, dK_mn> + 1/(2 s2) sum dk_ii
@@ -20,119 +20,13 @@
 #include <memory>
 #include <type_traits>
 
-#include "kernel_eval.hpp"
 #include "model.hpp"
-#include "tile_gemm.hpp"
 
 namespace gprx {
 
 int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err);
 void exact_upload_points(DevBuf<double>& dst, const double* src, int n, int d, long rows_pad, int dp,
                          hipStream_t s);
-
-// partial[tile] = contraction of W = (P U)_tile - crs c_i r_j against
-// dK(z_i, x_j), for the (M/128) x (N/128) tiles of P U (P: M x K, U: K x N).
-template <typename T, int DP>
-__global__ __launch_bounds__(256) void k_pu_contract(
-    const DevProgram* __restrict__ Pg, const T* __restrict__ P, long ldp, const T* __restrict__ U,
-    long ldu, int M, int N, int K, const double* __restrict__ z, long ldz, int m,
-    const double* __restrict__ x, long ldx, int n, const double* __restrict__ cv,
-    const double* __restrict__ rv, double crs, double* __restrict__ partial) {
-  using AV = typename Acc<T>::V;
-  const DevProgram& Pr = *Pg;
-  __shared__ TileSmem<T> sm;
-  __shared__ double zr[DP][kTile];
-  __shared__ double xc[DP][kTile];
-  __shared__ double cs[kTile], rs[kTile];
-  __shared__ double red[4 * kMaxAcc];
-  const int mt = M / kTile;
-  const int bid = xcd_remap(blockIdx.x, mt * (N / kTile));
-  const int ti = bid % mt, tj = bid / mt;
-  AV acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
-  tile_mainloop<T, false, false>(P + (long)ti * kTile, ldp, U + (long)tj * kTile * ldu, ldu, 0,
-                                 K / Cfg<T>::BK, sm, acc);
-  load_points<DP>(z, ldz, ti * kTile, zr);
-  load_points<DP>(x, ldx, tj * kTile, xc);
-  if (threadIdx.x < kTile) {
-    cs[threadIdx.x] = cv[ti * kTile + threadIdx.x];
-    rs[threadIdx.x] = rv[tj * kTile + threadIdx.x];
-  }
-  double a2[kMaxSlots][kAccPerLeaf];
-  double ad[DP];
-#pragma unroll
-  for (int l = 0; l < kMaxSlots; ++l) a2[l][0] = a2[l][1] = 0.0;
-#pragma unroll
-  for (int q = 0; q < DP; ++q) ad[q] = 0.0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // The GEMM's LDS is free after the main loop: stage one 16-row strip of
-  // each wave's 64 x 64 result at a time as f64, [wave][col][row].  The
-  // strip index goes through a switch so every accumulator index is a
-  // compile-time constant.
-  double* strip = reinterpret_cast<double*>(&sm);
-  auto stage = [&](auto I) {
-    constexpr int i = decltype(I)::value;
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        strip[wave * 1024 + (j * 16 + drow<T>(lane, q)) * 16 + (lane & 15)] = (double)acc[i][j][q];
-  };
-  for (int i = 0; i < 4; ++i) {
-    __syncthreads();
-    switch (i) {
-      case 0: stage(std::integral_constant<int, 0>{}); break;
-      case 1: stage(std::integral_constant<int, 1>{}); break;
-      case 2: stage(std::integral_constant<int, 2>{}); break;
-      default: stage(std::integral_constant<int, 3>{}); break;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 4096; e += 256) {
-      const int w = e >> 10, rem = e & 1023, cc = rem >> 4, rr = rem & 15;
-      const int r = (w & 1) * 64 + i * 16 + rr;
-      const int c = (w >> 1) * 64 + cc;
-      if (ti * kTile + r >= m || tj * kTile + c >= n) continue;
-      const double wv = strip[e] - crs * cs[r] * rs[c];
-      double dx[DP];
-#pragma unroll
-      for (int q = 0; q < DP; ++q) dx[q] = zr[q][r] - xc[q][c];
-      Leaves L;
-      eval_entry<DP, true>(Pr, dx, L);
-      accumulate_entry<DP>(Pr, dx, L, wv, a2, ad);
-    }
-  }
-  store_partials<DP>(a2, ad, red, partial + (long)blockIdx.x * kMaxAcc);
-}
-
-template <typename T>
-void pu_contract(int dp, const DevProgram* prog, const T* P, long ldp, const T* U, long ldu, int M,
-                 int N, int K, DevPoints z, DevPoints x, const double* c, const double* r,
-                 double crs, double* partial, hipStream_t s) {
-  const int tiles = (M / kTile) * (N / kTile);
-#define GPRX_PU(DPV)                                                                          \
-  hipLaunchKernelGGL((k_pu_contract<T, DPV>), dim3(tiles), dim3(256), 0, s, prog, P, ldp, U, ldu, \
-                     M, N, K, z.p, z.ld, z.rows, x.p, x.ld, x.rows, c, r, crs, partial)
-  switch (dp) {
-    case 1: GPRX_PU(1); break;
-    case 2: GPRX_PU(2); break;
-    case 4: GPRX_PU(4); break;
-    case 8: GPRX_PU(8); break;
-    case 16: GPRX_PU(16); break;
-    case 32: GPRX_PU(32); break;
-    default: throw DeviceError("unsupported padded dimension");
-  }
-#undef GPRX_PU
-  GPRX_HIP_CHECK(hipGetLastError());
-}
-template void pu_contract<double>(int, const DevProgram*, const double*, long, const double*, long,
-                                  int, int, int, DevPoints, DevPoints, const double*, const double*,
-                                  double, double*, hipStream_t);
-template void pu_contract<float>(int, const DevProgram*, const float*, long, const float*, long, int,
-                                 int, int, DevPoints, DevPoints, const double*, const double*,
-                                 double, double*, hipStream_t);
 
 }  // namespace gprx
 
@@ -312,14 +206,20 @@ struct SgprModel : gprx_hip_model {
       gemm_call<double>(true, false, Xmm.p, mpad, Dm.p, mpad, Q64.p, mpad, M, M, M, 1.0, 0.0,
                         kKloRow, 0, kOrderRowFast, s);
       gemv_t<double>(Xmm.p, mpad, M, M, w.p, v.p, 1, 0, 0, 0, s);
+      // w_mn = Q A - v r^T / s2: Q A on MFMA into U's storage (K_mn is not
+      // needed once A is formed), the rank-1 part inside the contraction.
       const int tiles1 = (int)(mpad / kTile) * (int)(npad / kTile);
+      const int N = (int)npad;
       if (mixed()) {
         convert<float, double>(Q32.p, Q64.p, (size_t)mpad * mpad, s);
-        pu_contract<float>(dp, prog.p, Q32.p, mpad, A32.p, mpad, M, (int)npad, M, zp(), xp(), v.p,
-                           r.p, 1.0 / s2, partials.p, s);
+        gemm_call<float>(false, false, Q32.p, mpad, A32.p, mpad, U32.p, mpad, M, N, M, 1.0, 0.0, 0,
+                         0, kOrderRowFast, s);
+        contract_cross<float>(dp, prog.p, zp(), xp(), U32.p, mpad, partials.p, s, v.p, r.p, 1.0 / s2);
       } else {
-        pu_contract<double>(dp, prog.p, Q64.p, mpad, A64.p, mpad, M, (int)npad, M, zp(), xp(), v.p,
-                            r.p, 1.0 / s2, partials.p, s);
+        gemm_call<double>(false, false, Q64.p, mpad, A64.p, mpad, U64.p, mpad, M, N, M, 1.0, 0.0, 0,
+                          0, kOrderRowFast, s);
+        contract_cross<double>(dp, prog.p, zp(), xp(), U64.p, mpad, partials.p, s, v.p, r.p,
+                               1.0 / s2);
       }
       reduce_partials(partials.p, tiles1, accs.p, s);
       // G A^T = I - s2 B^-1 - S/s2 - w (A r)^T / s2 ; w_mm = -sym(X^T tril_half(.) X)

@@ -106,6 +106,9 @@ SIGNATURES = {
                                         C.c_int32, _dp, _dp, _err]),
     "gprx_hip_batch_create": (C.c_int, [C.POINTER(ProblemC), C.c_size_t, C.c_int32, _mp, _err]),
     "gprx_hip_batch_value_and_grad": (C.c_int, [_vp, C.POINTER(_dp), _dp, C.POINTER(_dp), _err]),
+    "gprx_hip_batch_set_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32, _err]),
+    "gprx_hip_batch_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double)]),
     "gprx_hip_destroy": (None, [_vp]),
     "gprx_hip_last_eval_ms": (C.c_double, [_vp]),
     "gprx_hip_last_dense_ms": (C.c_double, [_vp]),

@@ -67,5 +67,19 @@ class GprBatch:
                 out_err.append(None)
         return nlml, grads, out_err
 
+    def set_schedule(self, policy: int, small_tile_below: int = 0) -> None:
+        """Fixes the launch schedule (policy 0, 1 or 2; -1 returns to the
+        autotuner), see gprx_hip_batch_set_schedule."""
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_batch_set_schedule(self._h, policy, small_tile_below, C.byref(err))
+        raise_for(rc, err)
+
+    def schedule(self):
+        """(policy, small_tile_below, tuned_ms) in use; policy -1 before the
+        first evaluation."""
+        pol, below, ms = C.c_int32(), C.c_int32(), C.c_double()
+        L.lib().gprx_hip_batch_schedule(self._h, C.byref(pol), C.byref(below), C.byref(ms))
+        return pol.value, below.value, ms.value
+
     def last_eval_ms(self) -> float:
         return L.lib().gprx_hip_last_eval_ms(self._h)

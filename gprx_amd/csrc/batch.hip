@@ -18,8 +18,11 @@
 // parameters re-uploaded.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <string>
 
 #include "kernel_eval.hpp"
 #include "model.hpp"
@@ -37,6 +40,8 @@ constexpr double kLog2Pi = 1.8378770664093453;
 
 // One output tile of a grouped product: C(c) = beta C + alpha op(A) op(B),
 // op(A) 128 x K at a, op(B) K x 128 at b (element offsets into the arenas).
+// Each operand lives in arena 0 (the kernel matrix, then L, Y and A^-1) or
+// arena 1 (X = L^-1): bit 0 of `arenas` for A, bit 1 for B, bit 2 for C.
 struct TileTask {
   long a, b, c;
   int ld;
@@ -44,31 +49,39 @@ struct TileTask {
   int tb;     // B stored N x K (NT) instead of K x N (NN)
   int ta;     // A stored K x M (TN)
   int lower;  // diagonal tile: keep r >= c
+  int arenas;
   float alpha, beta;
 };
 
-template <typename T, bool TA, bool TB>
-__device__ __forceinline__ void run_task(const TileTask& t, const T* Aar, const T* Bar, T* Car,
-                                         TileSmem<T>& sm) {
+// Sub-tile `quad` (TT = 64: row half quad & 1, column half quad >> 1) of a
+// task, or the whole tile (TT = 128).
+template <typename T, bool TA, bool TB, int TT>
+__device__ __forceinline__ void run_task(const TileTask& t, int quad, T* X0, T* X1,
+                                         TileSmem<T, TT>& sm) {
   using AV = typename Acc<T>::V;
-  AV acc[4][4];
+  constexpr int FB = TT / 32, WT = TT / 2;
+  const int qr = TT == 128 ? 0 : (quad & 1) * 64, qc = TT == 128 ? 0 : (quad >> 1) * 64;
+  const T* Ab = ((t.arenas & 1) ? X1 : X0) + t.a + (TA ? (long)qr * t.ld : qr);
+  const T* Bb = ((t.arenas & 2) ? X1 : X0) + t.b + (TB ? qc : (long)qc * t.ld);
+  T* C = ((t.arenas & 4) ? X1 : X0) + t.c + qr + (long)qc * t.ld;
+  AV acc[FB][FB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = AV{};
-  tile_mainloop<T, TA, TB>(Aar + t.a, t.ld, Bar + t.b, t.ld, 0, t.K / Cfg<T>::BK, sm, acc);
+    for (int j = 0; j < FB; ++j) acc[i][j] = AV{};
+  tile_mainloop<T, TA, TB, TT>(Ab, t.ld, Bb, t.ld, 0, t.K / Cfg<T, TT>::BK, sm, acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  T* C = Car + t.c;
+  const bool diag = t.lower && qr == qc;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < FB; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = wm * 64 + i * 16 + (lane & 15);
-        const int c = wn * 64 + j * 16 + drow<T>(lane, q);
-        if (t.lower && r < c) continue;
+        const int r = wm * WT + i * 16 + (lane & 15);
+        const int c = wn * WT + j * 16 + drow<T>(lane, q);
+        if (diag && r < c) continue;
         T* p = C + (long)c * t.ld + r;
         double v = (double)t.alpha * (double)acc[i][j][q];
         if (t.beta != 0.f) v += (double)t.beta * (double)(*p);
@@ -76,18 +89,24 @@ __device__ __forceinline__ void run_task(const TileTask& t, const T* Aar, const 
       }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256, kTileOccupancy) void k_gemm_tasks(const TileTask* __restrict__ tasks, int count,
-                                                    const T* Aar, const T* Bar, T* Car) {
-  __shared__ TileSmem<T> sm;
-  const int id = xcd_remap(blockIdx.x, count);
-  const TileTask t = tasks[id];
+// One launch of a task range: slot s (blockIdx.x) runs sched[s] = task << 2
+// | quadrant, or nothing for a padding slot (-1).  The host builds `sched`
+// per launch so XCD x (slots 8k + x) gets a balanced share (see
+// build_slots).
+template <typename T, int TT>
+__global__ __launch_bounds__(256, kTileOccupancy) void k_gemm_tasks(const TileTask* __restrict__ tasks,
+                                                                    const int* __restrict__ sched,
+                                                                    T* X0, T* X1) {
+  __shared__ TileSmem<T, TT> sm;
+  const int v = sched[blockIdx.x];
+  if (v < 0) return;
+  const TileTask t = tasks[v >> 2];
   if (t.ta)
-    run_task<T, true, false>(t, Aar, Bar, Car, sm);
+    run_task<T, true, false, TT>(t, v & 3, X0, X1, sm);
   else if (t.tb)
-    run_task<T, false, true>(t, Aar, Bar, Car, sm);
+    run_task<T, false, true, TT>(t, v & 3, X0, X1, sm);
   else
-    run_task<T, false, false>(t, Aar, Bar, Car, sm);
+    run_task<T, false, false, TT>(t, v & 3, X0, X1, sm);
 }
 
 // Per-problem descriptors on the device.
@@ -410,15 +429,27 @@ struct BatchModel : gprx_hip_model {
   DevBuf<int> info, active, first;
   HostBuf<int> hinfo, hactive;
   HostBuf<double> hscal, hacc;
-  // task lists
+  // Task lists (host copy kept for the schedules).
   struct Range {
     int off = 0, count = 0;
   };
+  // A launch's schedule: tile edge and its slots in d_slots.
+  struct Sched {
+    int tt = 128, off = 0, n = 0;
+  };
+  std::vector<TileTask> htasks;
+  std::vector<int> task_prob;
   DevBuf<TileTask> tasks;
-  // per step: panel (A, P -> A), row block (P, P -> P), trailing factor
-  // update (A, A -> A), trailing inverse update (A, P -> P)
-  std::vector<Range> step_pa, step_pb, step_ta, step_tb;
+  // per step j of the factor: column update, panel + Y row; per distance d
+  // of the inverse; X^T X
+  std::vector<Range> upd, pan, inv;
   Range lauum;
+  std::vector<Sched> upd_s, pan_s, inv_s;
+  Sched lauum_s;
+  DevBuf<int> d_slots;
+  int policy = -1, small_below = -1;  // the schedule in use (-1: not built)
+  bool forced = false;                // set by gprx_hip_batch_set_schedule
+  double tuned_ms = 0.0;
   DevBuf<long> d_aoff, d_xoff, d_ld, d_logoff;
   DevBuf<int> d_infoidx;
   std::vector<Range> step_diag;
@@ -429,76 +460,93 @@ struct BatchModel : gprx_hip_model {
 
   bool mixed() const { return precision == GPRX_MIXED; }
 
+  // Left-looking factor and inverse, one task per 128 x 128 output tile
+  // (arena 0: K, then L; arena 1: X).  Step j of the factor:
+  //   upd: A_ij -= L_i,0:j L_j,0:j^T      (i >= j; K = 128 j)
+  //   diag: X_jj = L_jj^-1 and log L_jj   (k_potrf_diag_list)
+  //   pan: L_ij = A_ij X_jj^T (i > j) and Y_jk = X_jj L_jk (k < j, in place:
+  //        row j of L is not read again by the factor)
+  // then the inverse by distance d = i - j, X_ij = -sum_{k=j}^{i-1} Y_ik X_kj
+  // (L X = I with Y_ik = X_ii L_ik; K = 128 d), and A^-1 = X^T X (lower).
   void build_tasks() {
-    std::vector<TileTask> tl;
     std::vector<long> aoff, xoff, ld, logoff;
     std::vector<int> infoidx;
-    auto push_range = [&](std::vector<TileTask>& v, Range& rg, std::vector<TileTask>&& add,
-                          bool sort_k) {
-      if (sort_k)
-        std::stable_sort(add.begin(), add.end(), [](const TileTask& a, const TileTask& b) { return a.K > b.K; });
-      rg.off = (int)v.size();
-      rg.count = (int)add.size();
-      v.insert(v.end(), add.begin(), add.end());
+    htasks.clear();
+    task_prob.clear();
+    auto open_range = [&](Range& rg) { rg.off = (int)htasks.size(); };
+    auto close_range = [&](Range& rg) { rg.count = (int)htasks.size() - rg.off; };
+    auto add = [&](int p, TileTask t) {
+      htasks.push_back(t);
+      task_prob.push_back(p);
     };
-    for (auto* v : {&step_pa, &step_pb, &step_ta, &step_tb}) v->assign(maxnt, Range{});
+    upd.assign(maxnt, Range{});
+    pan.assign(maxnt, Range{});
+    inv.assign(maxnt, Range{});
     step_diag.assign(maxnt, Range{});
-    for (int k = 0; k < maxnt; ++k) {
-      step_diag[k].off = (int)aoff.size();
+    for (int j = 0; j < maxnt; ++j) {
+      step_diag[j].off = (int)aoff.size();
       for (int p = 0; p < K; ++p) {
         const HostProb& h = hp[p];
-        if (h.nt <= k) continue;
-        const long ldl = h.npad;
-        const long kk = (long)k * kTile;
-        aoff.push_back(h.moff + kk + kk * ldl);
-        xoff.push_back(h.moff + kk + kk * ldl);
-        ld.push_back(ldl);
-        logoff.push_back(h.voff + kk);
+        if (h.nt <= j) continue;
+        const long L = h.npad, jj = (long)j * kTile;
+        aoff.push_back(h.moff + jj + jj * L);
+        xoff.push_back(h.moff + jj + jj * L);
+        ld.push_back(L);
+        logoff.push_back(h.voff + jj);
         infoidx.push_back(p);
       }
-      step_diag[k].count = (int)aoff.size() - step_diag[k].off;
-      std::vector<TileTask> pa, pb, ta, tb;
+      step_diag[j].count = (int)aoff.size() - step_diag[j].off;
+      open_range(upd[j]);
+      if (j > 0)
+        for (int p = 0; p < K; ++p) {
+          const HostProb& h = hp[p];
+          if (h.nt <= j) continue;
+          const long L = h.npad;
+          auto at = [&](int r, int c) { return h.moff + (long)r * kTile + (long)c * kTile * L; };
+          for (int i = j; i < h.nt; ++i)
+            add(p, TileTask{at(i, 0), at(j, 0), at(i, j), (int)L, j * kTile, 1, 0, i == j, 0, -1.f, 1.f});
+        }
+      close_range(upd[j]);
+      open_range(pan[j]);
       for (int p = 0; p < K; ++p) {
         const HostProb& h = hp[p];
-        if (h.nt <= k) continue;
+        if (h.nt <= j) continue;
         const long L = h.npad;
         auto at = [&](int r, int c) { return h.moff + (long)r * kTile + (long)c * kTile * L; };
-        // L_ik = A_ik X_kk^T  (A arena; X_kk in the P arena)
-        for (int i = k + 1; i < h.nt; ++i)
-          pa.push_back(TileTask{at(i, k), at(k, k), at(i, k), (int)L, kTile, 1, 0, 0, 1.f, 0.f});
-        // P_kj <- X_kk P_kj (in place), j < k
-        for (int j = 0; j < k; ++j)
-          pb.push_back(TileTask{at(k, k), at(k, j), at(k, j), (int)L, kTile, 0, 0, 0, 1.f, 0.f});
-        // A_ij -= L_ik L_jk^T
-        for (int i = k + 1; i < h.nt; ++i)
-          for (int j = k + 1; j <= i; ++j)
-            ta.push_back(TileTask{at(i, k), at(j, k), at(i, j), (int)L, kTile, 1, 0, i == j, -1.f, 1.f});
-        // P_ij -= L_ik P_kj (j <= k; first touch of column k: beta 0)
-        for (int i = k + 1; i < h.nt; ++i)
-          for (int j = 0; j <= k; ++j)
-            tb.push_back(TileTask{at(i, k), at(k, j), at(i, j), (int)L, kTile, 0, 0, 0, -1.f,
-                                  j == k ? 0.f : 1.f});
+        for (int i = j + 1; i < h.nt; ++i)  // L_ij = A_ij X_jj^T: B from arena 1
+          add(p, TileTask{at(i, j), at(j, j), at(i, j), (int)L, kTile, 1, 0, 0, 2, 1.f, 0.f});
+        for (int k = 0; k < j; ++k)  // Y_jk = X_jj L_jk: A from arena 1
+          add(p, TileTask{at(j, j), at(j, k), at(j, k), (int)L, kTile, 0, 0, 0, 1, 1.f, 0.f});
       }
-      push_range(tl, step_pa[k], std::move(pa), false);
-      push_range(tl, step_pb[k], std::move(pb), false);
-      push_range(tl, step_ta[k], std::move(ta), false);
-      push_range(tl, step_tb[k], std::move(tb), false);
+      close_range(pan[j]);
     }
-    // A^-1 (lower) = X^T X: C_ij = sum_{k >= i} X_ki^T X_kj
-    std::vector<TileTask> la;
+    for (int d = 1; d < maxnt; ++d) {
+      open_range(inv[d]);
+      for (int p = 0; p < K; ++p) {
+        const HostProb& h = hp[p];
+        const long L = h.npad;
+        auto at = [&](int r, int c) { return h.moff + (long)r * kTile + (long)c * kTile * L; };
+        for (int j = 0; j + d < h.nt; ++j)  // Y from arena 0, X (B and C) in arena 1
+          add(p, TileTask{at(j + d, j), at(j, j), at(j + d, j), (int)L, d * kTile, 0, 0, 0, 6,
+                          -1.f, 0.f});
+      }
+      close_range(inv[d]);
+    }
+    // A^-1 (lower) = X^T X: C_ij = sum_{k >= i} X_ki^T X_kj, into arena 0.
+    open_range(lauum);
     for (int p = 0; p < K; ++p) {
       const HostProb& h = hp[p];
       const long L = h.npad;
       for (int i = 0; i < h.nt; ++i)
         for (int j = 0; j <= i; ++j) {
-          const long a = h.moff + (long)i * kTile + (long)i * kTile * L;  // X(i*128.., i*128..)
-          const long b = h.moff + (long)i * kTile + (long)j * kTile * L;  // X(i*128.., j*128..)
+          const long a = h.moff + (long)i * kTile + (long)i * kTile * L;
+          const long b = h.moff + (long)i * kTile + (long)j * kTile * L;
           const long c = h.moff + (long)i * kTile + (long)j * kTile * L;
-          la.push_back(TileTask{a, b, c, (int)L, (h.nt - i) * kTile, 0, 1, i == j, 1.f, 0.f});
+          add(p, TileTask{a, b, c, (int)L, (h.nt - i) * kTile, 0, 1, i == j, 3, 1.f, 0.f});
         }
     }
-    push_range(tl, lauum, std::move(la), true);
-    upload(tasks, tl, stream);
+    close_range(lauum);
+    upload(tasks, htasks, stream);
     upload(d_aoff, aoff, stream);
     upload(d_xoff, xoff, stream);
     upload(d_ld, ld, stream);
@@ -522,11 +570,124 @@ struct BatchModel : gprx_hip_model {
     upload(first, fst, stream);
   }
 
+  // Slots of one launch under a schedule policy.  Items are tasks (tile
+  // edge 128) or their quadrants (64; the upper quadrant of a diagonal tile
+  // is skipped); an item's work is its K in 128-blocks plus one.
+  //   policy 0: build order (problem-major), one contiguous run per XCD
+  //   policy 1: longest first, each to the least-loaded XCD
+  //   policy 2: problems as units, heaviest first to the least-loaded XCD,
+  //             a problem's items longest first (its panels share an L2)
+  // Slot 8k + x holds XCD x's k-th item; -1 pads.
+  std::vector<int> build_slots(const Range& rg, int tt, int pol) const {
+    struct Item {
+      int code, prob;
+      long work;
+    };
+    std::vector<Item> items;
+    for (int t = rg.off; t < rg.off + rg.count; ++t) {
+      const TileTask& tk = htasks[t];
+      const long w = tk.K / kTile + 1;
+      if (tt == 128) {
+        items.push_back(Item{t << 2, task_prob[t], w});
+      } else {
+        for (int q = 0; q < 4; ++q) {
+          if (tk.lower && q == 2) continue;  // rows 0-63, columns 64-127: above the diagonal
+          items.push_back(Item{t << 2 | q, task_prob[t], w});
+        }
+      }
+    }
+    constexpr int kX = 8;
+    std::vector<std::vector<int>> lists(kX);
+    const int total = (int)items.size();
+    if (pol == 0) {
+      const int q = total / kX, r = total % kX;
+      int pos = 0;
+      for (int x = 0; x < kX; ++x) {
+        const int len = q + (x < r ? 1 : 0);
+        for (int k = 0; k < len; ++k) lists[x].push_back(items[pos + k].code);
+        pos += len;
+      }
+    } else if (pol == 1) {
+      std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.work > b.work; });
+      long load[kX] = {0};
+      for (const Item& it : items) {
+        int x = 0;
+        for (int i = 1; i < kX; ++i)
+          if (load[i] < load[x]) x = i;
+        load[x] += it.work;
+        lists[x].push_back(it.code);
+      }
+    } else {
+      std::vector<std::vector<Item>> groups(K);
+      for (const Item& it : items) groups[it.prob].push_back(it);
+      std::vector<int> order;
+      std::vector<long> gw(K, 0);
+      for (int p = 0; p < K; ++p) {
+        for (const Item& it : groups[p]) gw[p] += it.work;
+        if (!groups[p].empty()) order.push_back(p);
+        std::stable_sort(groups[p].begin(), groups[p].end(),
+                         [](const Item& a, const Item& b) { return a.work > b.work; });
+      }
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gw[a] > gw[b]; });
+      long load[kX] = {0};
+      for (int p : order) {
+        int x = 0;
+        for (int i = 1; i < kX; ++i)
+          if (load[i] < load[x]) x = i;
+        load[x] += gw[p];
+        for (const Item& it : groups[p]) lists[x].push_back(it.code);
+      }
+    }
+    size_t len = 0;
+    for (auto& l : lists) len = std::max(len, l.size());
+    std::vector<int> out(len * kX, -1);
+    for (int x = 0; x < kX; ++x)
+      for (size_t k = 0; k < lists[x].size(); ++k) out[k * kX + x] = lists[x][k];
+    return out;
+  }
+
+  // Builds and uploads the slots of every launch for (policy, small_below):
+  // launches with fewer than small_below tasks run 64 x 64 sub-tiles, except
+  // the panel launches, whose tasks overwrite the tile they read (a quadrant
+  // would clobber rows its sibling quadrants still read).
+  void set_schedule(int pol, int below) {
+    std::vector<int> all;
+    auto one = [&](const Range& rg, Sched& sc, bool in_place) {
+      sc.tt = (!in_place && rg.count < below) ? 64 : 128;
+      sc.off = (int)all.size();
+      const std::vector<int> sl = rg.count ? build_slots(rg, sc.tt, pol) : std::vector<int>{};
+      sc.n = (int)sl.size();
+      all.insert(all.end(), sl.begin(), sl.end());
+    };
+    upd_s.assign(maxnt, Sched{});
+    pan_s.assign(maxnt, Sched{});
+    inv_s.assign(maxnt, Sched{});
+    for (int j = 0; j < maxnt; ++j) {
+      one(upd[j], upd_s[j], false);
+      one(pan[j], pan_s[j], true);
+      one(inv[j], inv_s[j], false);
+    }
+    one(lauum, lauum_s, false);
+    if (all.empty()) all.push_back(-1);
+    if (d_slots.n < all.size()) {
+      d_slots.release();
+      d_slots.alloc(all.size());
+    }
+    GPRX_HIP_CHECK(hipMemcpyAsync(d_slots.p, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, stream));
+    GPRX_HIP_CHECK(hipStreamSynchronize(stream));
+    policy = pol;
+    small_below = below;
+  }
+
   template <typename T>
-  void launch_tasks(const Range& rg, const T* Aar, const T* Bar, T* Car) {
-    if (rg.count == 0) return;
-    hipLaunchKernelGGL((k_gemm_tasks<T>), dim3(rg.count), dim3(256), 0, stream, tasks.p + rg.off,
-                       rg.count, Aar, Bar, Car);
+  void launch_tasks(const Sched& sc, T* X0, T* X1) {
+    if (sc.n == 0) return;
+    if (sc.tt == 64)
+      hipLaunchKernelGGL((k_gemm_tasks<T, 64>), dim3(sc.n), dim3(256), 0, stream, tasks.p,
+                         d_slots.p + sc.off, X0, X1);
+    else
+      hipLaunchKernelGGL((k_gemm_tasks<T, 128>), dim3(sc.n), dim3(256), 0, stream, tasks.p,
+                         d_slots.p + sc.off, X0, X1);
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
@@ -579,21 +740,84 @@ struct BatchModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
-  // Factor + inverse of every problem (arenas of type T).
+  // Factor + inverse of every problem: L (then Y) in Aar, X in Par.
   template <typename T>
   void factor(T* Aar, T* Par) {
-    for (int k = 0; k < maxnt; ++k) {
-      const Range& dg = step_diag[k];
+    for (int j = 0; j < maxnt; ++j) {
+      launch_tasks<T>(upd_s[j], Aar, Par);
+      const Range& dg = step_diag[j];
       potrf_diag_list<T>(Aar, Par, d_aoff.p + dg.off, d_xoff.p + dg.off, d_ld.p + dg.off,
                          d_logoff.p + dg.off, logd.p, info.p, d_infoidx.p + dg.off, dg.count, stream);
-      launch_tasks<T>(step_pa[k], Aar, Par, Aar);
-      launch_tasks<T>(step_pb[k], Par, Par, Par);
-      launch_tasks<T>(step_ta[k], Aar, Aar, Aar);
-      launch_tasks<T>(step_tb[k], Aar, Par, Par);
+      launch_tasks<T>(pan_s[j], Aar, Par);
     }
+    for (int d = 1; d < maxnt; ++d) launch_tasks<T>(inv_s[d], Aar, Par);
   }
 
+  // The schedule: timed once per candidate on this batch's own factor and
+  // X^T X (the launches the schedule shapes), the fastest kept.  Candidates:
+  // the three policies of build_slots, each with 64-tiles below 0, 256 and
+  // 1024 tasks per launch.  A pick is shared by batches of the same shape
+  // in the process.
+  void tune() {
+    if (policy >= 0) return;
+    const std::string key = shape_key();
+    {
+      std::lock_guard<std::mutex> lock(tune_mu());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end()) {
+        set_schedule(it->second.first, it->second.second);
+        return;
+      }
+    }
+    hipStream_t s = stream;
+    GPRX_HIP_CHECK(hipMemcpyAsync(progs.p, hprogs.p, sizeof(DevProgram) * K, hipMemcpyHostToDevice, s));
+    double best = 1e300;
+    int bp = 0, bb = 0;
+    for (int pol = 0; pol < 3; ++pol)
+      for (int below : {0, 256, 1024}) {
+        set_schedule(pol, below);
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // first pass warms caches and code
+          if (mixed()) {
+            fill<float>(A32.p);
+            GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
+            factor<float>(A32.p, P32.p);
+            launch_tasks<float>(lauum_s, A32.p, P32.p);
+          } else {
+            fill<double>(A64.p);
+            GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
+            factor<double>(A64.p, P64.p);
+            launch_tasks<double>(lauum_s, A64.p, P64.p);
+          }
+          GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
+          GPRX_HIP_CHECK(hipEventSynchronize(ev[1]));
+          GPRX_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        }
+        if (ms < best) {
+          best = ms;
+          bp = pol;
+          bb = below;
+        }
+      }
+    set_schedule(bp, bb);
+    tuned_ms = best;
+    std::lock_guard<std::mutex> lock(tune_mu());
+    tune_cache()[key] = {bp, bb};
+  }
 
+  std::string shape_key() const {
+    std::string k = std::to_string(precision) + ":" + std::to_string(dp);
+    for (const HostProb& h : hp) k += "," + std::to_string(h.nt);
+    return k;
+  }
+  static std::mutex& tune_mu() {
+    static std::mutex m;
+    return m;
+  }
+  static std::map<std::string, std::pair<int, int>>& tune_cache() {
+    static std::map<std::string, std::pair<int, int>> c;
+    return c;
+  }
 
   // The device part of one FP64 evaluation (capturable: kernels and
   // device copies only).
@@ -602,7 +826,7 @@ struct BatchModel : gprx_hip_model {
     factor<double>(A64.p, P64.p);
     trmv<double>(P64.p, yar.p, tv.p, nullptr, 0, 0);
     trmv<double>(P64.p, tv.p, nullptr, alpha.p, 0, 0);
-    launch_tasks<double>(lauum, P64.p, P64.p, A64.p);
+    launch_tasks<double>(lauum_s, A64.p, P64.p);
     hipLaunchKernelGGL((k_problem_scalars<double>), dim3(K), dim3(256), 0, stream, probs.p, yar.p,
                        alpha.p, logd.p, A64.p, scal.p);
     contract<double>(A64.p);
@@ -753,6 +977,7 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
       }
     }
     GPRX_HIP_CHECK(hipMemcpyAsync(B->progs.p, B->hprogs.p, sizeof(DevProgram) * B->K, hipMemcpyHostToDevice, s));
+    B->tune();
     GPRX_HIP_CHECK(hipMemsetAsync(B->info.p, 0, sizeof(int) * B->K, s));
     GPRX_HIP_CHECK(hipEventRecord(B->ev[0], s));
     if (B->precision == GPRX_FP64) {
@@ -824,7 +1049,7 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
           if (errs)
             set_error(errs + p, GPRX_CHOLESKY_FAILED, "mixed refinement did not converge (use FP64)");
         }
-      B->launch_tasks<float>(B->lauum, B->P32.p, B->P32.p, B->A32.p);
+      B->launch_tasks<float>(B->lauum_s, B->A32.p, B->P32.p);
       hipLaunchKernelGGL((k_problem_scalars<float>), dim3(B->K), dim3(256), 0, s, B->probs.p,
                          B->yar.p, B->alpha.p, B->logd.p, B->A32.p, B->scal.p);
       B->contract<float>(B->A32.p);
@@ -862,6 +1087,45 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
     }
     return GPRX_OK;
   });
+}
+
+int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t small_tile_below,
+                                gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kBatch || policy < -1 || policy > 2) {
+      set_error(err, GPRX_UNSUPPORTED, "bad handle or schedule policy");
+      return GPRX_UNSUPPORTED;
+    }
+    auto* B = static_cast<BatchModel*>(model);
+    // a captured evaluation holds the old slots: capture again
+    if (B->graph_exec) {
+      GPRX_HIP_CHECK(hipGraphExecDestroy(B->graph_exec));
+      B->graph_exec = nullptr;
+    }
+    if (B->graph) {
+      GPRX_HIP_CHECK(hipGraphDestroy(B->graph));
+      B->graph = nullptr;
+    }
+    if (policy < 0) {
+      B->policy = -1;
+      B->forced = false;
+      return GPRX_OK;
+    }
+    B->set_schedule(policy, small_tile_below < 0 ? 0 : small_tile_below);
+    B->forced = true;
+    B->tuned_ms = 0.0;
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy, int32_t* small_tile_below,
+                            double* tuned_ms) {
+  if (!model || model->kind != gprx_hip_model::kBatch) return GPRX_UNSUPPORTED;
+  const auto* B = static_cast<const BatchModel*>(model);
+  if (policy) *policy = B->policy;
+  if (small_tile_below) *small_tile_below = B->small_below;
+  if (tuned_ms) *tuned_ms = B->tuned_ms;
+  return GPRX_OK;
 }
 
 }  // extern "C"

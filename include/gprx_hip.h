@@ -219,6 +219,23 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
 int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* log_theta,
                                   double* nlml, double* const* grad, gprx_error* errs);
 
+/* Launch schedule of the batch's grouped products.  By default the first
+ * evaluation times every candidate on the batch itself and keeps the
+ * fastest (shared by batches of the same shape in the process).  A policy
+ * orders each launch's tiles over the 8 XCDs:
+ *   0 build order, one contiguous run of tiles per XCD
+ *   1 longest reduction first, each tile to the least-loaded XCD
+ *   2 problems as units, heaviest first to the least-loaded XCD
+ * and launches with fewer than small_tile_below tiles run 64 x 64 sub-tiles.
+ * set_schedule fixes one (policy -1 returns to tuning); schedule reports the
+ * one in use (policy -1 before the first evaluation) and the tuned time of
+ * the factor + inverse product in ms (0 when fixed by hand).  No counterpart
+ * in the reference (it evaluates problems one by one). */
+int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t small_tile_below,
+                                gprx_error* err);
+int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy,
+                            int32_t* small_tile_below, double* tuned_ms);
+
 /* ------------------------------------------------------------------ all */
 
 /* Frees a handle of any kind (NULL is ignored). */

@@ -51,3 +51,32 @@ def test_batch_matches_single_path(gpu):
         v = m.value_and_gradient_into(thetas[i], gr)
         assert_close(nlml[i], v, 1e-11, "nlml")
         assert_close(grads[i], gr, 1e-10, "grad")
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+@pytest.mark.parametrize("below", [0, 1 << 20])
+def test_every_schedule_matches_oracle(gpu, policy, below):
+    """Each launch schedule (tile order over the XCDs; 128- or 64-tiles)
+    computes the same products: FP64 results match the oracle."""
+    probs = _problems(k=5, seed=13)
+    print(f"[batch schedule] policy={policy} below={below}", flush=True)
+    b = g.GprBatch(probs)
+    b.set_schedule(policy, below)
+    thetas = [np.concatenate([p[0].get_params(), [p[1].log_param()]]) for p in probs]
+    nlml, grads, errs = b.value_and_gradient(thetas)
+    assert b.schedule()[:2] == (policy, below)
+    for i, (spec, lik, x, n, d, y) in enumerate(probs):
+        assert errs[i] is None, errs[i]
+        ov, og, _, _ = oracle.exact_value_grad(spec.program(), thetas[i], x, n, d, y)
+        assert_close(nlml[i], ov, 1e-10, f"nlml {i}")
+        assert_close(grads[i], og, 1e-9, f"grad {i}")
+
+
+def test_autotuner_picks_a_candidate(gpu):
+    probs = _problems(k=4, seed=14)
+    b = g.GprBatch(probs)
+    assert b.schedule()[0] == -1
+    thetas = [np.concatenate([p[0].get_params(), [p[1].log_param()]]) for p in probs]
+    b.value_and_gradient(thetas)
+    pol, below, ms = b.schedule()
+    assert pol in (0, 1, 2) and below in (0, 256, 1024)

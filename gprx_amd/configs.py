@@ -24,13 +24,18 @@ class ExactWorkload:
     lengthscales: np.ndarray
     constant: float
     noise: float
+    isotropic: bool = False  # RbfKernel(lengthscales[0]) alone
 
     def kernel(self):
-        from .kernels import ConstantKernel, RbfArdKernel
+        from .kernels import ConstantKernel, RbfArdKernel, RbfKernel
 
+        if self.isotropic:
+            return RbfKernel(float(self.lengthscales[0]))
         return ConstantKernel(self.constant) * RbfArdKernel(self.lengthscales)
 
     def theta(self) -> np.ndarray:
+        if self.isotropic:
+            return np.array([math.log(self.lengthscales[0]), math.log(self.noise)])
         return np.concatenate([[math.log(self.constant)], np.log(self.lengthscales), [math.log(self.noise)]])
 
 
@@ -56,7 +61,7 @@ def exact_rbf_small(n: int = 1024, d: int = 4, q: int = 256) -> ExactWorkload:
     X = x.reshape(d, n)
     y = np.sin(3 * X[0]) + np.cos(2 * X[1]) + 0.5 * X[2] * X[3] + 0.1 * eps
     xs = SeededRng(1).units(q * d)
-    return ExactWorkload(n, d, x, y, xs, q, np.ones(1), 1.0, 0.1)
+    return ExactWorkload(n, d, x, y, xs, q, np.ones(1), 1.0, 0.1, isotropic=True)
 
 
 def batched_problems(k: int = 256, seed: int = 5, n_lo: int = 256, n_hi: int = 4096):

@@ -25,6 +25,7 @@ constexpr double kLog2Pi = 1.8378770664093453;  // log(2 pi)
 struct ExactModel : gprx_hip_model {
   ExactModel() : gprx_hip_model(kExact) {}
   ~ExactModel() override {
+    fstreams.destroy();
     if (stream) {
       (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
@@ -43,6 +44,7 @@ struct ExactModel : gprx_hip_model {
   bool factored = false;
   int cur_form = GPRX_FP64;  // which buffers hold X and alpha
   hipStream_t stream = nullptr;
+  FactorStreams fstreams;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double phase_ms[5] = {0, 0, 0, 0, 0};
   DevBuf<double> x, y, ones;
@@ -61,7 +63,7 @@ struct ExactModel : gprx_hip_model {
     if (A64.p) return;
     A64.alloc(npad * npad);
     X64.alloc(npad * npad);
-    T64.alloc(ldt * (npad / 2 + kTile));
+    T64.alloc(ldt * factor_tmp_cols(nt) * kTile);
     GPRX_HIP_CHECK(hipMemsetAsync(A64.p, 0, sizeof(double) * npad * npad, stream));
     GPRX_HIP_CHECK(hipMemsetAsync(X64.p, 0, sizeof(double) * npad * npad, stream));
   }
@@ -69,7 +71,7 @@ struct ExactModel : gprx_hip_model {
     if (A32.p) return;
     A32.alloc(npad * npad);
     X32.alloc(npad * npad);
-    T32.alloc(ldt * (npad / 2 + kTile));
+    T32.alloc(ldt * factor_tmp_cols(nt) * kTile);
     K64.alloc(npad * npad);
     GPRX_HIP_CHECK(hipMemsetAsync(A32.p, 0, sizeof(float) * npad * npad, stream));
     GPRX_HIP_CHECK(hipMemsetAsync(X32.p, 0, sizeof(float) * npad * npad, stream));
@@ -98,7 +100,7 @@ struct ExactModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
     fill_lower<double>(dp, prog.p, pts(), A64.p, npad, nullptr, info.p, s);
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
-    factor_inverse<double>(A64.p, X64.p, T64.p, npad, ldt, nt, logdiag.p, info.p, s);
+    factor_inverse<double>(A64.p, X64.p, T64.p, npad, ldt, nt, logdiag.p, info.p, fstreams);
     GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
     gemv_n<double>(X64.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
     gemv_t<double>(X64.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
@@ -126,7 +128,7 @@ struct ExactModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
     fill_lower<float>(dp, prog.p, pts(), A32.p, npad, K64.p, info.p, s);
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
-    factor_inverse<float>(A32.p, X32.p, T32.p, npad, ldt, nt, logdiag.p, info.p, s);
+    factor_inverse<float>(A32.p, X32.p, T32.p, npad, ldt, nt, logdiag.p, info.p, fstreams);
     GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
     gemv_n<float>(X32.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
     gemv_t<float>(X32.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
@@ -374,6 +376,7 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
     m->precision = precision;
     m->theta.assign(log_theta, log_theta + p);
     GPRX_HIP_CHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    m->fstreams.init(m->stream);
     for (auto& e : m->ev) GPRX_HIP_CHECK(hipEventCreate(&e));
     hipStream_t s = m->stream;
     upload_points(m->x, x, (int)n, (int)d, m->npad, m->dp, s);

@@ -62,13 +62,51 @@ inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long l
   gemm<T>(ta, tb, g, s);
 }
 
+// Streams of the recursive factor: `main` carries the recursion; each
+// node's product Tmp = L21 X11, which only the node's last step reads, runs
+// on `side` meanwhile (event ev[2 k] hands L21 over at depth k, ev[2 k + 1]
+// hands Tmp back), overlapping the latency-bound trailing recursion.  With
+// side == nullptr everything runs on main.  Both streams keep the default
+// priority.
+constexpr int kMaxDepth = 24;
+struct FactorStreams {
+  hipStream_t main = nullptr, side = nullptr;
+  hipEvent_t ev[2 * kMaxDepth] = {};
+  void init(hipStream_t s) {
+    main = s;
+    GPRX_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (auto& e : ev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  void destroy() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+      side = nullptr;
+    }
+    for (auto& e : ev)
+      if (e) {
+        (void)hipEventDestroy(e);
+        e = nullptr;
+      }
+  }
+};
+
+// Columns (in 128-tiles) of the Tmp workspace of factor_inverse for nt
+// tiles: one block per recursion depth, ceil(nt/2) tiles of rows (ldt =
+// ceil(nt/2) * 128), since a depth's Tmp must outlive the deeper ones.
+inline long factor_tmp_cols(int nt) {
+  long cols = 0;
+  for (int s = nt; s > 1; s = s - s / 2) cols += s / 2;
+  return cols > 0 ? cols : 1;
+}
+
 // Recursive blocked Cholesky + triangular inverse of the lower part of the
 // nt*128 square A (ld), writing X = L^-1 (lower) into X and the log of every
-// pivot into logdiag.  A's lower triangle is consumed; Tmp holds
-// ceil(nt/2)*128 x floor(nt/2)*128 (ld ldt).
+// pivot into logdiag.  A's lower triangle is consumed; Tmp holds ldt x
+// factor_tmp_cols(nt) * 128 (ldt = ceil(nt/2) * 128).
 template <typename T>
 void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdiag, int* info,
-                    hipStream_t s);
+                    const FactorStreams& fs);
 
 }  // namespace gprx
 

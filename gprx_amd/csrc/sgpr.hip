@@ -52,6 +52,7 @@ int guarded(gprx_error* err, F&& f) {
 struct SgprModel : gprx_hip_model {
   SgprModel() : gprx_hip_model(kSgpr) {}
   ~SgprModel() override {
+    fstreams.destroy();
     if (stream) {
       (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
@@ -67,6 +68,7 @@ struct SgprModel : gprx_hip_model {
   double ynorm2 = 0.0, kmm_jitter = 0.0;
   bool factored = false;
   hipStream_t stream = nullptr;
+  FactorStreams fstreams;
   hipEvent_t ev[2] = {nullptr, nullptr};
   DevBuf<double> x, y, z;
   DevBuf<DevProgram> prog;
@@ -102,7 +104,7 @@ struct SgprModel : gprx_hip_model {
     upload_program(th, j);
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
     fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, s);
-    factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd_mm.p, info.p, s);
+    factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd_mm.p, info.p, fstreams);
     return sync_info();
   }
 
@@ -181,7 +183,7 @@ struct SgprModel : gprx_hip_model {
     mat_combine(BA.p, mpad, M, M, 1.0, S.p, mpad, 0.0, nullptr, 0, s);
     add_diag(BA.p, mpad, M, s2, s);
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), s));
-    factor_inverse<double>(BA.p, XB.p, Tm.p, mpad, ldt, mt, logd_B.p, info.p, s);
+    factor_inverse<double>(BA.p, XB.p, Tm.p, mpad, ldt, mt, logd_B.p, info.p, fstreams);
     gemv_n<double>(XB.p, mpad, M, M, b.p, tv.p, gpart.p, 1, 0, 0, s);
     gemv_t<double>(XB.p, mpad, M, M, tv.p, w.p, 1, 0, 0, 0, s);
     // B^-1 into BA (consumed by the factor), full storage.
@@ -388,6 +390,7 @@ int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, c
     M.ynorm2 = 0.0;
     for (size_t i = 0; i < n; ++i) M.ynorm2 += y[i] * y[i];
     GPRX_HIP_CHECK(hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking));
+    M.fstreams.init(M.stream);
     for (auto& e : M.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
     hipStream_t s = M.stream;
     exact_upload_points(M.x, x, (int)n, (int)d, M.npad, M.dp, s);
@@ -403,7 +406,7 @@ int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, c
       b->alloc(mm);
       GPRX_HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(double) * mm, s));
     }
-    M.Tm.alloc((size_t)M.ldt * (M.mpad / 2 + kTile));
+    M.Tm.alloc((size_t)M.ldt * factor_tmp_cols(M.mt) * kTile);
     const size_t un = (size_t)M.mpad * M.npad;
     if (precision == GPRX_MIXED) {
       M.U32.alloc(un);

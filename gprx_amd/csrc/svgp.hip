@@ -283,6 +283,7 @@ inline int grid_for(long total) { return (int)std::min<long>((total + 255) / 256
 struct SvgpModel : gprx_hip_model {
   SvgpModel() : gprx_hip_model(kSvgp) {}
   ~SvgpModel() override {
+    fstreams.destroy();
     if (stream) {
       (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
@@ -296,6 +297,7 @@ struct SvgpModel : gprx_hip_model {
   size_t plen = 0, ptot = 0;
   int precision = GPRX_FP64;
   hipStream_t stream = nullptr;
+  FactorStreams fstreams;
   hipEvent_t ev[2] = {nullptr, nullptr};
   DevBuf<double> x, y, z;
   DevBuf<DevProgram> prog;
@@ -348,7 +350,7 @@ struct SvgpModel : gprx_hip_model {
       upload_program(th, j);
       GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
       fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, stream);
-      factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd.p, info.p, stream);
+      factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd.p, info.p, fstreams);
       GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
       GPRX_HIP_CHECK(hipStreamSynchronize(stream));
       const int bits = *hinfo.p;
@@ -524,6 +526,7 @@ int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, c
     M.ptot = n_theta + m + M.plen;
     M.precision = precision;
     GPRX_HIP_CHECK(hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking));
+    M.fstreams.init(M.stream);
     for (auto& e : M.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
     hipStream_t s = M.stream;
     exact_upload_points(M.x, x, (int)n, (int)d, M.npad, M.dp, s);
@@ -537,7 +540,7 @@ int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, c
       p->alloc(mm);
       GPRX_HIP_CHECK(hipMemsetAsync(p->p, 0, sizeof(double) * mm, s));
     }
-    M.Tm.alloc((size_t)M.ldt * (M.mpad / 2 + kTile));
+    M.Tm.alloc((size_t)M.ldt * factor_tmp_cols(M.mt) * kTile);
     if (precision == GPRX_MIXED) {
       for (DevBuf<float>* p : {&M.X32, &M.L32, &M.Mq32, &M.Wpre32}) p->alloc(mm);
     } else {

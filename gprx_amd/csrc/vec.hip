@@ -140,8 +140,8 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 //   (L11, X11) = rec(A11)
 //   L21  = A21 X11^T                 (stored in X's 21 block meanwhile)
 //   A22 -= L21 L21^T                 (lower tiles)
+//   Tmp  = L21 X11                   (f64: side stream, concurrent with the update and:)
 //   (L22, X22) = rec(A22)
-//   Tmp  = L21 X11
 //   X21  = -X22 Tmp
 // Only X = L^-1 leaves the recursion (the exact path needs A^-1 = X^T X,
 // alpha = X^T X y and predictions through X), so L is never stored.
@@ -149,41 +149,63 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 // sitting at the top of the recursion.
 // ---------------------------------------------------------------------------
 template <typename T>
-static void rec(T* A, T* X, T* Tmp, long ld, long ldt, int o, int s, double* logdiag, int* info,
-                hipStream_t st) {
+static void rec(T* A, T* X, T* Tmp, long ld, long ldt, int o, int s, int depth, const long* toff,
+                double* logdiag, int* info, const FactorStreams& fs) {
+  hipStream_t st = fs.main;
   auto at = [&](T* base, int r, int c) { return base + (long)r * kTile + (long)c * kTile * ld; };
   if (s == 1) {
     potrf_diag<T>(at(A, o, o), ld, at(X, o, o), ld, logdiag + (long)o * kTile, info, 1, 0, 0, 0, st);
     return;
   }
+  if (depth >= kMaxDepth) throw DeviceError("factor recursion deeper than supported");
   const int s1 = s / 2, s2 = s - s1;
   const int o2 = o + s1;
-  rec<T>(A, X, Tmp, ld, ldt, o, s1, logdiag, info, st);
+  rec<T>(A, X, Tmp, ld, ldt, o, s1, depth + 1, toff, logdiag, info, fs);
   // L21 = A21 X11^T (X11 lower, stored N x K: k <= j)
   gemm_call<T>(false, true, at(A, o2, o), ld, at(X, o, o), ld, at(X, o2, o), ld, s2 * kTile,
                s1 * kTile, s1 * kTile, 1.0, 0.0, kKhiCol, 0, kOrderRowFastRev, st);
+  // Tmp = L21 X11 (X11 lower K x N: k >= j), this depth's block.  In f64
+  // it runs on the side stream from here on, overlapping the update and
+  // factor of the trailing block (measured on gfx950 at n = 16384: factor +
+  // inverse 69.7 ms against 74.1 ms on one stream; started after the
+  // update instead, 72.2 ms).  The f32 products are short enough that the
+  // overlap only delays the latency-bound launches of the trailing block
+  // (46.2 ms against 44.4 ms), so f32 stays on one stream.
+  const bool overlap = fs.side && std::is_same<T, double>::value;
+  T* Tk = Tmp + toff[depth] * kTile * ldt;
+  hipStream_t ts = st;
+  if (overlap) {
+    GPRX_HIP_CHECK(hipEventRecord(fs.ev[2 * depth], st));
+    GPRX_HIP_CHECK(hipStreamWaitEvent(fs.side, fs.ev[2 * depth], 0));
+    ts = fs.side;
+  }
+  gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tk, ldt, s2 * kTile, s1 * kTile,
+               s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, ts);
+  if (overlap) GPRX_HIP_CHECK(hipEventRecord(fs.ev[2 * depth + 1], fs.side));
   // A22 -= L21 L21^T
   gemm_call<T>(false, true, at(X, o2, o), ld, at(X, o2, o), ld, at(A, o2, o2), ld, s2 * kTile,
                s2 * kTile, s1 * kTile, -1.0, 1.0, 0, 1, kOrderRowFast, st);
-  rec<T>(A, X, Tmp, ld, ldt, o2, s2, logdiag, info, st);
-  // Tmp = L21 X11 (X11 lower K x N: k >= j)
-  gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tmp, ldt, s2 * kTile, s1 * kTile,
-               s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, st);
-  // X21 = -X22 Tmp (X22 lower: k <= i)
-  gemm_call<T>(false, false, at(X, o2, o2), ld, Tmp, ldt, at(X, o2, o), ld, s2 * kTile, s1 * kTile,
+  rec<T>(A, X, Tmp, ld, ldt, o2, s2, depth + 1, toff, logdiag, info, fs);
+  // X21 = -X22 Tmp (X22 lower: k <= i), over L21's storage
+  if (overlap) GPRX_HIP_CHECK(hipStreamWaitEvent(st, fs.ev[2 * depth + 1], 0));
+  gemm_call<T>(false, false, at(X, o2, o2), ld, Tk, ldt, at(X, o2, o), ld, s2 * kTile, s1 * kTile,
                s2 * kTile, -1.0, 0.0, kKhiRow, 0, kOrderColFastRev, st);
 }
 
 template <typename T>
 void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdiag, int* info,
-                    hipStream_t s) {
-  rec<T>(A, X, Tmp, ld, ldt, 0, nt, logdiag, info, s);
+                    const FactorStreams& fs) {
+  // Tmp column offset (tiles) of each depth, along the widest path
+  long toff[kMaxDepth + 1] = {0};
+  int depth = 0;
+  for (int s = nt; s > 1 && depth < kMaxDepth; s = s - s / 2, ++depth) toff[depth + 1] = toff[depth] + s / 2;
+  rec<T>(A, X, Tmp, ld, ldt, 0, nt, 0, toff, logdiag, info, fs);
 }
 
 template void factor_inverse<double>(double*, double*, double*, long, long, int, double*, int*,
-                                     hipStream_t);
+                                     const FactorStreams&);
 template void factor_inverse<float>(float*, float*, float*, long, long, int, double*, int*,
-                                    hipStream_t);
+                                    const FactorStreams&);
 
 }  // namespace gprx
 

@@ -70,7 +70,7 @@ __device__ __forceinline__ void run_task(const TileTask& t, int quad, T* X0, T* 
 #pragma unroll
     for (int j = 0; j < FB; ++j) acc[i][j] = AV{};
   tile_mainloop<T, TA, TB, TT>(Ab, t.ld, Bb, t.ld, 0, t.K / Cfg<T, TT>::BK, sm, acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
   const bool diag = t.lower && qr == qc;
 #pragma unroll

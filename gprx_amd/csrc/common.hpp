@@ -43,6 +43,14 @@ enum LeafKind : int32_t { kLeafConst = 0, kLeafRbf = 1, kLeafArd = 2, kLeafPerio
 constexpr int kAccPerLeaf = 2;
 constexpr int kMaxAcc = kMaxSlots * kAccPerLeaf + kMaxDim;
 
+// Index of the calling wave in its workgroup, read through the scalar unit:
+// the compiler cannot prove threadIdx.x >> 6 uniform, and branches or
+// register indices that depend on a value it takes as divergent become
+// exec-mask control flow and per-lane selects.
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 struct DevProgram {
   int32_t ns;       // slots
   int32_t d;        // true input dimension (<= padded dimension)

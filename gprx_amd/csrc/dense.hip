@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
   }
   const int ti = tc.ti, tj = tc.tj;
   const int nk = tc.ke > tc.kb ? (tc.ke - tc.kb) / Cfg<T, TT>::BK : 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
   AV acc[FB][FB];
 #pragma unroll
@@ -247,8 +247,13 @@ template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 // ---------------------------------------------------------------------------
 constexpr int DS = 129;
 
+// X[r][c] from the block's LDS layout (xd[kTile] holds 0.0).  The address
+// is selected and one load issued: a selected value would be turned back
+// into exec-mask branches around the loads, which keeps the compiler from
+// issuing the operands of several MFMAs ahead of their use.
 __device__ __forceinline__ double x_at(const double* s, const double* xd, int r, int c) {
-  return r > c ? s[r * DS + c] : (r == c ? xd[r] : 0.0);
+  const double* p = r > c ? s + r * DS + c : (r == c ? xd + r : xd + kTile);
+  return *p;
 }
 
 // Value of v in lane `src` of the wave, src uniform (a constant once the
@@ -284,70 +289,185 @@ __device__ __forceinline__ double rsqrt_nr(double p) {
 //   X21 = -X22 T    (X22 lower: k <= row)     -> X's transposed storage
 // L21 is read by other waves while T is formed, so T is held in registers
 // until a barrier; X21's slots are read by no one in the second product.
-template <int HB>
-__device__ __forceinline__ void inverse_level(double* s, const double* xd) {
-  constexpr int kPairs = 8 / (2 * HB), kPer = HB * HB;
-  constexpr int kTasks = kPairs * kPer, kMine = (kTasks + 3) / 4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  d4 acc[kMine];
+// Wave w takes HB blocks (ui, uj) = (i, (w + i) mod HB), i < HB, of pair
+// w * pairs / 4: every wave then has the same reduction lengths in both
+// products (sum over uj of HB - uj, and over ui of ui + 1), and its HB
+// accumulator chains advance together in one K loop.
+template <int HB, int W>
+__device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) {
+  constexpr int kPairs = 8 / (2 * HB);
+  constexpr int o = (W * kPairs / 4) * 2 * HB;
+  const int lane = threadIdx.x & 63;
+  d4 acc[HB];
 #pragma unroll
-  for (int i = 0; i < kMine; ++i) {
-    acc[i] = d4{};
-    const int t = wave + 4 * i;
-    if (t >= kTasks) continue;
-    const int o = (t / kPer) * 2 * HB, u = t % kPer;
-    const int bi = o + HB + u / HB, bj = o + u % HB;
-    for (int kb = bj; kb < o + HB; ++kb) {
+  for (int i = 0; i < HB; ++i) acc[i] = d4{};
+  // operands of every chain for one k step are loaded before its MFMAs, so
+  // one LDS round trip covers up to HB products; every bound is static
 #pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const int k = kb * 16 + kk + (lane >> 4);
-        const double a = s[k * DS + bi * 16 + (lane & 15)];
-        const double b = x_at(s, xd, k, bj * 16 + (lane & 15));
-        acc[i] = mfma16(a, b, acc[i]);
+  for (int kb = 0; kb < HB; ++kb)
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const int k = (o + kb) * 16 + kk + (lane >> 4);
+      double a[HB], b[HB];
+#pragma unroll
+      for (int i = 0; i < HB; ++i) {
+        const int bi = o + HB + i, bj = o + (W + i) % HB;
+        if (kb >= (W + i) % HB) {  // X11 is lower: k >= column block
+          a[i] = s[k * DS + bi * 16 + (lane & 15)];
+          b[i] = x_at(s, xd, k, bj * 16 + (lane & 15));
+        }
       }
+#pragma unroll
+      for (int i = 0; i < HB; ++i)
+        if (kb >= (W + i) % HB) acc[i] = mfma16(a[i], b[i], acc[i]);
     }
-  }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kMine; ++i) {
-    const int t = wave + 4 * i;
-    if (t >= kTasks) continue;
-    const int o = (t / kPer) * 2 * HB, u = t % kPer;
-    const int bi = o + HB + u / HB, bj = o + u % HB;
+  for (int i = 0; i < HB; ++i) {
+    const int bi = o + HB + i, bj = o + (W + i) % HB;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       s[(bj * 16 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[i][q];
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kMine; ++i) {
-    const int t = wave + 4 * i;
-    if (t >= kTasks) continue;
-    const int o = (t / kPer) * 2 * HB, u = t % kPer;
-    const int bi = o + HB + u / HB, bj = o + u % HB;
-    d4 x = d4{};
-    for (int kb = o + HB; kb <= bi; ++kb) {
+  for (int i = 0; i < HB; ++i) acc[i] = d4{};
 #pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const int k = kb * 16 + kk + (lane >> 4);
-        const double a = x_at(s, xd, bi * 16 + (lane & 15), k);
-        const double b = s[(bj * 16 + (lane & 15)) * DS + k];
-        x = mfma16(a, b, x);
+  for (int kb = 0; kb < HB; ++kb)
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const int k = (o + HB + kb) * 16 + kk + (lane >> 4);
+      double a[HB], b[HB];
+#pragma unroll
+      for (int i = kb; i < HB; ++i) {  // X22 is lower: k <= row block
+        const int bi = o + HB + i, bj = o + (W + i) % HB;
+        a[i] = x_at(s, xd, bi * 16 + (lane & 15), k);
+        b[i] = s[(bj * 16 + (lane & 15)) * DS + k];
       }
+#pragma unroll
+      for (int i = kb; i < HB; ++i) acc[i] = mfma16(a[i], b[i], acc[i]);
     }
 #pragma unroll
+  for (int i = 0; i < HB; ++i) {
+    const int bi = o + HB + i, bj = o + (W + i) % HB;
+#pragma unroll
     for (int q = 0; q < 4; ++q)
-      s[(bi * 16 + drow<double>(lane, q)) * DS + bj * 16 + (lane & 15)] = -x[q];
+      s[(bi * 16 + drow<double>(lane, q)) * DS + bj * 16 + (lane & 15)] = -acc[i][q];
   }
   __syncthreads();
+}
+
+// Dispatch on the (scalar) wave index, so each wave's reduction bounds are
+// compile-time constants: data-dependent guards on MFMA accumulators make
+// the compiler copy every accumulator at each merge.
+template <int HB>
+__device__ __forceinline__ void inverse_level(double* s, const double* xd) {
+  switch (wave_index()) {
+    case 0: inverse_level_wave<HB, 0>(s, xd); break;
+    case 1: inverse_level_wave<HB, 1>(s, xd); break;
+    case 2: inverse_level_wave<HB, 2>(s, xd); break;
+    default: inverse_level_wave<HB, 3>(s, xd); break;
+  }
+}
+
+// Stage (a) of the diagonal block, run by one wave: factors the 16 x 16
+// block at (c0, c0) of s in registers and writes L (lower), X = L^-1
+// (transposed storage above the diagonal, diagonal in xd) and log L_ii.
+__device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag, int* sbad, int c0) {
+  const int lane = threadIdx.x & 63;
+  // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k], xr[k] =
+  // X[r][k].  Entries right of the diagonal (k > r) start at zero and
+  // collect values no lower entry ever reads, so every update runs
+  // unconditionally.  Step j: pivot p = A[j][j], rs = p^-1/2; column j
+  // of L is d[j] * rs (lane j: p rs = sqrt(p)); rank-1 update of the
+  // columns right of j; row j of X = L^-1 is
+  //   X[j][c] = -rs * sum_{k<j} L[j][k] X[k][c],  X[j][j] = rs,
+  // whose sums (acc) gain column j of L as soon as it is final.
+  const int r = lane & 15;
+  double d[16], xr[16], acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    d[k] = s[(c0 + k) * DS + c0 + r];
+    xr[k] = 0.0;
+    acc[k] = 0.0;
+  }
+  bool bad = false;
+  double lrr = 1.0, xrr = 1.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    double p = lane_bcast(d[j], j);
+    if (!(p > 0.0) || !isfinite(p)) {
+      bad = true;
+      p = 1.0;
+    }
+    const double rs = rsqrt_nr(p);
+    d[j] *= rs;
+#pragma unroll
+    for (int c = j + 1; c < 16; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
+    if (r == j) {
+#pragma unroll
+      for (int c = 0; c < j; ++c) xr[c] = -acc[c] * rs;
+      xr[j] = rs;
+      lrr = d[j];
+      xrr = rs;
+    }
+#pragma unroll
+    for (int c = 0; c <= j; ++c) acc[c] = fma(d[j], lane_bcast(xr[c], j), acc[c]);
+  }
+  if (bad && lane == 0) *sbad = 1;
+  if (lane < 16) {
+    // Every register index is a compile-time constant: a lane-dependent
+    // index (d[r]) would move the arrays to scratch memory.
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
+      if (k < r) s[(c0 + r) * DS + c0 + k] = xr[k];
+    }
+    xd[c0 + r] = xrr;
+    logdiag[c0 + r] = log(lrr);
+  }
+}
+
+// A_(bi,bj) -= L_bi L_bj^T (panel columns c0 .. c0+15) for one block, or two
+// when `two` (independent chains interleaved); diagonal blocks keep r >= c.
+__device__ __forceinline__ void update_blocks(double* s, int c0, int bi, int bj, bool two, int bi2,
+                                              int bj2) {
+  const int lane = threadIdx.x & 63;
+  d4 acc = d4{}, acc2 = d4{};
+#pragma unroll
+  for (int kk = 0; kk < 16; kk += 4) {
+    const int kr = kk + (lane >> 4);
+    const double a = s[(c0 + kr) * DS + bi * 16 + (lane & 15)];
+    const double b = s[(c0 + kr) * DS + bj * 16 + (lane & 15)];
+    acc = mfma16(a, b, acc);
+    if (two) {
+      const double a2 = s[(c0 + kr) * DS + bi2 * 16 + (lane & 15)];
+      const double b2 = s[(c0 + kr) * DS + bj2 * 16 + (lane & 15)];
+      acc2 = mfma16(a2, b2, acc2);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = bi * 16 + drow<double>(lane, q);
+    const int col = bj * 16 + (lane & 15);
+    if (bi != bj || row >= col) s[col * DS + row] -= acc[q];
+    if (two) {
+      const int row2 = bi2 * 16 + drow<double>(lane, q);
+      const int col2 = bj2 * 16 + (lane & 15);
+      if (bi2 != bj2 || row2 >= col2) s[col2 * DS + row2] -= acc2[q];
+    }
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
                                 double* __restrict__ logdiag, int* info, double* s, double* xd,
                                 int* sbad) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) *sbad = 0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
+  if (tid == 0) {
+    *sbad = 0;
+    xd[kTile] = 0.0;  // the zero x_at reads above the diagonal
+  }
   // 16 loads in flight per thread: the whole block lies inside the matrix,
   // so the upper part is read and dropped rather than branched around.
   constexpr int kBatch = 16;
@@ -368,63 +488,15 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
   }
   __syncthreads();
 
+  // Right-looking over 16-column panels with lookahead: while waves 1-3
+  // apply panel jb to the trailing matrix, wave 0 applies it to diagonal
+  // block jb + 1 and factors that block (stage (a) is a serial chain on one
+  // wave, so it runs beside the bulk of the update instead of after it).
+  if (wave == 0) factor16(s, xd, logdiag, sbad, 0);
+  __syncthreads();
   for (int jb = 0; jb < 8; ++jb) {
     const int c0 = jb * 16;
-    if (wave == 0) {
-      // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k], xr[k] =
-      // X[r][k].  Entries right of the diagonal (k > r) start at zero and
-      // collect values no lower entry ever reads, so every update runs
-      // unconditionally.  Step j: pivot p = A[j][j], rs = p^-1/2; column j
-      // of L is d[j] * rs (lane j: p rs = sqrt(p)); rank-1 update of the
-      // columns right of j; row j of X = L^-1 is
-      //   X[j][c] = -rs * sum_{k<j} L[j][k] X[k][c],  X[j][j] = rs,
-      // whose sums (acc) gain column j of L as soon as it is final.
-      const int r = lane & 15;
-      double d[16], xr[16], acc[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        d[k] = s[(c0 + k) * DS + c0 + r];
-        xr[k] = 0.0;
-        acc[k] = 0.0;
-      }
-      bool bad = false;
-      double lrr = 1.0, xrr = 1.0;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        double p = lane_bcast(d[j], j);
-        if (!(p > 0.0) || !isfinite(p)) {
-          bad = true;
-          p = 1.0;
-        }
-        const double rs = rsqrt_nr(p);
-        d[j] *= rs;
-#pragma unroll
-        for (int c = j + 1; c < 16; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
-        if (r == j) {
-#pragma unroll
-          for (int c = 0; c < j; ++c) xr[c] = -acc[c] * rs;
-          xr[j] = rs;
-          lrr = d[j];
-          xrr = rs;
-        }
-#pragma unroll
-        for (int c = 0; c <= j; ++c) acc[c] = fma(d[j], lane_bcast(xr[c], j), acc[c]);
-      }
-      if (bad && lane == 0) *sbad = 1;
-      if (lane < 16) {
-        // Every register index is a compile-time constant: a lane-dependent
-        // index (d[r]) would move the arrays to scratch memory.
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
-          if (k < r) s[(c0 + r) * DS + c0 + k] = xr[k];
-        }
-        xd[c0 + r] = xrr;
-        logdiag[c0 + r] = log(lrr);
-      }
-    }
-    __syncthreads();
-    // (b) panel
+    // (b) panel: L_i = A_i X_D^T for the blocks below
     for (int bi = jb + 1 + wave; bi < 8; bi += 4) {
       d4 acc = d4{};
 #pragma unroll
@@ -439,26 +511,24 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
       for (int q = 0; q < 4; ++q) s[(c0 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[q];
     }
     __syncthreads();
-    // (c) trailing update
-    const int nb = 7 - jb;
-    const int count = nb * (nb + 1) / 2;
-    for (int t = wave; t < count; t += 4) {
-      int u, v;
-      tri_index_small(t, u, v);
-      const int bi = jb + 1 + u, bj = jb + 1 + v;
-      d4 acc = d4{};
-#pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        const double a = s[(c0 + kr) * DS + bi * 16 + (lane & 15)];
-        const double b = s[(c0 + kr) * DS + bj * 16 + (lane & 15)];
-        acc = mfma16(a, b, acc);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = bi * 16 + drow<double>(lane, q);
-        const int col = bj * 16 + (lane & 15);
-        if (bi != bj || row >= col) s[col * DS + row] -= acc[q];
+    if (jb == 7) break;
+    // (c) trailing update A_ij -= L_i L_j^T over the lower blocks of
+    // (jb+1 .. 7)^2: block (jb+1, jb+1) and stage (a) on wave 0, the others
+    // two at a time on waves 1-3.
+    if (wave == 0) {
+      update_blocks(s, c0, jb + 1, jb + 1, false, 0, 0);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      factor16(s, xd, logdiag, sbad, c0 + 16);
+    } else {
+      const int nb = 7 - jb;
+      const int count = nb * (nb + 1) / 2;
+      for (int t = wave; t < count; t += 6) {
+        int u, v;
+        tri_index_small(t, u, v);
+        int u2 = 0, v2 = 0;
+        const bool two = t + 3 < count;
+        if (two) tri_index_small(t + 3, u2, v2);
+        update_blocks(s, c0, jb + 1 + u, jb + 1 + v, two, jb + 1 + u2, jb + 1 + v2);
       }
     }
     __syncthreads();
@@ -469,10 +539,19 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
   inverse_level<1>(s, xd);
   inverse_level<2>(s, xd);
   inverse_level<4>(s, xd);
-#pragma unroll 16
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    const int r = e & 127, c = e >> 7;
-    X[r + (long)c * ldx] = (T)x_at(s, xd, r, c);
+#pragma unroll
+  for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int e = tid + 256 * (b0 + i);
+      v[i] = x_at(s, xd, e & 127, e >> 7);
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int e = tid + 256 * (b0 + i);
+      X[(e & 127) + (long)(e >> 7) * ldx] = (T)v[i];
+    }
   }
   if (tid == 0 && *sbad) atomicExch(info, 1);
 }
@@ -484,7 +563,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const T* A, long lda, T* X, 
   extern __shared__ double smem[];
   double* s = smem;
   double* xd = smem + kTile * DS;
-  int* sbad = reinterpret_cast<int*>(xd + kTile);
+  int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const long z = blockIdx.x;
   potrf_block_lds<T>(A + z * sa, lda, X + z * sx, ldx, logdiag + z * slog, info, s, xd, sbad);
 }
@@ -497,13 +576,13 @@ __global__ __launch_bounds__(256) void k_potrf_diag_list(T* A, T* X, const long*
   extern __shared__ double smem[];
   double* s = smem;
   double* xd = smem + kTile * DS;
-  int* sbad = reinterpret_cast<int*>(xd + kTile);
+  int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const int b = blockIdx.x;
   potrf_block_lds<T>(A + a_off[b], ld[b], X + x_off[b], ld[b], logdiag + log_off[b],
                      info + info_idx[b], s, xd, sbad);
 }
 
-constexpr size_t kDiagLds = (size_t)kTile * DS * sizeof(double) + kTile * sizeof(double) + 16;
+constexpr size_t kDiagLds = (size_t)kTile * DS * sizeof(double) + (kTile + 1) * sizeof(double) + 16;
 
 template <typename T>
 static void set_diag_lds_once() {

@@ -97,7 +97,7 @@ template <int DP>
 __device__ __forceinline__ void store_partials(double (&acc)[kMaxSlots][kAccPerLeaf],
                                                double (&accd)[DP], double* red /* [4][kMaxAcc] */,
                                                double* out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_index();
 #pragma unroll
   for (int l = 0; l < kMaxSlots; ++l)
 #pragma unroll

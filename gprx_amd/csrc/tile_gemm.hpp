@@ -138,7 +138,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   using V = typename Vec<T>::V;
   constexpr int NV = TT * BK / VEC / 256;  // 16-byte vectors per thread per operand
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
   V ra[P][NV], rb[P][NV];
   auto gload = [&](int k0, V(&xa)[NV], V(&xb)[NV]) {

@@ -1,21 +1,21 @@
 // Many independent exact-GPR problems of uneven n in grouped launches.
 //
 // Every problem is padded to whole 128-tiles and lives in two arenas (the
-// kernel matrix / its factor, and the inverse X = L^-1).  The factor is
-// right-looking and level-synchronous: step k of every problem with more
-// than k tiles runs in the same three launches,
-//   1. diagonal block k: Cholesky + inverse in LDS (f64)           -> X_kk
-//   2. panel L_ik = A_ik X_kk^T (i > k) and P_kj <- X_kk P_kj (j < k)
-//   3. trailing A_ij -= L_ik L_jk^T (k < j <= i) and P_ij -= L_ik P_kj
-//      (i > k, j <= k)
-// where P accumulates L^-1 = E_{nt-1}^-1 ... E_0^-1 (E_k the block column k
-// of L), so the inverse costs no extra pass: its updates ride in the
-// factor's launches.  Then A^-1 = X^T X, alpha, the NLML and the fused
-// gradient contraction run as one grouped launch each.  Each launch reads a
-// task list (one 128 x 128 output tile per workgroup) built once at set-up,
-// longest reductions first.  The whole evaluation can be captured in one
-// hipGraph and replayed per optimizer iteration with only the per-problem
-// parameters re-uploaded.
+// kernel matrix, then L, Y and A^-1; and X = L^-1).  The factor is
+// left-looking and level-synchronous: step j of every problem with more than
+// j tiles runs in the same three launches,
+//   1. column update A_ij -= L_i,0:j L_j,0:j^T, i >= j (one tile task each,
+//      reduction length 128 j)
+//   2. diagonal block j: Cholesky + inverse in LDS (f64) -> X_jj
+//   3. panel L_ij = A_ij X_jj^T (i > j) and Y_jk = X_jj L_jk (k < j)
+// then the inverse by distance, X_ij = -sum_k Y_ik X_kj (reduction 128 (i-j)),
+// A^-1 = X^T X, alpha and the fused gradient contraction, each as one
+// grouped launch.  Task lists are built once; each launch runs a slot
+// schedule over the 8 XCDs picked by timing candidates on the batch itself.
+// The FP64 evaluation is captured in one hipGraph and replayed per optimizer
+// iteration with only the per-problem parameters re-uploaded; the mixed form
+// refines alpha with f64 residuals, tile matrix-vector products restricted
+// to the problems still refining.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -222,13 +222,13 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
   store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);
 }
 
-// Sums each problem's partial rows [first[p], first[p+1]) into out[p].
-__global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial, const int* first,
-                                                           double* out) {
+// Sums each problem's partial rows [beg[p], end[p]) into out[p].
+__global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial, const int* beg,
+                                                           const int* end, double* out) {
   __shared__ double red[256];
   const int p = blockIdx.x / kMaxAcc, k = blockIdx.x % kMaxAcc;
   double acc = 0.0;
-  for (int t = first[p] + threadIdx.x; t < first[p + 1]; t += 256) acc += partial[(long)t * kMaxAcc + k];
+  for (int t = beg[p] + threadIdx.x; t < end[p]; t += 256) acc += partial[(long)t * kMaxAcc + k];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -238,53 +238,86 @@ __global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial
   if (threadIdx.x == 0) out[(long)p * kMaxAcc + k] = red[0];
 }
 
-// out_v[i] = sum_{c <= i (strict: c < i)} M[i, c] v[c] per problem; one
-// thread per row, one workgroup per (problem, 256-row block).
-struct RowTask {
-  int prob, r0;
-};
-
+// Matrix-vector products over lower 128-tiles (prob, ti, tj), accumulated
+// with atomics into out (cleared beforehand):
+//   row: out[ti-rows] += M_tile v[tj-cols]      (diagonal tile: c <= r, or c < r if row_strict)
+//   col: out[tj-cols] += M_tile^T v[ti-rows]    (diagonal tile: r >= c, or r > c if col_strict)
+// abs_mode uses |M|.  The tile streams through LDS in 32-column slabs with
+// coalesced loads; row and column partial sums are both formed from the
+// slab, so a symmetric product (row + strict col) reads the lower triangle
+// once.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trmv_n_tasks(const RowTask* tasks, const ProbDesc* probs,
-                                                      const T* Mar, const double* v, double* out,
-                                                      int strict, int abs_mode) {
-  const RowTask t = tasks[blockIdx.x];
+__global__ __launch_bounds__(256) void k_tile_mv(const FillTask* __restrict__ tasks,
+                                                 const ProbDesc* __restrict__ probs,
+                                                 const T* __restrict__ Mar, const double* __restrict__ v,
+                                                 double* __restrict__ out, int row, int col,
+                                                 int row_strict, int col_strict, int abs_mode) {
+  __shared__ double slab[32][kTile + 1];
+  __shared__ double vr[kTile], vc[kTile];
+  __shared__ double rpart[2][kTile];
+  __shared__ double cpart[8][32];
+  const int tid = threadIdx.x;
+  const FillTask t = tasks[blockIdx.x];
   const ProbDesc pd = probs[t.prob];
-  const int i = t.r0 + threadIdx.x;
-  if (i >= pd.npad) return;
-  const T* M = Mar + pd.moff;
-  const double* vv = v + pd.voff;
-  const int c1 = strict ? i : i + 1;
-  double acc = 0.0;
-  for (int c = 0; c < c1; ++c) {
-    const double m = (double)M[i + (long)c * pd.npad];
-    acc += (abs_mode ? fabs(m) : m) * vv[c];
+  const int r0 = t.ti * kTile, c0 = t.tj * kTile;
+  const T* M = Mar + pd.moff + r0 + (long)c0 * pd.npad;
+  if (tid < kTile) {
+    vr[tid] = v[pd.voff + r0 + tid];
+    vc[tid] = v[pd.voff + c0 + tid];
   }
-  out[pd.voff + i] = acc;
+  const bool diag = t.ti == t.tj;
+  const int rr = tid & 127, half = tid >> 7;
+  double racc = 0.0;
+  for (int sb = 0; sb < 4; ++sb) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = tid + 256 * i;
+      const int r = e & 127, cc = e >> 7;
+      const double m = (double)M[r + (long)(sb * 32 + cc) * pd.npad];
+      slab[cc][r] = abs_mode ? fabs(m) : m;
+    }
+    __syncthreads();
+    if (row) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int cc = half * 16 + j, c = sb * 32 + cc;
+        const bool in = !diag || c < rr || (c == rr && !row_strict);
+        racc += in ? slab[cc][rr] * vc[c] : 0.0;
+      }
+    }
+    if (col) {
+      const int cc = tid & 31, part = tid >> 5, c = sb * 32 + cc;
+      double cacc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = part * 16 + j;
+        const bool in = !diag || r > c || (r == c && !col_strict);
+        cacc += in ? slab[cc][r] * vr[r] : 0.0;
+      }
+      cpart[part][cc] = cacc;
+    }
+    __syncthreads();
+    if (col && tid < 32) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sum += cpart[q][tid];
+      unsafeAtomicAdd(out + pd.voff + c0 + sb * 32 + tid, sum);
+    }
+  }
+  if (row) {
+    rpart[half][rr] = racc;
+    __syncthreads();
+    if (tid < kTile) unsafeAtomicAdd(out + pd.voff + r0 + tid, rpart[0][tid] + rpart[1][tid]);
+  }
 }
 
-// out_v[j] = sum_{i >= j (strict: i > j)} M[i, j] v[i]; one wave per column.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trmv_t_tasks(const RowTask* tasks, const ProbDesc* probs,
-                                                      const T* Mar, const double* v, double* out,
-                                                      int strict, int abs_mode) {
-  const RowTask t = tasks[blockIdx.x];
-  const ProbDesc pd = probs[t.prob];
-  const int lane = threadIdx.x & 63;
-  const T* M = Mar + pd.moff;
-  const double* vv = v + pd.voff;
-  for (int jj = threadIdx.x >> 6; jj < 256; jj += 4) {
-    const int j = t.r0 + jj;
-    if (j >= pd.npad) break;
-    double acc = 0.0;
-    for (int i = (strict ? j + 1 : j) + lane; i < pd.npad; i += 64) {
-      const double m = (double)M[i + (long)j * pd.npad];
-      acc += (abs_mode ? fabs(m) : m) * vv[i];
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) out[pd.voff + j] = acc;
-  }
+__global__ void k_zero(double* p, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0.0;
+}
+
+__global__ void k_fill_value(double* p, long n, double v) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = v;
 }
 
 // Per-problem scalars, one block each:
@@ -337,12 +370,12 @@ __global__ __launch_bounds__(256) void k_problem_residual(const ProbDesc* probs,
       r[k] = 0.0;
       continue;
     }
-    const double ri = y[k] - (p1[k] + p2[k]);
+    const double ri = y[k] - (p2 ? p1[k] + p2[k] : p1[k]);
     r[k] = ri;
     a0 = fmax(a0, fabs(ri));
     a1 = fmax(a1, fabs(alpha[k]));
     a2 = fmax(a2, fabs(y[k]));
-    if (q1) a3 = fmax(a3, q1[k] + q2[k]);
+    if (q1) a3 = fmax(a3, q2 ? q1[k] + q2[k] : q1[k]);
   }
   m0[threadIdx.x] = a0;
   m1[threadIdx.x] = a1;
@@ -426,7 +459,7 @@ struct BatchModel : gprx_hip_model {
   DevBuf<ProbDesc> probs;
   DevBuf<DevProgram> progs;
   HostBuf<DevProgram> hprogs;
-  DevBuf<int> info, active, first;
+  DevBuf<int> info, active;
   HostBuf<int> hinfo, hactive;
   HostBuf<double> hscal, hacc;
   // Task lists (host copy kept for the schedules).
@@ -455,8 +488,15 @@ struct BatchModel : gprx_hip_model {
   std::vector<Range> step_diag;
   DevBuf<FillTask> fill_tasks;
   int n_fill = 0;
-  DevBuf<RowTask> row_tasks;
-  int n_rows = 0;
+  std::vector<FillTask> hfill;  // host copy; problem p owns [hbeg[p], hend[p])
+  std::vector<int> hbeg, hend;
+  DevBuf<int> d_fbeg, d_fend;
+  struct DimGroup {
+    int dp, off, count;
+  };
+  std::vector<DimGroup> dp_groups;
+  DevBuf<FillTask> act_tasks;   // lower tiles of the problems still refining
+  HostBuf<FillTask> hact;
 
   bool mixed() const { return precision == GPRX_MIXED; }
 
@@ -552,22 +592,32 @@ struct BatchModel : gprx_hip_model {
     upload(d_ld, ld, stream);
     upload(d_logoff, logoff, stream);
     upload(d_infoidx, infoidx, stream);
-    // fill / contraction tiles (problem-major), row tasks
+    // Fill / contraction tiles: problems grouped by padded dimension (one
+    // launch per group, so a d = 2 problem does not pay for d = 16), each
+    // problem's tiles contiguous.
+    std::vector<int> order(K);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return padded_dim(hp[a].d) < padded_dim(hp[b].d); });
     std::vector<FillTask> ft;
-    std::vector<int> fst(K + 1, 0);
-    std::vector<RowTask> rt;
-    for (int p = 0; p < K; ++p) {
-      fst[p] = (int)ft.size();
+    std::vector<int> fbeg(K, 0), fend(K, 0);
+    dp_groups.clear();
+    for (int p : order) {
+      const int pd = padded_dim(hp[p].d);
+      if (dp_groups.empty() || dp_groups.back().dp != pd) dp_groups.push_back(DimGroup{pd, (int)ft.size(), 0});
+      fbeg[p] = (int)ft.size();
       for (int i = 0; i < hp[p].nt; ++i)
         for (int j = 0; j <= i; ++j) ft.push_back(FillTask{p, i, j});
-      for (int r0 = 0; r0 < hp[p].npad; r0 += 256) rt.push_back(RowTask{p, r0});
+      fend[p] = (int)ft.size();
+      dp_groups.back().count = (int)ft.size() - dp_groups.back().off;
     }
-    fst[K] = (int)ft.size();
     n_fill = (int)ft.size();
-    n_rows = (int)rt.size();
+    hfill = ft;
+    hbeg = fbeg;
+    hend = fend;
     upload(fill_tasks, ft, stream);
-    upload(row_tasks, rt, stream);
-    upload(first, fst, stream);
+    upload(d_fbeg, fbeg, stream);
+    upload(d_fend, fend, stream);
   }
 
   // Slots of one launch under a schedule policy.  Items are tasks (tile
@@ -691,52 +741,74 @@ struct BatchModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
+  // out = products of k_tile_mv over the lower tiles of every problem, or
+  // of the n_act tiles in act_tasks (out is cleared whole first).
   template <typename T>
-  void trmv(const T* M, const double* v, double* outn, double* outt, int strict, int abs_mode) {
-    // outn = tril(M) v (incl diag unless strict) ; outt = tril(M)^T v
-    if (outn)
-      hipLaunchKernelGGL((k_trmv_n_tasks<T>), dim3(n_rows), dim3(256), 0, stream, row_tasks.p,
-                         probs.p, M, v, outn, strict, abs_mode);
-    if (outt)
-      hipLaunchKernelGGL((k_trmv_t_tasks<T>), dim3(n_rows), dim3(256), 0, stream, row_tasks.p,
-                         probs.p, M, v, outt, strict, abs_mode);
+  void tile_mv(const T* M, const double* v, double* out, int row, int col, int row_strict,
+               int col_strict, int abs_mode, int n_act = -1) {
+    hipLaunchKernelGGL(k_zero, dim3(std::min<long>((vtot + 255) / 256, 4096)), dim3(256), 0, stream, out,
+                       vtot);
+    const int count = n_act < 0 ? n_fill : n_act;
+    if (count > 0)
+      hipLaunchKernelGGL((k_tile_mv<T>), dim3(count), dim3(256), 0, stream,
+                         n_act < 0 ? fill_tasks.p : act_tasks.p, probs.p, M, v, out, row, col,
+                         row_strict, col_strict, abs_mode);
     GPRX_HIP_CHECK(hipGetLastError());
+  }
+
+  // Uploads the lower tiles of the problems with act[p] set; returns their
+  // count.  The pinned staging buffer is rewritten only after the stream has
+  // been synchronised (the refinement loop reads its residuals each pass).
+  int set_active(const std::vector<int>& act) {
+    int c = 0;
+    hact.alloc(std::max(n_fill, 1));
+    for (int p = 0; p < K; ++p)
+      if (act[p])
+        for (int t = hbeg[p]; t < hend[p]; ++t) hact.p[c++] = hfill[t];
+    if (c > 0)
+      GPRX_HIP_CHECK(hipMemcpyAsync(act_tasks.p, hact.p, sizeof(FillTask) * c, hipMemcpyHostToDevice, stream));
+    return c;
   }
 
   template <typename T>
   void fill(T* Aar) {
-#define GPRX_FILL(DPV, DUAL)                                                                     \
-  hipLaunchKernelGGL((k_fill_tasks<DPV, T, DUAL>), dim3(n_fill), dim3(256), 0, stream,           \
-                     fill_tasks.p, probs.p, progs.p, xar.p, Aar, K64.p, info.p)
     const bool dual = mixed();
-    switch (dp) {
-      case 1: if (dual) GPRX_FILL(1, true); else GPRX_FILL(1, false); break;
-      case 2: if (dual) GPRX_FILL(2, true); else GPRX_FILL(2, false); break;
-      case 4: if (dual) GPRX_FILL(4, true); else GPRX_FILL(4, false); break;
-      case 8: if (dual) GPRX_FILL(8, true); else GPRX_FILL(8, false); break;
-      case 16: if (dual) GPRX_FILL(16, true); else GPRX_FILL(16, false); break;
-      default: if (dual) GPRX_FILL(32, true); else GPRX_FILL(32, false); break;
-    }
+    for (const DimGroup& g : dp_groups) {
+#define GPRX_FILL(DPV, DUAL)                                                                        \
+  hipLaunchKernelGGL((k_fill_tasks<DPV, T, DUAL>), dim3(g.count), dim3(256), 0, stream,             \
+                     fill_tasks.p + g.off, probs.p, progs.p, xar.p, Aar, K64.p, info.p)
+      switch (g.dp) {
+        case 1: if (dual) GPRX_FILL(1, true); else GPRX_FILL(1, false); break;
+        case 2: if (dual) GPRX_FILL(2, true); else GPRX_FILL(2, false); break;
+        case 4: if (dual) GPRX_FILL(4, true); else GPRX_FILL(4, false); break;
+        case 8: if (dual) GPRX_FILL(8, true); else GPRX_FILL(8, false); break;
+        case 16: if (dual) GPRX_FILL(16, true); else GPRX_FILL(16, false); break;
+        default: if (dual) GPRX_FILL(32, true); else GPRX_FILL(32, false); break;
+      }
 #undef GPRX_FILL
+    }
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
   template <typename T>
   void contract(const T* Ainv) {
-#define GPRX_CON(DPV)                                                                           \
-  hipLaunchKernelGGL((k_contract_tasks<DPV, T>), dim3(n_fill), dim3(256), 0, stream,            \
-                     fill_tasks.p, probs.p, progs.p, xar.p, Ainv, alpha.p, partial.p)
-    switch (dp) {
-      case 1: GPRX_CON(1); break;
-      case 2: GPRX_CON(2); break;
-      case 4: GPRX_CON(4); break;
-      case 8: GPRX_CON(8); break;
-      case 16: GPRX_CON(16); break;
-      default: GPRX_CON(32); break;
-    }
+    for (const DimGroup& g : dp_groups) {
+#define GPRX_CON(DPV)                                                                              \
+  hipLaunchKernelGGL((k_contract_tasks<DPV, T>), dim3(g.count), dim3(256), 0, stream,               \
+                     fill_tasks.p + g.off, probs.p, progs.p, xar.p, Ainv, alpha.p,                 \
+                     partial.p + (long)g.off * kMaxAcc)
+      switch (g.dp) {
+        case 1: GPRX_CON(1); break;
+        case 2: GPRX_CON(2); break;
+        case 4: GPRX_CON(4); break;
+        case 8: GPRX_CON(8); break;
+        case 16: GPRX_CON(16); break;
+        default: GPRX_CON(32); break;
+      }
 #undef GPRX_CON
+    }
     hipLaunchKernelGGL(k_reduce_by_problem, dim3(K * kMaxAcc), dim3(256), 0, stream, partial.p,
-                       first.p, accs.p);
+                       d_fbeg.p, d_fend.p, accs.p);
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
@@ -824,8 +896,8 @@ struct BatchModel : gprx_hip_model {
   void record_fp64() {
     fill<double>(A64.p);
     factor<double>(A64.p, P64.p);
-    trmv<double>(P64.p, yar.p, tv.p, nullptr, 0, 0);
-    trmv<double>(P64.p, tv.p, nullptr, alpha.p, 0, 0);
+    tile_mv<double>(P64.p, yar.p, tv.p, 1, 0, 0, 0, 0);     // X y
+    tile_mv<double>(P64.p, tv.p, alpha.p, 0, 1, 0, 0, 0);   // X^T (X y)
     launch_tasks<double>(lauum_s, A64.p, P64.p);
     hipLaunchKernelGGL((k_problem_scalars<double>), dim3(K), dim3(256), 0, stream, probs.p, yar.p,
                        alpha.p, logd.p, A64.p, scal.p);
@@ -946,6 +1018,7 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
     B.accs.alloc(k * kMaxAcc);
     B.hacc.alloc(k * kMaxAcc);
     B.build_tasks();
+    B.act_tasks.alloc(std::max(B.n_fill, 1));
     prepare_dense_kernels();
     GPRX_HIP_CHECK(hipStreamSynchronize(s));
     B.partial.alloc((size_t)B.n_fill * kMaxAcc);
@@ -993,20 +1066,20 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
       // Mixed: f32 arenas, f64 diagonal blocks, alpha refined with f64 residuals.
       B->fill<float>(B->A32.p);
       B->factor<float>(B->A32.p, B->P32.p);
-      B->trmv<float>(B->P32.p, B->yar.p, B->tv.p, nullptr, 0, 0);
-      B->trmv<float>(B->P32.p, B->tv.p, nullptr, B->alpha.p, 0, 0);
-      // row abs sums of A (first residual pass uses them)
-      std::vector<double> ones(B->vtot, 1.0);
-      GPRX_HIP_CHECK(hipMemcpyAsync(B->dlt.p, ones.data(), sizeof(double) * B->vtot, hipMemcpyHostToDevice, s));
-      B->trmv<double>(B->K64.p, B->dlt.p, B->q1.p, nullptr, 0, 1);
-      B->trmv<double>(B->K64.p, B->dlt.p, nullptr, B->q2.p, 1, 1);
+      B->tile_mv<float>(B->P32.p, B->yar.p, B->tv.p, 1, 0, 0, 0, 0);
+      B->tile_mv<float>(B->P32.p, B->tv.p, B->alpha.p, 0, 1, 0, 0, 0);
+      // row abs sums of A (first residual pass uses them): symmetric, one pass
+      hipLaunchKernelGGL(k_fill_value, dim3(std::min<long>((B->vtot + 255) / 256, 4096)), dim3(256), 0, s,
+                         B->dlt.p, B->vtot, 1.0);
+      B->tile_mv<double>(B->K64.p, B->dlt.p, B->q1.p, 1, 1, 0, 1, 1);
       std::vector<int> act(B->K, 1), streak(B->K, 0), conv(B->K, 0);
       std::vector<double> prev(B->K, -1.0);
+      int n_act = -1;  // every problem
       for (int it = 0; it < 10; ++it) {
-        B->trmv<double>(B->K64.p, B->alpha.p, B->p1.p, nullptr, 0, 0);
-        B->trmv<double>(B->K64.p, B->alpha.p, nullptr, B->p2.p, 1, 0);
+        // p1 = A alpha over the lower triangle (row + strict transpose)
+        B->tile_mv<double>(B->K64.p, B->alpha.p, B->p1.p, 1, 1, 0, 1, 0, n_act);
         hipLaunchKernelGGL(k_problem_residual, dim3(B->K), dim3(256), 0, s, B->probs.p, B->yar.p,
-                           B->p1.p, B->p2.p, B->alpha.p, it == 0 ? B->q1.p : nullptr, B->q2.p,
+                           B->p1.p, nullptr, B->alpha.p, it == 0 ? B->q1.p : nullptr, nullptr,
                            B->r.p, B->scal.p);
         GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
         GPRX_HIP_CHECK(hipStreamSynchronize(s));
@@ -1038,8 +1111,10 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
         if (!any) break;
         for (int p = 0; p < B->K; ++p) B->hactive.p[p] = act[p];
         GPRX_HIP_CHECK(hipMemcpyAsync(B->active.p, B->hactive.p, sizeof(int) * B->K, hipMemcpyHostToDevice, s));
-        B->trmv<float>(B->P32.p, B->r.p, B->tv.p, nullptr, 0, 0);
-        B->trmv<float>(B->P32.p, B->tv.p, nullptr, B->dlt.p, 0, 0);
+        n_act = B->set_active(act);
+        // correction d = X^T (X r) on the problems still refining
+        B->tile_mv<float>(B->P32.p, B->r.p, B->tv.p, 1, 0, 0, 0, 0, n_act);
+        B->tile_mv<float>(B->P32.p, B->tv.p, B->dlt.p, 0, 1, 0, 0, 0, n_act);
         hipLaunchKernelGGL(k_masked_axpy, dim3((65536 + 255) / 256, B->K), dim3(256), 0, s,
                            B->probs.p, B->active.p, B->alpha.p, B->dlt.p);
       }

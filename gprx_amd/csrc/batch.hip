@@ -177,6 +177,7 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
                                                         const T* __restrict__ Ainv,
                                                         const double* __restrict__ alpha,
                                                         double* __restrict__ partial) {
+  using R = T;  // f32 matrices: entries evaluated in f32
   __shared__ double xr[DP][kTile];
   __shared__ double xc[DP][kTile];
   __shared__ double ar[kTile], ac[kTile];
@@ -211,10 +212,10 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
       if (gj > gi || gj >= pd.n) continue;
       double w = ar[i] * ac[j] - (double)M[gi + (long)gj * pd.npad];
       if (gi != gj) w *= 2.0;
-      double dx[DP];
+      R dx[DP];
 #pragma unroll
-      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
-      Leaves L;
+      for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
+      LeavesT<R> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }

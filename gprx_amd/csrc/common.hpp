@@ -62,6 +62,8 @@ struct DevProgram {
   double b[kMaxSlots];     // periodic: 1/l^2
   double ard_w[kMaxDim];   // 1/l_d^2 of the ARD leaf, 0 beyond d
   double noise;            // sigma^2 + jitter on the training diagonal
+  // the same constants in f32, for entries evaluated in f32
+  float af[kMaxSlots], bf[kMaxSlots], ard_wf[kMaxDim];
 };
 
 // Host-side description of a program: leaves in depth-first order, terms,

@@ -162,6 +162,11 @@ void make_dev_program(const HostProgram& hp, const double* th, double diag_noise
     }
   }
   P->noise = diag_noise;
+  for (int k = 0; k < kMaxSlots; ++k) {
+    P->af[k] = (float)P->a[k];
+    P->bf[k] = (float)P->b[k];
+  }
+  for (int k = 0; k < kMaxDim; ++k) P->ard_wf[k] = (float)P->ard_w[k];
 }
 
 void acc_to_grad(const HostProgram& hp, const double* th, const double* acc, double* grad) {
@@ -251,6 +256,7 @@ __global__ __launch_bounds__(256) void k_fill_cross(const DevProgram* __restrict
                                                     const double* __restrict__ a, long lda, int na,
                                                     const double* __restrict__ b, long ldb, int nb,
                                                     T* __restrict__ out, long ldo, int mt) {
+  using R = T;  // f32 matrices: entries evaluated in f32
   const DevProgram& P = *Pg;
   __shared__ double xr[DP][kTile];
   __shared__ double xc[DP][kTile];
@@ -269,10 +275,10 @@ __global__ __launch_bounds__(256) void k_fill_cross(const DevProgram* __restrict
     const int gj = c0 + j;
     double val = 0.0;
     if (gi < na && gj < nb) {
-      double dx[DP];
+      R dx[DP];
 #pragma unroll
-      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
-      Leaves L;
+      for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
+      LeavesT<R> L;
       val = eval_entry<DP, false>(P, dx, L);
     }
     out[gi + (long)gj * ldo] = (T)val;
@@ -297,6 +303,7 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
                                                         int n, const T* __restrict__ Ainv, long lda,
                                                         const double* __restrict__ alpha,
                                                         double* __restrict__ partial) {
+  using R = T;  // f32 matrices: entries evaluated in f32
   const DevProgram& P = *Pg;
   __shared__ double xr[DP][kTile];
   __shared__ double xc[DP][kTile];
@@ -331,10 +338,10 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
       if (gj > gi || gj >= n) continue;
       double w = ai * ac[j] - (double)Ainv[gi + (long)gj * lda];
       if (gi != gj) w *= 2.0;
-      double dx[DP];
+      R dx[DP];
 #pragma unroll
-      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
-      Leaves L;
+      for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
+      LeavesT<R> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }
@@ -351,6 +358,7 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
                                                         double* __restrict__ partial,
                                                         const double* __restrict__ cv,
                                                         const double* __restrict__ rv, double crs) {
+  using R = T;  // f32 matrices: entries evaluated in f32
   const DevProgram& P = *Pg;
   __shared__ double xr[DP][kTile];
   __shared__ double xc[DP][kTile];
@@ -379,10 +387,10 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
       if (gj >= nb) continue;
       double w = (double)W[gi + (long)gj * ldw];
       if (cv) w -= ci * rv[gj];
-      double dx[DP];
+      R dx[DP];
 #pragma unroll
-      for (int q = 0; q < DP; ++q) dx[q] = xi[q] - xc[q][j];
-      Leaves L;
+      for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
+      LeavesT<R> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }

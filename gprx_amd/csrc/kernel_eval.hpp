@@ -10,43 +10,73 @@ namespace gprx {
 // ---------------------------------------------------------------------------
 // Device: one entry of the program.
 // ---------------------------------------------------------------------------
-struct Leaves {
-  double f0[kMaxSlots];  // d v_s / d log theta = v_s * (feature) per slot
-  double f1[kMaxSlots];
-  double tp[kMaxSlots];  // product of the slot's term up to and including it
+// Per-entry arithmetic in R (double, or float for the mixed form's f32
+// matrices): leaf features and term products; accumulation stays f64.
+template <typename R>
+struct LeavesT {
+  R f0[kMaxSlots];  // d v_s / d log theta = v_s * (feature) per slot
+  R f1[kMaxSlots];
+  R tp[kMaxSlots];  // product of the slot's term up to and including it
 };
+using Leaves = LeavesT<double>;
+
+__device__ __forceinline__ double k_exp(double x) { return exp(x); }
+__device__ __forceinline__ float k_exp(float x) { return expf(x); }
+__device__ __forceinline__ double k_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float k_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ void k_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void k_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+
+template <typename R>
+__device__ __forceinline__ R prog_a(const DevProgram& P, int s);
+template <>
+__device__ __forceinline__ double prog_a<double>(const DevProgram& P, int s) { return P.a[s]; }
+template <>
+__device__ __forceinline__ float prog_a<float>(const DevProgram& P, int s) { return P.af[s]; }
+template <typename R>
+__device__ __forceinline__ R prog_b(const DevProgram& P, int s);
+template <>
+__device__ __forceinline__ double prog_b<double>(const DevProgram& P, int s) { return P.b[s]; }
+template <>
+__device__ __forceinline__ float prog_b<float>(const DevProgram& P, int s) { return P.bf[s]; }
+template <typename R>
+__device__ __forceinline__ R prog_w(const DevProgram& P, int q);
+template <>
+__device__ __forceinline__ double prog_w<double>(const DevProgram& P, int q) { return P.ard_w[q]; }
+template <>
+__device__ __forceinline__ float prog_w<float>(const DevProgram& P, int q) { return P.ard_wf[q]; }
 
 // K at one entry (dx = a - b over the padded dimensions): leaf values slot by
 // slot, the running product of the open term, and the sum over closed terms.
-template <int DP, bool GRAD>
-__device__ __forceinline__ double eval_entry(const DevProgram& P, const double (&dx)[DP], Leaves& L) {
-  double s = 0.0, r2 = 0.0;
+template <int DP, bool GRAD, typename R>
+__device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], LeavesT<R>& L) {
+  R s = 0, r2 = 0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) {
-    const double t = dx[q] * dx[q];
+    const R t = dx[q] * dx[q];
     s += t;
-    r2 += t * P.ard_w[q];
+    r2 += t * prog_w<R>(P, q);
   }
-  double k = 0.0, rp = 1.0;
+  R k = 0, rp = 1;
 #pragma unroll
   for (int sl = 0; sl < kMaxSlots; ++sl) {
     if (sl >= P.ns) break;
     const int kd = P.kind[sl];
-    double v, f0 = 0.0, f1 = 0.0;
+    R v, f0 = 0, f1 = 0;
     if (kd == kLeafConst) {
-      v = P.a[sl];
-      f0 = 1.0;
+      v = prog_a<R>(P, sl);
+      f0 = 1;
     } else if (kd == kLeafRbf) {
-      v = exp(-0.5 * s * P.a[sl]);
+      v = k_exp(R(-0.5) * s * prog_a<R>(P, sl));
       f0 = s;
     } else if (kd == kLeafArd) {
-      v = exp(-0.5 * r2);
+      v = k_exp(R(-0.5) * r2);
     } else {
-      const double r = sqrt(s);
-      const double al = r * P.a[sl];
-      double sn, cs;
-      sincos(al, &sn, &cs);
-      v = exp(-2.0 * sn * sn * P.b[sl]);
+      const R r = k_sqrt(s);
+      const R al = r * prog_a<R>(P, sl);
+      R sn, cs;
+      k_sincos(al, &sn, &cs);
+      v = k_exp(R(-2) * sn * sn * prog_b<R>(P, sl));
       f0 = sn * sn;
       f1 = sn * cs * al;
     }
@@ -61,28 +91,28 @@ __device__ __forceinline__ double eval_entry(const DevProgram& P, const double (
   return k;
 }
 
-// Adds w * dK/d(slot features) of one entry to the accumulators: sweeping
-// the slots backwards, each term's full product (held by its last slot) is
-// carried to the slots before it.
-template <int DP>
-__device__ __forceinline__ void accumulate_entry(const DevProgram& P, const double (&dx)[DP],
-                                                 const Leaves& L, double w,
+// Adds w * dK/d(slot features) of one entry to the (f64) accumulators:
+// sweeping the slots backwards, each term's full product (held by its last
+// slot) is carried to the slots before it.
+template <int DP, typename R>
+__device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&dx)[DP],
+                                                 const LeavesT<R>& L, double w,
                                                  double (&acc)[kMaxSlots][kAccPerLeaf],
                                                  double (&accd)[DP]) {
   double gard = 0.0, term = 0.0;
 #pragma unroll
   for (int sl = kMaxSlots - 1; sl >= 0; --sl) {
     if (sl < P.ns) {
-      if ((P.end >> sl) & 1u) term = L.tp[sl];
+      if ((P.end >> sl) & 1u) term = (double)L.tp[sl];
       const double g = w * term;
-      acc[sl][0] += g * L.f0[sl];
-      acc[sl][1] += g * L.f1[sl];
+      acc[sl][0] += g * (double)L.f0[sl];
+      acc[sl][1] += g * (double)L.f1[sl];
       if ((P.ard >> sl) & 1u) gard += g;
     }
   }
   if (P.ard) {
 #pragma unroll
-    for (int q = 0; q < DP; ++q) accd[q] += gard * dx[q] * dx[q];
+    for (int q = 0; q < DP; ++q) accd[q] += gard * (double)(dx[q] * dx[q]);
   }
 }
 

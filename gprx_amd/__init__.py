@@ -8,8 +8,8 @@ the library or a GPU is missing.
 from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InvalidHyperparameter,
                      InvalidNoiseVariance, LengthMismatch, NonFiniteInput, NonFiniteKernelValue,
                      Unsupported)
-from .gpr import (FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision, Prediction,
-                  VarianceKind)
+from .gpr import (Fallback, FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision,
+                  Prediction, VarianceKind)
 from .batch import GprBatch
 from .sgpr import FittedSgpr, Sgpr
 from .svgp import FittedSvgp, Svgp, prior_params
@@ -19,7 +19,7 @@ from .kernels import (ConstantKernel, KernelSpec, PeriodicKernel, Product, RbfAr
 __all__ = [
     "CholeskyFailed", "DeviceError", "EmptyInput", "GprError", "InvalidHyperparameter",
     "InvalidNoiseVariance", "LengthMismatch", "NonFiniteInput", "NonFiniteKernelValue",
-    "Unsupported", "FittedGpr", "GaussianLikelihood", "Gpr", "JitterPolicy", "Precision",
+    "Unsupported", "Fallback", "FittedGpr", "GaussianLikelihood", "Gpr", "JitterPolicy", "Precision",
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
     "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
 ]

@@ -32,6 +32,10 @@ MIXED = 1
 VARIANCE_OBSERVATION = 0
 VARIANCE_LATENT = 1
 
+FALLBACK_NONE = 0
+FALLBACK_REFINEMENT = 1
+FALLBACK_JITTER = 2
+
 # Every symbol the header declares, with (restype, argtypes).
 _dp = C.POINTER(C.c_double)
 _vp = C.c_void_p
@@ -88,6 +92,7 @@ SIGNATURES = {
     "gprx_hip_last_jitter": (C.c_double, [_vp]),
     "gprx_hip_last_fallback": (C.c_int, [_vp]),
     "gprx_hip_last_refine_steps": (C.c_int, [_vp]),
+    "gprx_hip_set_mixed_jitter_bound": (C.c_int, [_vp, C.c_double, _err]),
     "gprx_hip_sgpr_create": (C.c_int, [_dp, C.c_size_t, C.c_size_t, _dp, _dp, C.c_size_t,
                                        C.POINTER(KernelProgram), _dp, C.c_size_t, C.c_int32, _mp,
                                        _err]),
@@ -109,6 +114,7 @@ SIGNATURES = {
     "gprx_hip_batch_set_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32, _err]),
     "gprx_hip_batch_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                           C.POINTER(C.c_double)]),
+    "gprx_hip_batch_last_fallback": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_size_t]),
     "gprx_hip_destroy": (None, [_vp]),
     "gprx_hip_last_eval_ms": (C.c_double, [_vp]),
     "gprx_hip_last_dense_ms": (C.c_double, [_vp]),

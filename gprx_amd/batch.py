@@ -81,5 +81,14 @@ class GprBatch:
         L.lib().gprx_hip_batch_schedule(self._h, C.byref(pol), C.byref(below), C.byref(ms))
         return pol.value, below.value, ms.value
 
+    def last_fallback(self) -> np.ndarray:
+        """Per problem, why the last mixed call evaluated it alone in FP64
+        (FALLBACK_NONE 0, FALLBACK_REFINEMENT 1, FALLBACK_JITTER 2)."""
+        out = np.zeros(self.k, dtype=np.int32)
+        rc = L.lib().gprx_hip_batch_last_fallback(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                  self.k)
+        raise_for(rc, L.GprxError())
+        return out
+
     def last_eval_ms(self) -> float:
         return L.lib().gprx_hip_last_eval_ms(self._h)

@@ -427,6 +427,9 @@ struct HostProb {
   int n, d, npad, nt;
   long xoff, voff, moff;
   std::vector<double> theta;
+  // Mixed batches: the problem as given, for an FP64 re-evaluation.
+  std::vector<double> x, y;
+  std::vector<int32_t> ops;
 };
 
 template <typename V>
@@ -439,6 +442,7 @@ void upload(DevBuf<V>& dst, const std::vector<V>& src, hipStream_t s) {
 struct BatchModel : gprx_hip_model {
   BatchModel() : gprx_hip_model(kBatch) {}
   ~BatchModel() override {
+    for (gprx_hip_model* m : fb) gprx_hip_destroy(m);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) {
@@ -451,6 +455,13 @@ struct BatchModel : gprx_hip_model {
   int K = 0, dp = 1, maxnt = 0, precision = GPRX_FP64;
   long xtot = 0, vtot = 0, mtot = 0;
   std::vector<HostProb> hp;
+  // Mixed batches: a problem whose f32 factor fails or whose refinement
+  // stalls is evaluated alone in FP64 (fb[p], created on first need and
+  // kept while their matrices stay under kFallbackKeepBytes).
+  static constexpr double kFallbackKeepBytes = 16.0 * (1 << 30);
+  std::vector<gprx_hip_model*> fb;
+  std::vector<int> fb_reason;
+  double fb_bytes = 0.0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
@@ -760,6 +771,28 @@ struct BatchModel : gprx_hip_model {
   // Uploads the lower tiles of the problems with act[p] set; returns their
   // count.  The pinned staging buffer is rewritten only after the stream has
   // been synchronised (the refinement loop reads its residuals each pass).
+  // Problem p alone in the FP64 form (default jitter policy fixed(0), as the
+  // batch's own factor): the mixed batch's fallback.
+  int eval_fp64_alone(int p, const double* th, double* nlml, double* grad, gprx_error* e) {
+    HostProb& h = hp[p];
+    const size_t np = h.theta.size();
+    if (!fb[p]) {
+      const gprx_kernel_program prog{h.ops.data(), h.ops.size() / 2};
+      const int rc = gprx_hip_exact_create(h.x.data(), (size_t)h.n, (size_t)h.d, h.y.data(), &prog, th,
+                                           np, GPRX_FP64, nullptr, &fb[p], e);
+      if (rc != GPRX_OK) return rc;
+    }
+    const int rc = gprx_hip_exact_value_and_grad(fb[p], th, np, nlml, grad, e);
+    const double bytes = 3.0 * sizeof(double) * (double)h.npad * (double)h.npad;
+    if (fb_bytes + bytes <= kFallbackKeepBytes) {
+      fb_bytes += bytes;
+    } else {
+      gprx_hip_destroy(fb[p]);
+      fb[p] = nullptr;
+    }
+    return rc;
+  }
+
   int set_active(const std::vector<int>& act) {
     int c = 0;
     hact.alloc(std::max(n_fill, 1));
@@ -927,6 +960,8 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
     B.K = (int)k;
     B.precision = precision;
     B.hp.resize(k);
+    B.fb.assign(k, nullptr);
+    B.fb_reason.assign(k, GPRX_FALLBACK_NONE);
     int maxd = 1;
     for (size_t p = 0; p < k; ++p) {
       const gprx_problem& pr = problems[p];
@@ -960,6 +995,11 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
       h.npad = (int)round_up(pr.n, kTile);
       h.nt = h.npad / kTile;
       h.theta.assign(pr.log_theta, pr.log_theta + pr.p);
+      if (precision == GPRX_MIXED) {
+        h.x.assign(pr.x, pr.x + pr.n * pr.d);
+        h.y.assign(pr.y, pr.y + pr.n);
+        h.ops.assign(pr.kernel->ops, pr.kernel->ops + 2 * pr.kernel->n_ops);
+      }
       maxd = std::max(maxd, h.d);
     }
     B.dp = padded_dim(maxd);
@@ -1039,6 +1079,7 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
     auto* B = static_cast<BatchModel*>(model);
     hipStream_t s = B->stream;
     std::vector<int> ok(B->K, 1);
+    std::fill(B->fb_reason.begin(), B->fb_reason.end(), GPRX_FALLBACK_NONE);
     for (int p = 0; p < B->K; ++p) {
       HostProb& h = B->hp[p];
       gprx_error* e = errs ? errs + p : nullptr;
@@ -1083,9 +1124,12 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
                            B->p1.p, nullptr, B->alpha.p, it == 0 ? B->q1.p : nullptr, nullptr,
                            B->r.p, B->scal.p);
         GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
+        if (it == 0)
+          GPRX_HIP_CHECK(hipMemcpyAsync(B->hinfo.p, B->info.p, sizeof(int) * B->K, hipMemcpyDeviceToHost, s));
         GPRX_HIP_CHECK(hipStreamSynchronize(s));
         bool any = false;
         for (int p = 0; p < B->K; ++p) {
+          if (it == 0 && B->hinfo.p[p]) act[p] = 0;  // f32 factor failed: FP64 below
           if (!act[p]) continue;
           const double* sc = B->hscal.p + 8 * p;
           const double tol = 10.0 * B->hp[p].n * 2.220446049250313e-16;
@@ -1120,11 +1164,8 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
                            B->probs.p, B->active.p, B->alpha.p, B->dlt.p);
       }
       for (int p = 0; p < B->K; ++p)
-        if (!conv[p] && ok[p]) {
-          ok[p] = 0;
-          if (errs)
-            set_error(errs + p, GPRX_CHOLESKY_FAILED, "mixed refinement did not converge (use FP64)");
-        }
+        if (!conv[p] && ok[p])
+          B->fb_reason[p] = B->hinfo.p[p] ? GPRX_FALLBACK_JITTER : GPRX_FALLBACK_REFINEMENT;
       B->launch_tasks<float>(B->lauum_s, B->A32.p, B->P32.p);
       hipLaunchKernelGGL((k_problem_scalars<float>), dim3(B->K), dim3(256), 0, s, B->probs.p,
                          B->yar.p, B->alpha.p, B->logd.p, B->A32.p, B->scal.p);
@@ -1144,6 +1185,13 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
       const int P = h.hp.n_params;
       if (!ok[p]) {
         nlml[p] = NAN;
+        continue;
+      }
+      if (B->fb_reason[p] != GPRX_FALLBACK_NONE) {
+        if (B->eval_fp64_alone(p, log_theta[p], nlml + p, grad ? grad[p] : nullptr, e) == GPRX_OK)
+          h.theta.assign(log_theta[p], log_theta[p] + P + 1);
+        else
+          nlml[p] = NAN;
         continue;
       }
       if (B->hinfo.p[p]) {
@@ -1201,6 +1249,14 @@ int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy, int32_
   if (policy) *policy = B->policy;
   if (small_tile_below) *small_tile_below = B->small_below;
   if (tuned_ms) *tuned_ms = B->tuned_ms;
+  return GPRX_OK;
+}
+
+int gprx_hip_batch_last_fallback(const gprx_hip_model* model, int32_t* reasons, size_t k) {
+  if (!model || model->kind != gprx_hip_model::kBatch) return GPRX_UNSUPPORTED;
+  const auto* B = static_cast<const BatchModel*>(model);
+  if (!reasons || k != (size_t)B->K) return GPRX_LENGTH_MISMATCH;
+  for (int p = 0; p < B->K; ++p) reasons[p] = B->fb_reason[p];
   return GPRX_OK;
 }
 

@@ -38,9 +38,12 @@ struct ExactModel : gprx_hip_model {
   long npad = 0, ldt = 0;
   int precision = GPRX_FP64;
   gprx_jitter_policy jit{0.0, 1.0, 0.0, 0};
+  // Largest jitter the mixed form's f32 factor may take; a factor that needs
+  // more is formed in FP64 instead (0: the f32 factor adds none of its own).
+  double mixed_jitter_bound = 0.0;
   std::vector<double> theta;
   double last_jitter = 0.0;
-  int fallback = 0, refine_steps = 0;
+  int fallback = 0, refine_steps = 0;  // fallback: GPRX_FALLBACK_* of the last evaluation
   bool factored = false;
   int cur_form = GPRX_FP64;  // which buffers hold X and alpha
   hipStream_t stream = nullptr;
@@ -187,16 +190,16 @@ struct ExactModel : gprx_hip_model {
     return readback(3, want_grad);
   }
 
-  // One attempt at jitter j; fills nlml/grad on success.  Returns info bits.
-  int attempt(const double* th, double j, bool want_grad, double* nlml, double* grad) {
+  // One attempt at jitter j in the mixed form (FP64 when refinement stalls)
+  // or in FP64; fills nlml/grad on success.  Returns info bits.
+  int attempt(const double* th, double j, bool fp64, bool want_grad, double* nlml, double* grad) {
     upload_program(th, j);
     GPRX_HIP_CHECK(hipEventRecord(ev[0], stream));
     int bits;
-    fallback = 0;
-    if (precision == GPRX_MIXED) {
+    if (!fp64) {
       bits = run_mixed(want_grad);
       if (bits == -1) {
-        fallback = 1;
+        fallback = GPRX_FALLBACK_REFINEMENT;
         bits = run_fp64(want_grad);
       }
     } else {
@@ -225,12 +228,17 @@ struct ExactModel : gprx_hip_model {
     return 0;
   }
 
-  // The jitter loop of src/linalg/chol.rs:20-43 around one evaluation.
+  // The jitter loop of src/linalg/chol.rs:20-43 around one evaluation.  In
+  // the mixed form, a failed f32 factor whose next jitter would pass
+  // mixed_jitter_bound (or the policy's end) is formed again in FP64 at the
+  // same jitter, and the loop goes on in FP64.
   int evaluate(const double* th, bool want_grad, double* nlml, double* grad, gprx_error* err) {
     double j = 0.0, next = jit.initial;
     int remaining = jit.max_retries;
+    bool fp64 = precision != GPRX_MIXED;
+    fallback = GPRX_FALLBACK_NONE;
     for (;;) {
-      const int bits = attempt(th, j, want_grad, nlml, grad);
+      const int bits = attempt(th, j, fp64, want_grad, nlml, grad);
       if (bits == 0) {
         last_jitter = j;
         factored = true;
@@ -241,7 +249,13 @@ struct ExactModel : gprx_hip_model {
         set_error(err, GPRX_NON_FINITE_KERNEL_VALUE, "kernel matrix holds a non-finite value");
         return GPRX_NON_FINITE_KERNEL_VALUE;
       }
-      if (remaining == 0 || !(next > 0.0) || next > jit.max_jitter) {
+      const bool exhausted = remaining == 0 || !(next > 0.0) || next > jit.max_jitter;
+      if (!fp64 && (exhausted || next > mixed_jitter_bound)) {
+        fp64 = true;
+        fallback = GPRX_FALLBACK_JITTER;
+        continue;
+      }
+      if (exhausted) {
         set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky factor failed", GPRX_STAGE_FIT, j, (size_t)n);
         return GPRX_CHOLESKY_FAILED;
       }
@@ -541,6 +555,21 @@ double gprx_hip_last_jitter(const gprx_hip_model* model) {
 int gprx_hip_last_fallback(const gprx_hip_model* model) {
   if (!model || model->kind != gprx_hip_model::kExact) return 0;
   return static_cast<const ExactModel*>(model)->fallback;
+}
+
+int gprx_hip_set_mixed_jitter_bound(gprx_hip_model* model, double bound, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact) {
+      set_error(err, GPRX_UNSUPPORTED, "not an exact model");
+      return GPRX_UNSUPPORTED;
+    }
+    if (!(bound >= 0.0)) {
+      set_error(err, GPRX_INVALID_HYPERPARAMETER, "mixed jitter bound must be >= 0");
+      return GPRX_INVALID_HYPERPARAMETER;
+    }
+    static_cast<ExactModel*>(model)->mixed_jitter_bound = bound;
+    return GPRX_OK;
+  });
 }
 
 int gprx_hip_last_refine_steps(const gprx_hip_model* model) {

@@ -59,6 +59,12 @@ class Precision:
     MIXED = L.MIXED
 
 
+class Fallback:
+    NONE = L.FALLBACK_NONE
+    REFINEMENT = L.FALLBACK_REFINEMENT
+    JITTER = L.FALLBACK_JITTER
+
+
 class VarianceKind:
     OBSERVATION = L.VARIANCE_OBSERVATION
     LATENT = L.VARIANCE_LATENT
@@ -82,6 +88,7 @@ class Gpr:
         self.likelihood = likelihood
         self.jitter = JitterPolicy()
         self.precision = Precision.FP64
+        self.mixed_jitter_bound = 0.0
 
     def with_jitter_policy(self, policy: JitterPolicy) -> "Gpr":
         self.jitter = policy
@@ -89,6 +96,12 @@ class Gpr:
 
     def with_precision(self, precision: int) -> "Gpr":
         self.precision = precision
+        return self
+
+    def with_mixed_jitter_bound(self, bound: float) -> "Gpr":
+        """Largest jitter the mixed form's f32 factor may add before the
+        evaluation is formed in FP64 instead (default 0)."""
+        self.mixed_jitter_bound = bound
         return self
 
     def factor(self, x, n: int, d: int, y) -> "FittedGpr":
@@ -112,7 +125,12 @@ class Gpr:
                                            L.dptr(theta), theta.size, self.precision,
                                            C.byref(jit), C.byref(handle), C.byref(err))
         raise_for(rc, err)
-        return FittedGpr(handle, theta, n, d)
+        fitted = FittedGpr(handle, theta, n, d)
+        if self.mixed_jitter_bound != 0.0:
+            rc = L.lib().gprx_hip_set_mixed_jitter_bound(handle, self.mixed_jitter_bound,
+                                                         C.byref(err))
+            raise_for(rc, err)
+        return fitted
 
 
 class FittedGpr:
@@ -185,8 +203,10 @@ class FittedGpr:
     def last_jitter(self) -> float:
         return L.lib().gprx_hip_last_jitter(self._h)
 
-    def last_fallback(self) -> bool:
-        return bool(L.lib().gprx_hip_last_fallback(self._h))
+    def last_fallback(self) -> int:
+        """Why the last mixed evaluation ran in FP64: FALLBACK_NONE (0),
+        FALLBACK_REFINEMENT (1) or FALLBACK_JITTER (2)."""
+        return L.lib().gprx_hip_last_fallback(self._h)
 
     def last_refine_steps(self) -> int:
         return L.lib().gprx_hip_last_refine_steps(self._h)

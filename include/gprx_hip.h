@@ -81,8 +81,16 @@ typedef struct gprx_kernel_program {
  * fills and all dense trailing products in f32 on MFMA with f32 accumulation;
  * f64 for the diagonal-block factorizations, the log-determinant, small
  * m x m factors, and the iterative refinement of alpha with f64 residuals;
- * falls back to FP64 when refinement stalls. */
+ * falls back to FP64 when refinement stalls or when the f32 factor needs
+ * more jitter than a set bound (gprx_hip_set_mixed_jitter_bound). */
 typedef enum gprx_precision { GPRX_FP64 = 0, GPRX_MIXED = 1 } gprx_precision;
+
+/* Why the last mixed evaluation ran in FP64 (gprx_hip_last_fallback). */
+typedef enum gprx_fallback {
+  GPRX_FALLBACK_NONE = 0,
+  GPRX_FALLBACK_REFINEMENT = 1, /* refinement stalled (src/precision/refine.rs:66-98) */
+  GPRX_FALLBACK_JITTER = 2      /* the f32 factor needed jitter above the bound */
+} gprx_fallback;
 
 /* JitterPolicy (src/policy.rs:180-396).  The factor is tried without an
  * offset first; on failure with offsets initial, initial*multiplier, ...,
@@ -137,9 +145,16 @@ int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, si
  * took. */
 int gprx_hip_get_params(const gprx_hip_model* model, double* log_theta, size_t p);
 double gprx_hip_last_jitter(const gprx_hip_model* model);
-/* For the mixed form: 1 if the last evaluation fell back to FP64, and the
- * number of refinement corrections it took. */
+/* For the mixed form: the gprx_fallback reason if the last evaluation fell
+ * back to FP64 (GPRX_FALLBACK_NONE if not), and the number of refinement
+ * corrections it took. */
 int gprx_hip_last_fallback(const gprx_hip_model* model);
+/* Largest jitter the mixed form's f32 factor may add (default 0: it adds
+ * none).  When the jitter policy's next offset after a failed f32 factor is
+ * above the bound, or the policy has none left, the evaluation forms the
+ * FP64 factor at the same jitter and continues the policy in FP64.  No
+ * counterpart in the reference (its f32 factor takes the policy's jitter). */
+int gprx_hip_set_mixed_jitter_bound(gprx_hip_model* model, double bound, gprx_error* err);
 int gprx_hip_last_refine_steps(const gprx_hip_model* model);
 
 /* ------------------------------------------------------------------ sgpr */
@@ -235,6 +250,13 @@ int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t s
                                 gprx_error* err);
 int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy,
                             int32_t* small_tile_below, double* tuned_ms);
+
+/* A mixed batch evaluates a problem whose f32 factor fails or whose
+ * refinement stalls again alone in FP64 (as gprx_hip_exact_value_and_grad
+ * with the default jitter policy), so its results and status are FP64's.
+ * reasons[k] receives each problem's gprx_fallback for the last call.  No
+ * counterpart in the reference (it evaluates problems one by one). */
+int gprx_hip_batch_last_fallback(const gprx_hip_model* model, int32_t* reasons, size_t k);
 
 /* ------------------------------------------------------------------ all */
 

@@ -105,3 +105,18 @@ def gpu():
     if not has_gpu():
         pytest.skip("no GPU / library")
     return True
+
+
+def duplicate_start_problem(n=300, d=3, seed=21):
+    """Points 0 and 1 coincide, RBF lengthscale 0.05 on uniform points,
+    noise 1e-10.  Rounded to f32, K + sigma^2 I holds 1.0 at (0,0), (0,1)
+    and (1,1), so its second pivot is exactly 1 - 1*1 = 0 and an f32 factor
+    fails without jitter; in f64 that pivot is 2e-10 and the factor holds.
+    Returns (spec, noise, x column-major, n, d, y)."""
+    import gprx_amd as g
+
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, d))
+    pts[1] = pts[0]
+    y = np.sin(6 * pts[:, 0]) + 0.1 * rng.standard_normal(n)
+    return g.RbfKernel(0.05), 1e-10, pts.T.copy().ravel(), n, d, y

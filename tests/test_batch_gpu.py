@@ -9,7 +9,7 @@ import pytest
 
 import gprx_amd as g
 import oracle
-from conftest import assert_close, rel_err
+from conftest import assert_close, duplicate_start_problem, rel_err
 from gprx_amd.configs import batched_problems
 
 pytestmark = pytest.mark.gpu
@@ -80,3 +80,30 @@ def test_autotuner_picks_a_candidate(gpu):
     b.value_and_gradient(thetas)
     pol, below, ms = b.schedule()
     assert pol in (0, 1, 2) and below in (0, 256, 1024)
+
+
+def test_mixed_batch_falls_back_per_problem(gpu):
+    """A problem whose f32 factor fails (an exactly-zero f32 pivot) is
+    evaluated alone in FP64: its results equal the single FP64 model's bit
+    for bit, and the other problems keep the mixed form."""
+    probs = _problems(k=3, seed=15)
+    spec, noise, x, n, d, y = duplicate_start_problem()
+    probs.insert(1, (spec, g.GaussianLikelihood(noise), x, n, d, y))
+    print(f"[batch mixed fallback] sizes={[p[3] for p in probs]}", flush=True)
+    b = g.GprBatch(probs, precision=g.Precision.MIXED)
+    thetas = [np.concatenate([p[0].get_params(), [p[1].log_param()]]) for p in probs]
+    nlml, grads, errs = b.value_and_gradient(thetas)
+    assert list(b.last_fallback()) == [0, g.Fallback.JITTER, 0, 0]
+    assert all(e is None for e in errs), errs
+    m = g.Gpr(spec, g.GaussianLikelihood(noise)).prepare(x, n, d, y)
+    gr = np.empty(thetas[1].size)
+    v = m.value_and_gradient_into(thetas[1], gr)
+    assert nlml[1] == v and np.array_equal(grads[1], gr)
+    for i in (0, 2, 3):
+        sp, lik, xi, ni, di, yi = probs[i]
+        ov, og, _, _ = oracle.exact_value_grad(sp.program(), thetas[i], xi, ni, di, yi)
+        assert_close(nlml[i], ov, 1e-4, f"nlml {i}")
+        assert_close(grads[i], og, 1e-2, f"grad {i}")
+    # a second call reuses the kept FP64 model
+    nlml2, grads2, _ = b.value_and_gradient(thetas)
+    assert nlml2[1] == v and np.array_equal(grads2[1], gr)

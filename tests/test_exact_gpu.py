@@ -13,7 +13,8 @@ import pytest
 
 import gprx_amd as g
 import oracle
-from conftest import EXACT_COMPOSITE, EXACT_RBF, assert_close, golden_problem, rel_err
+from conftest import (EXACT_COMPOSITE, EXACT_RBF, assert_close, duplicate_start_problem,
+                      golden_problem, rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -101,7 +102,70 @@ def test_random_vs_oracle_mixed(gpu, n, d, kname):
           f"refine {m.last_refine_steps()} fallback {m.last_fallback()}")
     assert_close(val, ov, 1e-4, "nlml")
     assert_close(grad, og, 1e-2, "grad")
-    assert not m.last_fallback()
+    assert m.last_fallback() == g.Fallback.NONE
+
+
+def _eval(m, th):
+    grad = np.empty(th.size)
+    return m.value_and_gradient_into(th, grad), grad
+
+
+def test_mixed_falls_back_when_f32_factor_needs_jitter(gpu):
+    """Default bound 0: the f32 factor fails at its exactly-zero pivot, so
+    the evaluation is the FP64 form's at jitter 0, bit for bit."""
+    spec, noise, x, n, d, y = duplicate_start_problem()
+    print(f"[exact mixed jitter fallback] n={n} d={d}", flush=True)
+    th = np.concatenate([spec.get_params(), [math.log(noise)]])
+    m32 = _fit(spec, noise, x, n, d, y, precision=g.Precision.MIXED)
+    m64 = _fit(spec, noise, x, n, d, y)
+    v32, g32 = _eval(m32, th)
+    v64, g64 = _eval(m64, th)
+    assert m32.last_fallback() == g.Fallback.JITTER
+    assert m32.last_jitter() == 0.0 and m64.last_jitter() == 0.0
+    assert v32 == v64 and np.array_equal(g32, g64)
+    xs = np.random.default_rng(4).random(17 * d)
+    p32, p64 = m32.predict(xs, 17, d), m64.predict(xs, 17, d)
+    assert np.array_equal(p32.mean, p64.mean) and np.array_equal(p32.variance, p64.variance)
+
+
+def test_mixed_jitter_within_bound_stays_f32(gpu):
+    """A bound at or above the policy's first offset lets the f32 factor
+    take it: no fallback, and the result is the jittered system's (the
+    oracle at noise sigma^2 + j)."""
+    spec, noise, x, n, d, y = duplicate_start_problem()
+    print(f"[exact mixed jitter within bound] n={n} d={d}", flush=True)
+    th = np.concatenate([spec.get_params(), [math.log(noise)]])
+    pol = g.JitterPolicy.adaptive(1e-3, 10.0, 3, 1e-2)
+    m = (g.Gpr(spec, g.GaussianLikelihood(noise)).with_precision(g.Precision.MIXED)
+         .with_jitter_policy(pol).with_mixed_jitter_bound(1e-2).prepare(x, n, d, y))
+    v, gr = _eval(m, th)
+    assert m.last_fallback() == g.Fallback.NONE
+    assert m.last_jitter() == 1e-3
+    thj = th.copy()
+    thj[-1] = math.log(noise + 1e-3)
+    ov, og, _, _ = oracle.exact_value_grad(spec.program(), thj, x, n, d, y)
+    print(f"  nlml err {rel_err(v, ov):.2e} grad err {rel_err(gr[:-1], og[:-1]):.2e} "
+          f"refine {m.last_refine_steps()}")
+    assert_close(v, ov, 1e-4, "nlml")
+    assert_close(gr[:-1], og[:-1], 1e-2, "grad")
+    # d/dlog sigma^2 scales the trace by sigma^2 alone: the jitter is fixed
+    assert_close(gr[-1], og[-1] * noise / (noise + 1e-3), 1e-2, "grad noise")
+
+
+def test_mixed_jitter_above_bound_falls_back(gpu):
+    """The policy's first offset (1e-3) is above the bound (1e-4): the f32
+    failure goes to FP64 at jitter 0, where the factor holds."""
+    spec, noise, x, n, d, y = duplicate_start_problem()
+    print(f"[exact mixed jitter above bound] n={n} d={d}", flush=True)
+    th = np.concatenate([spec.get_params(), [math.log(noise)]])
+    pol = g.JitterPolicy.adaptive(1e-3, 10.0, 3, 1e-2)
+    m32 = (g.Gpr(spec, g.GaussianLikelihood(noise)).with_precision(g.Precision.MIXED)
+           .with_jitter_policy(pol).with_mixed_jitter_bound(1e-4).prepare(x, n, d, y))
+    m64 = _fit(spec, noise, x, n, d, y, jitter=pol)
+    v32, g32 = _eval(m32, th)
+    v64, g64 = _eval(m64, th)
+    assert m32.last_fallback() == g.Fallback.JITTER and m32.last_jitter() == 0.0
+    assert v32 == v64 and np.array_equal(g32, g64)
 
 
 def test_value_only_and_repeat_deterministic(gpu):

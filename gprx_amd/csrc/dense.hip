@@ -236,14 +236,17 @@ template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 //
 // Layout in LDS (column stride DS): L lower (r >= c) at s[c * DS + r]; the
 // inverse X = L^-1 stored transposed above the diagonal, X[r][c] (r > c) at
-// s[r * DS + c]; X[r][r] in xd[r].  Steps, per 16-column panel:
+// s[r * DS + c]; X[r][r] = 1 / L[r][r] in xd[r].  Steps, per 16-column panel:
 //   (a) wave 0 factors the 16 x 16 diagonal sub-block in registers (lane r
-//       owns row r; pivots and columns are broadcast with v_readlane) and
-//       forms its inverse in the same sweep;
-//   (b) panel rows below: L_i = A_i X_D^T on MFMA;
+//       owns row r; pivots and columns are broadcast with v_readlane);
+//   (b) panel rows below, one row per lane: L_i = A_i L_D^-T by forward
+//       substitution;
 //   (c) trailing update A_ij -= L_i L_j^T (lower) on MFMA.
-// Then the off-diagonal blocks of X by doubling, X21 = -X22 L21 X11 over
-// runs of 16, 32 and 64 rows, on MFMA.
+// Then the inverses of the eight 16 x 16 diagonal blocks, one column per
+// lane, and the off-diagonal blocks of X by doubling, X21 = -X22 L21 X11
+// over runs of 16, 32 and 64 rows, on MFMA.  Only the factor's 128 pivots
+// form a serial chain; everything on it runs on one wave with the pivot
+// and column broadcasts in registers.
 // ---------------------------------------------------------------------------
 constexpr int DS = 129;
 
@@ -375,22 +378,15 @@ __device__ __forceinline__ void inverse_level(double* s, const double* xd) {
 // (transposed storage above the diagonal, diagonal in xd) and log L_ii.
 __device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag, int* sbad, int c0) {
   const int lane = threadIdx.x & 63;
-  // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k], xr[k] =
-  // X[r][k].  Entries right of the diagonal (k > r) start at zero and
-  // collect values no lower entry ever reads, so every update runs
-  // unconditionally.  Step j: pivot p = A[j][j], rs = p^-1/2; column j
-  // of L is d[j] * rs (lane j: p rs = sqrt(p)); rank-1 update of the
-  // columns right of j; row j of X = L^-1 is
-  //   X[j][c] = -rs * sum_{k<j} L[j][k] X[k][c],  X[j][j] = rs,
-  // whose sums (acc) gain column j of L as soon as it is final.
+  // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k].  Entries
+  // right of the diagonal (k > r) start at zero and collect values no lower
+  // entry ever reads, so every update runs unconditionally.  Step j: pivot
+  // p = A[j][j], rs = p^-1/2; column j of L is d[j] * rs (lane j: p rs =
+  // sqrt(p)); rank-1 update of the columns right of j.
   const int r = lane & 15;
-  double d[16], xr[16], acc[16];
+  double d[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    d[k] = s[(c0 + k) * DS + c0 + r];
-    xr[k] = 0.0;
-    acc[k] = 0.0;
-  }
+  for (int k = 0; k < 16; ++k) d[k] = s[(c0 + k) * DS + c0 + r];
   bool bad = false;
   double lrr = 1.0, xrr = 1.0;
 #pragma unroll
@@ -402,30 +398,67 @@ __device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag,
     }
     const double rs = rsqrt_nr(p);
     d[j] *= rs;
+    // column j + 1 first: it holds the next pivot
 #pragma unroll
     for (int c = j + 1; c < 16; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
     if (r == j) {
-#pragma unroll
-      for (int c = 0; c < j; ++c) xr[c] = -acc[c] * rs;
-      xr[j] = rs;
       lrr = d[j];
       xrr = rs;
     }
-#pragma unroll
-    for (int c = 0; c <= j; ++c) acc[c] = fma(d[j], lane_bcast(xr[c], j), acc[c]);
   }
   if (bad && lane == 0) *sbad = 1;
   if (lane < 16) {
     // Every register index is a compile-time constant: a lane-dependent
     // index (d[r]) would move the arrays to scratch memory.
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < 16; ++k)
       if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
-      if (k < r) s[(c0 + r) * DS + c0 + k] = xr[k];
-    }
     xd[c0 + r] = xrr;
     logdiag[c0 + r] = log(lrr);
   }
+}
+
+// Panel rows below the diagonal block of columns c0..c0+15: row R of
+// L_i = A_i L_D^-T, one row per thread, by forward substitution with the
+// columns of L_D read as broadcasts (every lane the same address):
+//   x_j = a_j / L[j][j],  a_c -= x_j L[c][j]  (c > j).
+__device__ __forceinline__ void panel_rows(double* s, const double* xd, int c0, int rows) {
+  const int t = threadIdx.x;
+  if (t >= rows) return;
+  const int R = c0 + 16 + t;
+  double a[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = s[(c0 + k) * DS + R];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    a[j] *= xd[c0 + j];
+#pragma unroll
+    for (int c = j + 1; c < 16; ++c) a[c] = fma(-a[j], s[(c0 + j) * DS + c0 + c], a[c]);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s[(c0 + k) * DS + R] = a[k];
+}
+
+// X_D = L_D^-1 for the eight 16 x 16 diagonal blocks: thread (b, c) of the
+// first 128 solves L_b x = e_c by forward substitution (x_k = 0 for k < c
+// without a branch) and stores X[k][c] (k > c) transposed above the
+// diagonal; X[c][c] is in xd already.
+__device__ __forceinline__ void invert_diag16(double* s, const double* xd) {
+  const int t = threadIdx.x;
+  if (t >= kTile) return;
+  const int o = (t >> 4) * 16, c = t & 15;
+  double b[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) b[k] = k == c ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    b[k] *= xd[o + k];
+#pragma unroll
+    for (int m = k + 1; m < 16; ++m) b[m] = fma(-b[k], s[(o + k) * DS + o + m], b[m]);
+  }
+#pragma unroll
+  for (int k = 1; k < 16; ++k)
+    if (k > c) s[(o + k) * DS + o + c] = b[k];
 }
 
 // A_(bi,bj) -= L_bi L_bj^T (panel columns c0 .. c0+15) for one block, or two
@@ -496,20 +529,8 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
   __syncthreads();
   for (int jb = 0; jb < 8; ++jb) {
     const int c0 = jb * 16;
-    // (b) panel: L_i = A_i X_D^T for the blocks below
-    for (int bi = jb + 1 + wave; bi < 8; bi += 4) {
-      d4 acc = d4{};
-#pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        const int jr = lane & 15;
-        const double a = s[(c0 + kr) * DS + bi * 16 + jr];
-        const double b = x_at(s, xd, c0 + jr, c0 + kr);
-        acc = mfma16(a, b, acc);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) s[(c0 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[q];
-    }
+    // (b) panel: L_i = A_i L_D^-T for the rows below
+    panel_rows(s, xd, c0, kTile - c0 - 16);
     __syncthreads();
     if (jb == 7) break;
     // (c) trailing update A_ij -= L_i L_j^T over the lower blocks of
@@ -534,8 +555,10 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
     __syncthreads();
   }
 
-  // X = L^-1 by doubling: with X of the 16 x 16 diagonal blocks known, for
-  // halves of HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
+  // X = L^-1: the 16 x 16 diagonal blocks, then by doubling over halves of
+  // HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
+  invert_diag16(s, xd);
+  __syncthreads();
   inverse_level<1>(s, xd);
   inverse_level<2>(s, xd);
   inverse_level<4>(s, xd);

@@ -25,9 +25,9 @@ constexpr double kLog2Pi = 1.8378770664093453;  // log(2 pi)
 struct ExactModel : gprx_hip_model {
   ExactModel() : gprx_hip_model(kExact) {}
   ~ExactModel() override {
+    if (stream) (void)hipStreamSynchronize(stream);
     fstreams.destroy();
     if (stream) {
-      (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);

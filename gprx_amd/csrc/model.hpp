@@ -62,32 +62,36 @@ inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long l
   gemm<T>(ta, tb, g, s);
 }
 
-// Streams of the recursive factor: `main` carries the recursion; each
-// node's product Tmp = L21 X11, which only the node's last step reads, runs
-// on `side` meanwhile (event ev[2 k] hands L21 over at depth k, ev[2 k + 1]
-// hands Tmp back), overlapping the latency-bound trailing recursion.  With
-// side == nullptr everything runs on main.  Both streams keep the default
+// Streams of the recursive factor.  `main` carries the recursion and its
+// latency-bound chain of small launches; products whose results are read
+// later (the far part of each trailing update and Tmp = L21 X11) run on
+// `bulk` meanwhile.  `bulk` is one stream per device shared by every model,
+// restricted to all CUs but kReservedCus (2 on each of the 8 XCDs), so the
+// chain's kernels always find free CUs while bulk products fill the rest.
+// Per recursion depth k, ev[3k] hands L21 to bulk, ev[3k+1] returns the
+// far update and ev[3k+2] returns Tmp.  Both streams keep the default
 // priority.
 constexpr int kMaxDepth = 24;
+constexpr int kReservedCus = 16;
+// Shared bulk stream of the current device (created on first use).
+hipStream_t bulk_stream();
 struct FactorStreams {
-  hipStream_t main = nullptr, side = nullptr;
-  hipEvent_t ev[2 * kMaxDepth] = {};
+  hipStream_t main = nullptr, bulk = nullptr;  // bulk is shared, not owned
+  hipEvent_t ev[3 * kMaxDepth] = {};
   void init(hipStream_t s) {
     main = s;
-    GPRX_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    bulk = bulk_stream();
     for (auto& e : ev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
+  // Every bulk product is waited for on main before the factor returns, so
+  // once main is synchronized no bulk work of this model is pending.
   void destroy() {
-    if (side) {
-      (void)hipStreamSynchronize(side);
-      (void)hipStreamDestroy(side);
-      side = nullptr;
-    }
     for (auto& e : ev)
       if (e) {
         (void)hipEventDestroy(e);
         e = nullptr;
       }
+    bulk = nullptr;
   }
 };
 

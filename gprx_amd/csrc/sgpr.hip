@@ -52,9 +52,9 @@ int guarded(gprx_error* err, F&& f) {
 struct SgprModel : gprx_hip_model {
   SgprModel() : gprx_hip_model(kSgpr) {}
   ~SgprModel() override {
+    if (stream) (void)hipStreamSynchronize(stream);
     fstreams.destroy();
     if (stream) {
-      (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);

@@ -283,9 +283,9 @@ inline int grid_for(long total) { return (int)std::min<long>((total + 255) / 256
 struct SvgpModel : gprx_hip_model {
   SvgpModel() : gprx_hip_model(kSvgp) {}
   ~SvgpModel() override {
+    if (stream) (void)hipStreamSynchronize(stream);
     fstreams.destroy();
     if (stream) {
-      (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);

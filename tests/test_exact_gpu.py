@@ -65,7 +65,8 @@ def _problem(n, d, seed, kname):
 
 
 CASES = [(1, 1, "rbf"), (7, 2, "const_ard"), (128, 3, "const_ard"), (129, 4, "const_ard_plus_periodic"),
-         (300, 8, "const_ard"), (257, 1, "sum_times"), (1000, 8, "const_ard"), (700, 16, "const_ard")]
+         (300, 8, "const_ard"), (257, 1, "sum_times"), (1000, 8, "const_ard"), (700, 16, "const_ard"),
+         (2500, 8, "const_ard")]
 
 
 @pytest.mark.parametrize("n,d,kname", CASES)
@@ -88,8 +89,11 @@ def test_random_vs_oracle_fp64(gpu, n, d, kname):
     assert_close(p.variance, oo, TOL64, "var")
 
 
-@pytest.mark.parametrize("n,d,kname", [(300, 8, "const_ard"), (1000, 8, "const_ard"), (257, 1, "sum_times")])
-def test_random_vs_oracle_mixed(gpu, n, d, kname):
+@pytest.mark.parametrize("n,d,kname,tol", [(300, 8, "const_ard", 1e-4), (1000, 8, "const_ard", 1e-4),
+                                           (257, 1, "sum_times", 1e-4), (2500, 8, "const_ard", 1e-3)])
+def test_random_vs_oracle_mixed(gpu, n, d, kname, tol):
+    """The log-determinant sums n pivots that each carry an f32 error, so
+    the NLML bound grows with n (2500: 1e-3 against a value of about -70)."""
     print(f"[exact mixed] n={n} d={d} kernel={kname}", flush=True)
     x, y, spec = _problem(n, d, n + d, kname)
     noise = 0.05
@@ -100,7 +104,7 @@ def test_random_vs_oracle_mixed(gpu, n, d, kname):
     ov, og, _, _ = oracle.exact_value_grad(spec.program(), th, x, n, d, y)
     print(f"  nlml err {rel_err(val, ov):.2e} grad err {rel_err(grad, og):.2e} "
           f"refine {m.last_refine_steps()} fallback {m.last_fallback()}")
-    assert_close(val, ov, 1e-4, "nlml")
+    assert_close(val, ov, tol, "nlml")
     assert_close(grad, og, 1e-2, "grad")
     assert m.last_fallback() == g.Fallback.NONE
 

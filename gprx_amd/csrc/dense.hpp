@@ -73,6 +73,18 @@ template <typename T>
 void gemv_t(const T* M, long ld, int rows, int cols, const double* v, double* out, int tri,
             int strict, int abs_mode, int accumulate, hipStream_t stream);
 
+// Products over the lower 128-tiles of an nt x nt tile matrix M (ld), one
+// workgroup per tile reading it once, then a fixed-order sum of the tile
+// partials (part: lower_tiles(nt) * 4 * 128 doubles):
+//   kTmvRow   out = L v     (L: the stored lower tiles, diagonal tiles c <= r)
+//   kTmvCol   out = L^T v
+//   kTmvSym   out = A v     (A symmetric, stored by its lower triangle)
+//   kTmvSymAbs  also out2 = |A| 1, the absolute row sums
+enum { kTmvRow = 1, kTmvCol = 2, kTmvSym = 3, kTmvSymAbs = 7 };
+template <typename T>
+void tile_mv_lower(const T* M, long ld, int nt, int mode, const double* v, double* out, double* out2,
+                   double* part, hipStream_t stream);
+
 // out_j = sum_i M_ij^2 (column sums of squares, f64 accumulation).
 template <typename T>
 void col_sumsq(const T* M, long ld, int rows, int cols, double* out, hipStream_t stream);

@@ -50,12 +50,12 @@ struct ExactModel : gprx_hip_model {
   FactorStreams fstreams;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double phase_ms[5] = {0, 0, 0, 0, 0};
-  DevBuf<double> x, y, ones;
+  DevBuf<double> x, y;
   DevBuf<DevProgram> prog;
   HostBuf<DevProgram> hprog;
   DevBuf<double> A64, X64, T64, K64;
   DevBuf<float> A32, X32, T32;
-  DevBuf<double> logdiag, alpha, tv, r, dlt, p1, p2, q1, q2, gpart, partials, accs, scal;
+  DevBuf<double> logdiag, alpha, tv, r, dlt, p1, q1, tpart, partials, accs, scal;
   DevBuf<int> info;
   HostBuf<double> hscal, hacc;
   HostBuf<int> hinfo;
@@ -105,8 +105,8 @@ struct ExactModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
     factor_inverse<double>(A64.p, X64.p, T64.p, npad, ldt, nt, logdiag.p, info.p, fstreams);
     GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
-    gemv_n<double>(X64.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
-    gemv_t<double>(X64.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
+    tile_mv_lower<double>(X64.p, npad, nt, kTmvRow, y.p, tv.p, nullptr, tpart.p, s);
+    tile_mv_lower<double>(X64.p, npad, nt, kTmvCol, tv.p, alpha.p, nullptr, tpart.p, s);
     dot_and_sum(y.p, alpha.p, logdiag.p, n, scal.p, s);
     GPRX_HIP_CHECK(hipEventRecord(ev[3], s));
     if (want_grad) {
@@ -133,11 +133,8 @@ struct ExactModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
     factor_inverse<float>(A32.p, X32.p, T32.p, npad, ldt, nt, logdiag.p, info.p, fstreams);
     GPRX_HIP_CHECK(hipEventRecord(ev[2], s));
-    gemv_n<float>(X32.p, npad, (int)npad, (int)npad, y.p, tv.p, gpart.p, 1, 0, 0, s);
-    gemv_t<float>(X32.p, npad, (int)npad, (int)npad, tv.p, alpha.p, 1, 0, 0, 0, s);
-    // ||A||_inf from the f64 lower triangle (row sums of |A|).
-    gemv_n<double>(K64.p, npad, (int)npad, (int)npad, ones.p, q1.p, gpart.p, 1, 0, 1, s);
-    gemv_t<double>(K64.p, npad, (int)npad, (int)npad, ones.p, q2.p, 1, 1, 1, 0, s);
+    tile_mv_lower<float>(X32.p, npad, nt, kTmvRow, y.p, tv.p, nullptr, tpart.p, s);
+    tile_mv_lower<float>(X32.p, npad, nt, kTmvCol, tv.p, alpha.p, nullptr, tpart.p, s);
     int bits = readback(0, false);
     if (bits) return bits;
     // Iterative refinement (src/precision/refine.rs:56-98): residual in f64
@@ -148,9 +145,10 @@ struct ExactModel : gprx_hip_model {
     bool ok = false;
     refine_steps = 0;
     for (int it = 0; it < 10; ++it) {
-      gemv_n<double>(K64.p, npad, (int)npad, (int)npad, alpha.p, p1.p, gpart.p, 1, 0, 0, s);
-      gemv_t<double>(K64.p, npad, (int)npad, (int)npad, alpha.p, p2.p, 1, 1, 0, 0, s);
-      residual_norms(y.p, p1.p, p2.p, alpha.p, it == 0 ? q1.p : nullptr, q2.p, r.p, n, scal.p + 4, s);
+      // A alpha from the f64 lower triangle; on the first pass also the
+      // absolute row sums, whose maximum is ||A||_inf
+      tile_mv_lower<double>(K64.p, npad, nt, it == 0 ? kTmvSymAbs : kTmvSym, alpha.p, p1.p, q1.p, tpart.p, s);
+      residual_norms(y.p, p1.p, nullptr, alpha.p, it == 0 ? q1.p : nullptr, nullptr, r.p, n, scal.p + 4, s);
       readback(8, false);
       const double rinf = hscal.p[4], winf = hscal.p[5], yinf = hscal.p[6], ainf = hscal.p[7];
       const double denom = ainf * winf + yinf;
@@ -167,8 +165,8 @@ struct ExactModel : gprx_hip_model {
         }
       }
       prev = rinf;
-      gemv_n<float>(X32.p, npad, (int)npad, (int)npad, r.p, tv.p, gpart.p, 1, 0, 0, s);
-      gemv_t<float>(X32.p, npad, (int)npad, (int)npad, tv.p, dlt.p, 1, 0, 0, 0, s);
+      tile_mv_lower<float>(X32.p, npad, nt, kTmvRow, r.p, tv.p, nullptr, tpart.p, s);
+      tile_mv_lower<float>(X32.p, npad, nt, kTmvCol, tv.p, dlt.p, nullptr, tpart.p, s);
       axpy(alpha.p, dlt.p, n, s);
       ++refine_steps;
     }
@@ -395,12 +393,10 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
     hipStream_t s = m->stream;
     upload_points(m->x, x, (int)n, (int)d, m->npad, m->dp, s);
     {
-      std::vector<double> yh(m->npad, 0.0), oh(m->npad, 1.0);
+      std::vector<double> yh(m->npad, 0.0);
       std::memcpy(yh.data(), y, sizeof(double) * n);
       m->y.alloc(m->npad);
-      m->ones.alloc(m->npad);
       GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
-      GPRX_HIP_CHECK(hipMemcpyAsync(m->ones.p, oh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
       GPRX_HIP_CHECK(hipStreamSynchronize(s));
     }
     m->prog.alloc(1);
@@ -412,11 +408,9 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
     m->r.alloc(np);
     m->dlt.alloc(np);
     m->p1.alloc(np);
-    m->p2.alloc(np);
     m->q1.alloc(np);
-    m->q2.alloc(np);
-    m->gpart.alloc((size_t)((np + 255) / 256) * np);
     m->partials.alloc((size_t)lower_tiles(np) * kMaxAcc);
+    m->tpart.alloc((size_t)lower_tiles(np) * 4 * kTile);
     m->accs.alloc(kMaxAcc);
     m->scal.alloc(16);
     m->info.alloc(1);

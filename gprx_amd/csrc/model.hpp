@@ -25,7 +25,7 @@ template <typename T>
 void noise_trace(const double* alpha, const T* M, long ld, int n, double* out, hipStream_t s);
 // r = y - (p1 + p2) for i < n (zero beyond); out[0] = max|r|, out[1] = max|w|,
 // out[2] = max|y|, out[3] = max(rowsum) of the absolute row sums in q1 + q2
-// (q1, q2 may be null: out[3] untouched).
+// (q1 null: out[3] untouched; p2 / q2 null: p1 / q1 alone).
 void residual_norms(const double* y, const double* p1, const double* p2, const double* w,
                     const double* q1, const double* q2, double* r, int n, double* out,
                     hipStream_t s);

@@ -77,12 +77,12 @@ __global__ __launch_bounds__(1024) void k_residual_norms(const double* y, const 
   __shared__ double m0[1024], m1[1024], m2[1024], m3[1024];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) {
-    const double ri = y[i] - (p1[i] + p2[i]);
+    const double ri = y[i] - (p2 ? p1[i] + p2[i] : p1[i]);
     r[i] = ri;
     a0 = fmax(a0, fabs(ri));
     a1 = fmax(a1, fabs(w[i]));
     a2 = fmax(a2, fabs(y[i]));
-    if (q1) a3 = fmax(a3, q1[i] + q2[i]);
+    if (q1) a3 = fmax(a3, q2 ? q1[i] + q2[i] : q1[i]);
   }
   m0[threadIdx.x] = a0;
   m1[threadIdx.x] = a1;

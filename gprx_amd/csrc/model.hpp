@@ -63,21 +63,20 @@ inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long l
 }
 
 // Streams of the recursive factor.  `main` carries the recursion and its
-// latency-bound chain of small launches; products whose results are read
-// later (the far part of each trailing update and Tmp = L21 X11) run on
-// `bulk` meanwhile.  `bulk` is one stream per device shared by every model,
-// restricted to all CUs but kReservedCus (2 on each of the 8 XCDs), so the
-// chain's kernels always find free CUs while bulk products fill the rest.
-// Per recursion depth k, ev[3k] hands L21 to bulk, ev[3k+1] returns the
-// far update and ev[3k+2] returns Tmp.  Both streams keep the default
-// priority.
+// latency-bound chain of small launches; each node's product Tmp = L21 X11,
+// read only by the node's last step, runs on `bulk` meanwhile.  `bulk` is
+// one stream per device shared by every model, restricted to all CUs but
+// kReservedCus (2 on each of the 8 XCDs), so the chain's kernels always
+// find free CUs while the bulk product fills the rest.  Per recursion
+// depth k, ev[2k] hands L21 to bulk and ev[2k+1] returns Tmp.  Both streams
+// keep the default priority.
 constexpr int kMaxDepth = 24;
 constexpr int kReservedCus = 16;
 // Shared bulk stream of the current device (created on first use).
 hipStream_t bulk_stream();
 struct FactorStreams {
   hipStream_t main = nullptr, bulk = nullptr;  // bulk is shared, not owned
-  hipEvent_t ev[3 * kMaxDepth] = {};
+  hipEvent_t ev[2 * kMaxDepth] = {};
   void init(hipStream_t s) {
     main = s;
     bulk = bulk_stream();

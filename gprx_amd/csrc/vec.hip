@@ -4,9 +4,6 @@
 
 #include "model.hpp"
 
-#ifndef GPRX_OVERLAP_MIN
-#define GPRX_OVERLAP_MIN 8
-#endif
 
 namespace gprx {
 
@@ -155,16 +152,16 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 // Flops: n^3/3 for the factor plus n^3/3 for the inverse, the large products
 // sitting at the top of the recursion.
 //
-// Lookahead: in a block of at least kOverlapMin tiles the update of A22 is
-// split at h2 = s2/2, the split rec(A22) makes next.  Its leading h2 x h2
-// triangle is updated on main, and rec(A22) starts on it at once, while
-// bulk updates the rows below (h2..s2).  rec(A22) waits for that far update
-// (`pend`) just before it first reads those rows.  Tmp goes to bulk after
-// everything rec(A22) put there, since the far updates are needed sooner,
-// and X21 waits for it.  The chain of small launches down the leading
-// diagonal so runs beside the large products instead of after them.
+// Overlap: in a block of at least kOverlapMin tiles, Tmp (read only by the
+// block's last step) is formed on the bulk stream, which leaves two CUs of
+// every XCD free, while main applies the update and runs rec(A22), a chain
+// of launches too small to fill the chip.  Measured on gfx950 at n = 16384
+// (factor + inverse, f32 / f64): 38.3 / 66.0 ms, against 41.6 / 71.2 ms on
+// one stream.  Splitting the update as well (its far rows on bulk, so
+// rec(A22) starts sooner) measured slower, 39.7 / 69.8 ms: the split
+// launches have worse tile tails and queue ahead of Tmp on bulk.
 // ---------------------------------------------------------------------------
-constexpr int kOverlapMin = GPRX_OVERLAP_MIN;
+constexpr int kOverlapMin = 8;
 
 hipStream_t bulk_stream() {
   static std::mutex mu;
@@ -194,59 +191,39 @@ hipStream_t bulk_stream() {
 
 template <typename T>
 static void rec(T* A, T* X, T* Tmp, long ld, long ldt, int o, int s, int depth, const long* toff,
-                double* logdiag, int* info, const FactorStreams& fs, hipEvent_t pend) {
+                double* logdiag, int* info, const FactorStreams& fs) {
   hipStream_t st = fs.main;
   auto at = [&](T* base, int r, int c) { return base + (long)r * kTile + (long)c * kTile * ld; };
   if (s == 1) {
-    if (pend) GPRX_HIP_CHECK(hipStreamWaitEvent(st, pend, 0));
     potrf_diag<T>(at(A, o, o), ld, at(X, o, o), ld, logdiag + (long)o * kTile, info, 1, 0, 0, 0, st);
     return;
   }
   if (depth >= kMaxDepth) throw DeviceError("factor recursion deeper than supported");
   const int s1 = s / 2, s2 = s - s1;
   const int o2 = o + s1;
-  rec<T>(A, X, Tmp, ld, ldt, o, s1, depth + 1, toff, logdiag, info, fs, nullptr);
-  if (pend) GPRX_HIP_CHECK(hipStreamWaitEvent(st, pend, 0));
+  rec<T>(A, X, Tmp, ld, ldt, o, s1, depth + 1, toff, logdiag, info, fs);
   // L21 = A21 X11^T (X11 lower, stored N x K: k <= j)
   gemm_call<T>(false, true, at(A, o2, o), ld, at(X, o, o), ld, at(X, o2, o), ld, s2 * kTile,
                s1 * kTile, s1 * kTile, 1.0, 0.0, kKhiCol, 0, kOrderRowFastRev, st);
-  const bool overlap = fs.bulk && s >= kOverlapMin && s2 >= 2;
-  const int h2 = overlap ? s2 / 2 : s2;
-  hipStream_t bs = st;
-  const hipEvent_t* ev = fs.ev + 3 * depth;
+  // Tmp = L21 X11 (X11 lower K x N: k >= j), this depth's block
+  const bool overlap = fs.bulk && s >= kOverlapMin;
+  const hipEvent_t* ev = fs.ev + 2 * depth;
+  hipStream_t ts = st;
   if (overlap) {
     GPRX_HIP_CHECK(hipEventRecord(ev[0], st));
     GPRX_HIP_CHECK(hipStreamWaitEvent(fs.bulk, ev[0], 0));
-    bs = fs.bulk;
+    ts = fs.bulk;
   }
-  // A22 -= L21 L21^T: the leading h2 x h2 triangle on main
-  gemm_call<T>(false, true, at(X, o2, o), ld, at(X, o2, o), ld, at(A, o2, o2), ld, h2 * kTile,
-               h2 * kTile, s1 * kTile, -1.0, 1.0, 0, 1, kOrderRowFast, st);
-  T* Tk = Tmp + toff[depth] * kTile * ldt;  // this depth's Tmp block
-  if (overlap) {
-    // rows h2..s2 of the update on bulk: the rectangle left of the
-    // diagonal, then the triangle on it
-    const int r = o2 + h2, m = s2 - h2;
-    gemm_call<T>(false, true, at(X, r, o), ld, at(X, o2, o), ld, at(A, r, o2), ld, m * kTile,
-                 h2 * kTile, s1 * kTile, -1.0, 1.0, 0, 0, kOrderRowFast, bs);
-    gemm_call<T>(false, true, at(X, r, o), ld, at(X, r, o), ld, at(A, r, r), ld, m * kTile, m * kTile,
-                 s1 * kTile, -1.0, 1.0, 0, 1, kOrderRowFast, bs);
-    GPRX_HIP_CHECK(hipEventRecord(ev[1], bs));
-  } else {
-    // Tmp = L21 X11 (X11 lower K x N: k >= j)
-    gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tk, ldt, s2 * kTile, s1 * kTile,
-                 s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, st);
-  }
-  rec<T>(A, X, Tmp, ld, ldt, o2, s2, depth + 1, toff, logdiag, info, fs, overlap ? ev[1] : nullptr);
-  if (overlap) {
-    // Tmp on bulk, queued behind the bulk products rec(A22) issued (which
-    // its chain waits for), so it fills the end of that recursion
-    gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tk, ldt, s2 * kTile, s1 * kTile,
-                 s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, bs);
-    GPRX_HIP_CHECK(hipEventRecord(ev[2], bs));
-    GPRX_HIP_CHECK(hipStreamWaitEvent(st, ev[2], 0));
-  }
+  T* Tk = Tmp + toff[depth] * kTile * ldt;
+  gemm_call<T>(false, false, at(X, o2, o), ld, at(X, o, o), ld, Tk, ldt, s2 * kTile, s1 * kTile,
+               s1 * kTile, 1.0, 0.0, kKloCol, 0, kOrderRowFast, ts);
+  if (overlap) GPRX_HIP_CHECK(hipEventRecord(ev[1], fs.bulk));
+  // A22 -= L21 L21^T
+  gemm_call<T>(false, true, at(X, o2, o), ld, at(X, o2, o), ld, at(A, o2, o2), ld, s2 * kTile,
+               s2 * kTile, s1 * kTile, -1.0, 1.0, 0, 1, kOrderRowFast, st);
+  rec<T>(A, X, Tmp, ld, ldt, o2, s2, depth + 1, toff, logdiag, info, fs);
   // X21 = -X22 Tmp (X22 lower: k <= i), over L21's storage
+  if (overlap) GPRX_HIP_CHECK(hipStreamWaitEvent(st, ev[1], 0));
   gemm_call<T>(false, false, at(X, o2, o2), ld, Tk, ldt, at(X, o2, o), ld, s2 * kTile, s1 * kTile,
                s2 * kTile, -1.0, 0.0, kKhiRow, 0, kOrderColFastRev, st);
 }
@@ -258,7 +235,7 @@ void factor_inverse(T* A, T* X, T* Tmp, long ld, long ldt, int nt, double* logdi
   long toff[kMaxDepth + 1] = {0};
   int depth = 0;
   for (int s = nt; s > 1 && depth < kMaxDepth; s = s - s / 2, ++depth) toff[depth + 1] = toff[depth] + s / 2;
-  rec<T>(A, X, Tmp, ld, ldt, 0, nt, 0, toff, logdiag, info, fs, nullptr);
+  rec<T>(A, X, Tmp, ld, ldt, 0, nt, 0, toff, logdiag, info, fs);
 }
 
 template void factor_inverse<double>(double*, double*, double*, long, long, int, double*, int*,

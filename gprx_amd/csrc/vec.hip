@@ -4,7 +4,6 @@
 
 #include "model.hpp"
 
-
 namespace gprx {
 
 bool all_finite(const double* p, size_t n) {

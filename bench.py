@@ -300,6 +300,7 @@ def bench_batch(args, gx):
         print(f"[bench] batch of {len(probs)}: {name} batched", file=sys.stderr, flush=True)
         bt = gx.GprBatch(probs, precision=prec)
         (nl, gr, er), wall_b = _timed(lambda: bt.value_and_gradient(thetas), args.steps, args.warmup)
+        batched_fallbacks = int((bt.last_fallback() != 0).sum())
         bt.close()
         print(f"[bench] batch of {len(probs)}: {name} looped single-problem path", file=sys.stderr,
               flush=True)
@@ -310,11 +311,12 @@ def bench_batch(args, gx):
             return [mdl.value_and_gradient_into(t, g_) for mdl, t, g_ in zip(models, thetas, grads)]
 
         vals, wall_l = _timed(loop, args.steps, args.warmup)
-        fallbacks = sum(mdl.last_fallback() for mdl in models)
+        fallbacks = sum(mdl.last_fallback() != 0 for mdl in models)
         for mdl in models:
             mdl.close()
         form = {"batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
                 "speedup": wall_l / wall_b, "failed": sum(e is not None for e in er),
+                "batched_fp64_fallbacks": batched_fallbacks,
                 "looped_fp64_fallbacks": int(fallbacks),
                 "batched_vs_looped_err": {"nlml": err1(nl, vals),
                                           "grad": max(err1(a, b) for a, b in zip(gr, grads))}}

@@ -1,6 +1,6 @@
 // The MFMA tile main loop shared by the dense kernels (see dense.hip for
 // the lane layouts).  A 256-thread workgroup computes one TT x TT tile
-// (TT = 128, or 64 for launches too small to fill the chip) of op(A) op(B)
+// (TT = 128, or 64 / 32 for launches too small to fill the chip) of op(A) op(B)
 // with four waves, staging BK-deep slices of both operands through
 // double-buffered LDS fed by a register prefetch kPrefetch slices deep.
 #pragma once
@@ -84,6 +84,18 @@ template <>
 struct Cfg<float, 64> {
   static constexpr int BK = 32;
   static constexpr int LS = 96;
+};
+// 32 x 32 tiles for launches of a few 128-tiles (one 16 x 16 block per
+// wave): the reduction of such a product spreads over more CUs.
+template <>
+struct Cfg<double, 32> {
+  static constexpr int BK = 16;
+  static constexpr int LS = 48;
+};
+template <>
+struct Cfg<float, 32> {
+  static constexpr int BK = 32;
+  static constexpr int LS = 32;
 };
 
 // Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.

@@ -73,8 +73,20 @@ __device__ __forceinline__ void run_task(const TileTask& t, int quad, T* X0, T* 
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
   const bool diag = t.lower && qr == qc;
+  // old values of a row of FB blocks before their stores (as k_gemm's epilogue)
 #pragma unroll
-  for (int i = 0; i < FB; ++i)
+  for (int i = 0; i < FB; ++i) {
+    T old[FB][4];
+    if (t.beta != 0.f) {
+#pragma unroll
+      for (int j = 0; j < FB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = wm * WT + i * 16 + (lane & 15);
+          const int c = wn * WT + j * 16 + drow<T>(lane, q);
+          old[j][q] = C[(long)c * t.ld + r];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < FB; ++j)
 #pragma unroll
@@ -82,11 +94,11 @@ __device__ __forceinline__ void run_task(const TileTask& t, int quad, T* X0, T* 
         const int r = wm * WT + i * 16 + (lane & 15);
         const int c = wn * WT + j * 16 + drow<T>(lane, q);
         if (diag && r < c) continue;
-        T* p = C + (long)c * t.ld + r;
         double v = (double)t.alpha * (double)acc[i][j][q];
-        if (t.beta != 0.f) v += (double)t.beta * (double)(*p);
-        *p = (T)v;
+        if (t.beta != 0.f) v += (double)t.beta * (double)old[j][q];
+        C[(long)c * t.ld + r] = (T)v;
       }
+  }
 }
 
 // One launch of a task range: slot s (blockIdx.x) runs sched[s] = task << 2

@@ -52,9 +52,25 @@ __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
   const T* Bblk = TB ? B + (long)tj * TT : B + (long)tj * TT * g.ldb;
   tile_mainloop<T, TA, TB, TT>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc);
 
+  // Epilogue.  With beta != 0 the old values of a row of FB blocks are read
+  // before any of them is stored (the tile lies inside C, so entries above
+  // a diagonal tile's diagonal are read and dropped): interleaved, each
+  // store could alias the next load and the reads would run one at a time.
   const bool diag_tile = g.lower && ti == tj;
+  T* Ct = C + (long)(tj * TT) * g.ldc + ti * TT;
 #pragma unroll
-  for (int i = 0; i < FB; ++i)
+  for (int i = 0; i < FB; ++i) {
+    T old[FB][4];
+    if (g.beta != 0.0) {
+#pragma unroll
+      for (int j = 0; j < FB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = wm * WT + i * 16 + (lane & 15);
+          const int c = wn * WT + j * 16 + drow<T>(lane, q);
+          old[j][q] = Ct[(long)c * g.ldc + r];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < FB; ++j)
 #pragma unroll
@@ -62,11 +78,11 @@ __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
         const int r = wm * WT + i * 16 + (lane & 15);
         const int c = wn * WT + j * 16 + drow<T>(lane, q);
         if (diag_tile && r < c) continue;
-        T* p = C + (long)(tj * TT + c) * g.ldc + ti * TT + r;
         double v = g.alpha * (double)acc[i][j][q];
-        if (g.beta != 0.0) v += g.beta * (double)(*p);
-        *p = (T)v;
+        if (g.beta != 0.0) v += g.beta * (double)old[j][q];
+        Ct[(long)c * g.ldc + r] = (T)v;
       }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -215,18 +215,32 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
     }
   };
   if (nk <= 0) return;
+  // Loads are never conditional: a slice past the end is replaced by the
+  // last one (read again, never used), and the steady-state loop carries no
+  // branch, so the compiler's wait counters track each slot's loads and the
+  // refill of a slot does not wait for the slices still in flight.
 #pragma unroll
-  for (int u = 0; u < P; ++u)
-    if (u < nk) gload(kb + u * BK, ra[u], rb[u]);
+  for (int u = 0; u < P; ++u) gload(kb + min(u, nk - 1) * BK, ra[u], rb[u]);
   lstore(0, ra[0], rb[0]);
   __syncthreads();
-  for (int it = 0; it < nk; it += P) {
+  int it = 0;
+  for (; it + P <= nk; it += P) {
 #pragma unroll
     for (int u = 0; u < P; ++u) {
       const int i = it + u;
-      if (i >= nk) break;
-      // slot u (slice i) is in LDS already: refill it with slice i + P
-      if (i + P < nk) gload(kb + (i + P) * BK, ra[u], rb[u]);
+      // slot u (slice i) is in LDS already: refill it with slice i + P; the
+      // last slice's store below goes to the idle buffer and is not read
+      gload(kb + min(i + P, nk - 1) * BK, ra[u], rb[u]);
+      compute(i & 1);
+      lstore((i + 1) & 1, ra[(u + 1) % P], rb[(u + 1) % P]);
+      __syncthreads();
+    }
+  }
+  // the last nk mod P slices: slot u holds slice it + u
+#pragma unroll
+  for (int u = 0; u < P - 1; ++u) {
+    const int i = it + u;
+    if (i < nk) {
       compute(i & 1);
       if (i + 1 < nk) lstore((i + 1) & 1, ra[(u + 1) % P], rb[(u + 1) % P]);
       __syncthreads();

@@ -2,7 +2,7 @@
 // the lane layouts).  A 256-thread workgroup computes one TT x TT tile
 // (TT = 128, or 64 / 32 for launches too small to fill the chip) of op(A) op(B)
 // with four waves, staging BK-deep slices of both operands through
-// double-buffered LDS fed by a register prefetch kPrefetch slices deep.
+// double-buffered LDS fed by a register prefetch Prefetch<T>::P slices deep.
 #pragma once
 
 #include "common.hpp"
@@ -58,8 +58,13 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
 }
 
 // Tile-kernel configuration for a TT x TT tile: K step and LDS row stride
-// (elements).  The stride is 32 dwords past a multiple of 64 banks, so the
-// four 16-lane groups of a fragment read alternate bank halves.  BK keeps
+// LS (elements).  A k-contiguous operand is stored transposed, a thread's
+// vector of k = k0..k0+VEC-1 going to rows k0.. of one column, while the
+// lanes of one store instruction differ in k0 by multiples of VEC: LS is
+// chosen so those rows start on different banks (4 * LS = 16 mod 64 dwords
+// for f32 128-tiles, with k0 in {0, 4, 8, 12}), keeping rows 16-byte
+// aligned for the vector stores of the other layout; fragment reads (16
+// consecutive elements per 16-lane group) stay conflict-free.  BK keeps
 // two 16-byte global loads per thread per operand and slice.
 template <typename T, int TT = 128>
 struct Cfg;
@@ -68,34 +73,34 @@ struct Cfg;
 template <>
 struct Cfg<double, 128> {
   static constexpr int BK = 8;
-  static constexpr int LS = 144;
+  static constexpr int LS = 132;
 };
 template <>
 struct Cfg<float, 128> {
   static constexpr int BK = 16;
-  static constexpr int LS = 160;
+  static constexpr int LS = 164;
 };
 template <>
 struct Cfg<double, 64> {
   static constexpr int BK = 16;
-  static constexpr int LS = 80;
+  static constexpr int LS = 66;
 };
 template <>
 struct Cfg<float, 64> {
   static constexpr int BK = 32;
-  static constexpr int LS = 96;
+  static constexpr int LS = 68;
 };
 // 32 x 32 tiles for launches of a few 128-tiles (one 16 x 16 block per
 // wave): the reduction of such a product spreads over more CUs.
 template <>
 struct Cfg<double, 32> {
   static constexpr int BK = 16;
-  static constexpr int LS = 48;
+  static constexpr int LS = 34;
 };
 template <>
 struct Cfg<float, 32> {
   static constexpr int BK = 32;
-  static constexpr int LS = 32;
+  static constexpr int LS = 36;
 };
 
 // Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.
@@ -125,18 +130,27 @@ struct TileSmem {
   __attribute__((aligned(16))) T b[2][Cfg<T, TT>::BK * Cfg<T, TT>::LS];
 };
 
-// Global-load stages in flight per thread: a slice leaves HBM kPrefetch
+// Global-load stages in flight per thread: a slice leaves HBM P
 // iterations before its MFMAs run, which hides the load latency of a launch
-// too small to have a second workgroup per CU to switch to.
-constexpr int kPrefetch = 3;
+// too small to have a second workgroup per CU to switch to.  Measured on
+// gfx950 (n = 16384 X^T X): f32 11.84 ms at 3 slices, 11.85 at 4, with 4
+// faster in the factor (34.4 against 35.1 ms); f64 would need 4 slices'
+// registers beyond its 256 and spill, so it keeps 3.
+template <typename T>
+struct Prefetch {
+  static constexpr int P = 3;
+};
+template <>
+struct Prefetch<float> {
+  static constexpr int P = 4;
+};
 
 // acc += op(A)[ti tile, kb..kb+nk*BK) op(B)[.., tj tile], with Ablk / Bblk
 // the tile origins: A "row-contiguous" operand has element (mn, k) at
 // P[mn + k * ld], a "k-contiguous" one at P[k + mn * ld].  Four waves in a
 // 2 x 2 grid each own a (TT/2)^2 quarter as FB x FB 16 x 16 MFMA blocks.
-// Slice i sits in register slot i % kPrefetch until it is copied to LDS
-// buffer i % 2; the iteration loop is unrolled by kPrefetch so every slot
-// index is static.
+// Slice i sits in register slot i % P until it is copied to LDS buffer
+// i % 2; the iteration loop is unrolled by P so every slot index is static.
 template <typename T, bool TA, bool TB, int TT = 128>
 __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long lda,
                                               const T* __restrict__ Bblk, long ldb, int kb, int nk,
@@ -145,7 +159,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   constexpr int BK = Cfg<T, TT>::BK;
   constexpr int LS = Cfg<T, TT>::LS;
   constexpr int VEC = 16 / sizeof(T);
-  constexpr int P = kPrefetch;
+  constexpr int P = Prefetch<T>::P;
   constexpr int FB = TT / 32, WT = TT / 2;
   using V = typename Vec<T>::V;
   constexpr int NV = TT * BK / VEC / 256;  // 16-byte vectors per thread per operand

@@ -69,7 +69,8 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
 template <typename T, int TT = 128>
 struct Cfg;
 // BK 8 (f64) / 16 (f32) with two workgroups per CU measured fastest on
-// gfx950 among BK in {8, 16, 32} and 1-2 workgroups per CU.
+// gfx950 among BK in {8, 16, 32} and 1-2 workgroups per CU (f32 BK 32, one
+// workgroup per CU by its LDS: X^T X at n = 16384 14.5 ms against 12.0).
 template <>
 struct Cfg<double, 128> {
   static constexpr int BK = 8;

@@ -116,9 +116,11 @@ __global__ void k_unpack_lq(const double* packed, int m, long ld, T* L, double* 
 
 // resid_c, var_c and the one-block sums [0] sum (resid^2 + var).
 __global__ __launch_bounds__(1024) void k_point_terms(const double* yb, const double* mu_a,
-                                                      const double* a2, const double* u2, double kxx,
-                                                      int b, double* resid, double* out) {
+                                                      const double* a2, const double* u2,
+                                                      const double* kxx_p, int b, double* resid,
+                                                      double* out) {
   __shared__ double red[1024];
+  const double kxx = *kxx_p;
   double acc = 0.0;
   for (int c = threadIdx.x; c < b; c += 1024) {
     const double r = yb[c] - mu_a[c];
@@ -342,15 +344,20 @@ struct SvgpModel : gprx_hip_model {
     partials2.alloc((size_t)(mpad / kTile) * (bcap / kTile) * kMaxAcc);
   }
 
+  // Fill and factor K_mm at jitter j (info bits in info; no wait).
+  void launch_factor(const double* th, double j) {
+    upload_program(th, j);
+    GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
+    fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, stream);
+    factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd.p, info.p, fstreams);
+  }
+
   // K_mm factor with the adaptive default jitter; returns a status.
   int factor_kmm(const double* th, gprx_error* err) {
     double j = 0.0, next = 1e-8;
     int remaining = 5;
     for (;;) {
-      upload_program(th, j);
-      GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
-      fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, stream);
-      factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd.p, info.p, fstreams);
+      launch_factor(th, j);
       GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
       GPRX_HIP_CHECK(hipStreamSynchronize(stream));
       const int bits = *hinfo.p;
@@ -373,15 +380,30 @@ struct SvgpModel : gprx_hip_model {
   // kernel + noise part also in th on the host) over the batch in bidx
   // (b indices).  Writes value, the theta gradient (host, n_theta) and the
   // variational gradient (device gmu, gL).
+  // The K_mm factor's first try (no jitter) is checked only with the
+  // step's own results, so the common case runs the whole step without a
+  // host round trip; if it failed, the jitter loop runs and the step is
+  // evaluated again.
   template <typename T>
   int step(const double* th, long b, double* value, double* gtheta, gprx_error* err,
            T* Xc, T* Lc, T* Kzx, T* A, T* U, T* G, T* LU, T* Wmn, T* Mq, T* Wpre) {
+    launch_factor(th, 0.0);
+    const int rc = step_body<T>(th, b, value, gtheta, err, Xc, Lc, Kzx, A, U, G, LU, Wmn, Mq, Wpre);
+    if (*hinfo.p == 0) return rc;
+    const int frc = factor_kmm(th, err);
+    if (frc != GPRX_OK) return frc;
+    return step_body<T>(th, b, value, gtheta, err, Xc, Lc, Kzx, A, U, G, LU, Wmn, Mq, Wpre);
+  }
+
+  // The step after K_mm's factor has been launched; leaves the factor's
+  // info bits in hinfo.
+  template <typename T>
+  int step_body(const double* th, long b, double* value, double* gtheta, gprx_error* err,
+                T* Xc, T* Lc, T* Kzx, T* A, T* U, T* G, T* LU, T* Wmn, T* Mq, T* Wpre) {
     hipStream_t s = stream;
     const int P = hp.n_params;
     const double s2 = std::exp(th[P]);
     const double scale = (double)n / (double)b;
-    int rc = factor_kmm(th, err);
-    if (rc != GPRX_OK) return rc;
     GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
     const long bp = (long)round_up((size_t)b, kTile);
     const int M = (int)mpad, B = (int)bp;
@@ -407,12 +429,9 @@ struct SvgpModel : gprx_hip_model {
     gemv_t<T>(A, mpad, M, B, pu + n_theta, mu_a.p, 0, 0, 0, 0, s);
     col_sumsq<T>(A, mpad, M, B, a2.p, s);
     col_sumsq<T>(U, mpad, M, B, u2.p, s);
-    // resid, var; scal[0] = sum (resid^2 + var)  (needs k(x,x) in scal[7])
-    GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p + 7, scal.p + 7, sizeof(double), hipMemcpyDeviceToHost, s));
-    GPRX_HIP_CHECK(hipStreamSynchronize(s));
-    const double kxx = hscal.p[7];
-    hipLaunchKernelGGL(k_point_terms, dim3(1), dim3(1024), 0, s, yb.p, mu_a.p, a2.p, u2.p, kxx,
-                       (int)b, resid.p, scal.p);
+    // resid, var; scal[0] = sum (resid^2 + var)  (k(x,x) read from scal[7])
+    hipLaunchKernelGGL(k_point_terms, dim3(1), dim3(1024), 0, s, yb.p, mu_a.p, a2.p, u2.p,
+                       scal.p + 7, (int)b, resid.p, scal.p);
     gemv_n<T>(A, mpad, M, B, resid.p, Ares.p, partials.p, 0, 0, 0, s);
     // Mq = A U^T (lower used)
     gemm_call<T>(false, true, A, mpad, U, mpad, Mq, mpad, M, M, B, 1.0, 0.0, 0, 1, kOrderRowFast, s);
@@ -444,7 +463,9 @@ struct SvgpModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));
     GPRX_HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     GPRX_HIP_CHECK(hipMemcpyAsync(hacc.p, accs.p, sizeof(double) * 3 * kMaxAcc, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(hinfo.p, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
     GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    if (*hinfo.p) return GPRX_OK;  // the caller factors again with jitter
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
     last_eval_ms = ms;

@@ -853,25 +853,56 @@ __global__ __launch_bounds__(256) void k_tile_mv_lower(const T* __restrict__ M, 
 }
 
 // out[b * 128 + i] = row partials of tiles (b, 0..b), then column partials
-// of tiles (b..nt-1, b), in that order.
+// of tiles (b..nt-1, b), each list summed in eight interleaved running
+// sums (fixed order) so eight independent loads are in flight per thread.
+__device__ __forceinline__ double sum_strided(const double* __restrict__ part, long first, long step,
+                                              int count) {
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int k = 0;
+  for (; k + 8 <= count; k += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] += part[first + (long)(k + u) * step];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (k + u < count) s[u] += part[first + (long)(k + u) * step];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 __global__ void k_tile_mv_reduce(const double* __restrict__ part, int nt, int mode,
                                  double* __restrict__ out, double* __restrict__ out2) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nt * kTile) return;
   const int b = i / kTile, ii = i % kTile;
   double acc = 0.0, acc2 = 0.0;
-  if (mode & 1)
-    for (int j = 0; j <= b; ++j) {
-      const double* p = part + ((long)b * (b + 1) / 2 + j) * 4 * kTile;
-      acc += p[ii];
-      if (out2) acc2 += p[2 * kTile + ii];
+  // row partials: tiles (b, j), j = 0..b, consecutive tile indices
+  const long row0 = (long)b * (b + 1) / 2 * 4 * kTile;
+  if (mode & 1) {
+    acc += sum_strided(part, row0 + ii, 4 * kTile, b + 1);
+    if (out2) acc2 += sum_strided(part, row0 + 2 * kTile + ii, 4 * kTile, b + 1);
+  }
+  // column partials: tiles (q, b), q = b..nt-1, tile index q (q + 1) / 2 + b
+  if (mode & 2) {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int q = b;
+    for (; q + 8 <= nt; q += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double* p = part + ((long)(q + u) * (q + u + 1) / 2 + b) * 4 * kTile;
+        s[u] += p[kTile + ii];
+        if (out2) s2[u] += p[3 * kTile + ii];
+      }
     }
-  if (mode & 2)
-    for (int q = b; q < nt; ++q) {
-      const double* p = part + ((long)q * (q + 1) / 2 + b) * 4 * kTile;
-      acc += p[kTile + ii];
-      if (out2) acc2 += p[3 * kTile + ii];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (q + u >= nt) continue;
+      const double* p = part + ((long)(q + u) * (q + u + 1) / 2 + b) * 4 * kTile;
+      s[u] += p[kTile + ii];
+      if (out2) s2[u] += p[3 * kTile + ii];
     }
+    acc += ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    acc2 += ((s2[0] + s2[1]) + (s2[2] + s2[3])) + ((s2[4] + s2[5]) + (s2[6] + s2[7]));
+  }
   out[i] = acc;
   if (out2) out2[i] = acc2;
 }

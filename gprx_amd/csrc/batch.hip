@@ -27,6 +27,7 @@
 #include "kernel_eval.hpp"
 #include "model.hpp"
 #include "tile_gemm.hpp"
+#include "tile_mv.hpp"
 
 namespace gprx {
 int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err);
@@ -251,86 +252,44 @@ __global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial
   if (threadIdx.x == 0) out[(long)p * kMaxAcc + k] = red[0];
 }
 
-// Matrix-vector products over lower 128-tiles (prob, ti, tj), accumulated
-// with atomics into out (cleared beforehand):
-//   row: out[ti-rows] += M_tile v[tj-cols]      (diagonal tile: c <= r, or c < r if row_strict)
-//   col: out[tj-cols] += M_tile^T v[ti-rows]    (diagonal tile: r >= c, or r > c if col_strict)
-// abs_mode uses |M|.  The tile streams through LDS in 32-column slabs with
-// coalesced loads; row and column partial sums are both formed from the
-// slab, so a symmetric product (row + strict col) reads the lower triangle
-// once.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tile_mv(const FillTask* __restrict__ tasks,
-                                                 const ProbDesc* __restrict__ probs,
-                                                 const T* __restrict__ Mar, const double* __restrict__ v,
-                                                 double* __restrict__ out, int row, int col,
-                                                 int row_strict, int col_strict, int abs_mode) {
-  __shared__ double slab[32][kTile + 1];
-  __shared__ double vr[kTile], vc[kTile];
-  __shared__ double rpart[2][kTile];
-  __shared__ double cpart[8][32];
-  const int tid = threadIdx.x;
+// Matrix-vector products over the lower 128-tiles (prob, ti, tj) of every
+// problem (k_tile_mv_lower's tile body per task): task partials go to slot
+// fbeg[prob] + ti (ti + 1) / 2 + tj of part (4 * 128 doubles each), which
+// k_tile_mv_reduce_tasks adds per output block in a fixed order.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_tile_mv_tasks(const FillTask* __restrict__ tasks,
+                                                       const ProbDesc* __restrict__ probs,
+                                                       const int* __restrict__ fbeg,
+                                                       const T* __restrict__ Mar,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ part) {
   const FillTask t = tasks[blockIdx.x];
   const ProbDesc pd = probs[t.prob];
-  const int r0 = t.ti * kTile, c0 = t.tj * kTile;
-  const T* M = Mar + pd.moff + r0 + (long)c0 * pd.npad;
-  if (tid < kTile) {
-    vr[tid] = v[pd.voff + r0 + tid];
-    vc[tid] = v[pd.voff + c0 + tid];
-  }
-  const bool diag = t.ti == t.tj;
-  const int rr = tid & 127, half = tid >> 7;
-  double racc = 0.0;
-  for (int sb = 0; sb < 4; ++sb) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int e = tid + 256 * i;
-      const int r = e & 127, cc = e >> 7;
-      const double m = (double)M[r + (long)(sb * 32 + cc) * pd.npad];
-      slab[cc][r] = abs_mode ? fabs(m) : m;
-    }
-    __syncthreads();
-    if (row) {
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int cc = half * 16 + j, c = sb * 32 + cc;
-        const bool in = !diag || c < rr || (c == rr && !row_strict);
-        racc += in ? slab[cc][rr] * vc[c] : 0.0;
-      }
-    }
-    if (col) {
-      const int cc = tid & 31, part = tid >> 5, c = sb * 32 + cc;
-      double cacc = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = part * 16 + j;
-        const bool in = !diag || r > c || (r == c && !col_strict);
-        cacc += in ? slab[cc][r] * vr[r] : 0.0;
-      }
-      cpart[part][cc] = cacc;
-    }
-    __syncthreads();
-    if (col && tid < 32) {
-      double sum = 0.0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) sum += cpart[q][tid];
-      unsafeAtomicAdd(out + pd.voff + c0 + sb * 32 + tid, sum);
-    }
-  }
-  if (row) {
-    rpart[half][rr] = racc;
-    __syncthreads();
-    if (tid < kTile) unsafeAtomicAdd(out + pd.voff + r0 + tid, rpart[0][tid] + rpart[1][tid]);
-  }
+  const T* Mt = Mar + pd.moff + (long)t.ti * kTile + (long)t.tj * kTile * pd.npad;
+  const long slot = fbeg[t.prob] + (long)t.ti * (t.ti + 1) / 2 + t.tj;
+  tile_mv_tile<T, MODE>(Mt, pd.npad, v + pd.voff + (long)t.ti * kTile, v + pd.voff + (long)t.tj * kTile,
+                        t.ti == t.tj, part + slot * 4 * kTile);
 }
 
-__global__ void k_zero(double* p, long n) {
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0.0;
-}
-
-__global__ void k_fill_value(double* p, long n, double v) {
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = v;
+// out (and out2) over the vector arena: row v of the problem owning its
+// 128-block; zero for a problem with active[p] == 0 (active may be null).
+__global__ void k_tile_mv_reduce_tasks(const double* __restrict__ part,
+                                       const ProbDesc* __restrict__ probs,
+                                       const int* __restrict__ blk_prob, const int* __restrict__ fbeg,
+                                       const int* __restrict__ active, long vtot, int mode,
+                                       double* __restrict__ out, double* __restrict__ out2) {
+  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= vtot) return;
+  const int p = blk_prob[v / kTile];
+  const ProbDesc pd = probs[p];
+  double acc = 0.0, acc2 = 0.0;
+  if (!active || active[p]) {
+    const long local = v - pd.voff;
+    tile_mv_sum(part + (long)fbeg[p] * 4 * kTile, pd.npad / kTile, (int)(local / kTile),
+                (int)(local % kTile), mode, out2 != nullptr, acc, acc2);
+  }
+  out[v] = acc;
+  if (out2) out2[v] = acc2;
 }
 
 // Per-problem scalars, one block each:
@@ -521,6 +480,8 @@ struct BatchModel : gprx_hip_model {
   std::vector<DimGroup> dp_groups;
   DevBuf<FillTask> act_tasks;   // lower tiles of the problems still refining
   HostBuf<FillTask> hact;
+  DevBuf<double> mvpart;        // tile matrix-vector partials, 4 * 128 per lower tile
+  DevBuf<int> d_blkprob;        // problem of each 128-block of the vector arena
 
   bool mixed() const { return precision == GPRX_MIXED; }
 
@@ -765,18 +726,36 @@ struct BatchModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
-  // out = products of k_tile_mv over the lower tiles of every problem, or
-  // of the n_act tiles in act_tasks (out is cleared whole first).
+  // out (out2) = a tile product (kTmvRow / kTmvCol / kTmvSym / kTmvSymAbs)
+  // over the lower tiles of every problem, or over the n_act tiles in
+  // act_tasks, with out zero for the other problems.
   template <typename T>
-  void tile_mv(const T* M, const double* v, double* out, int row, int col, int row_strict,
-               int col_strict, int abs_mode, int n_act = -1) {
-    hipLaunchKernelGGL(k_zero, dim3(std::min<long>((vtot + 255) / 256, 4096)), dim3(256), 0, stream, out,
-                       vtot);
+  void tile_mv(const T* M, const double* v, double* out, int mode, double* out2 = nullptr, int n_act = -1) {
     const int count = n_act < 0 ? n_fill : n_act;
-    if (count > 0)
-      hipLaunchKernelGGL((k_tile_mv<T>), dim3(count), dim3(256), 0, stream,
-                         n_act < 0 ? fill_tasks.p : act_tasks.p, probs.p, M, v, out, row, col,
-                         row_strict, col_strict, abs_mode);
+    const FillTask* t = n_act < 0 ? fill_tasks.p : act_tasks.p;
+    if (count > 0) {
+      switch (mode) {
+        case kTmvRow:
+          hipLaunchKernelGGL((k_tile_mv_tasks<T, kTmvRow>), dim3(count), dim3(256), 0, stream, t, probs.p,
+                             d_fbeg.p, M, v, mvpart.p);
+          break;
+        case kTmvCol:
+          hipLaunchKernelGGL((k_tile_mv_tasks<T, kTmvCol>), dim3(count), dim3(256), 0, stream, t, probs.p,
+                             d_fbeg.p, M, v, mvpart.p);
+          break;
+        case kTmvSym:
+          hipLaunchKernelGGL((k_tile_mv_tasks<T, kTmvSym>), dim3(count), dim3(256), 0, stream, t, probs.p,
+                             d_fbeg.p, M, v, mvpart.p);
+          break;
+        default:
+          hipLaunchKernelGGL((k_tile_mv_tasks<T, kTmvSymAbs>), dim3(count), dim3(256), 0, stream, t,
+                             probs.p, d_fbeg.p, M, v, mvpart.p);
+          break;
+      }
+    }
+    hipLaunchKernelGGL(k_tile_mv_reduce_tasks, dim3((unsigned)((vtot + 255) / 256)), dim3(256), 0, stream,
+                       mvpart.p, probs.p, d_blkprob.p, d_fbeg.p, n_act < 0 ? nullptr : active.p, vtot,
+                       mode & 3, out, mode == kTmvSymAbs ? out2 : nullptr);
     GPRX_HIP_CHECK(hipGetLastError());
   }
 
@@ -942,8 +921,8 @@ struct BatchModel : gprx_hip_model {
   void record_fp64() {
     fill<double>(A64.p);
     factor<double>(A64.p, P64.p);
-    tile_mv<double>(P64.p, yar.p, tv.p, 1, 0, 0, 0, 0);     // X y
-    tile_mv<double>(P64.p, tv.p, alpha.p, 0, 1, 0, 0, 0);   // X^T (X y)
+    tile_mv<double>(P64.p, yar.p, tv.p, kTmvRow);     // X y
+    tile_mv<double>(P64.p, tv.p, alpha.p, kTmvCol);   // X^T (X y)
     launch_tasks<double>(lauum_s, A64.p, P64.p);
     hipLaunchKernelGGL((k_problem_scalars<double>), dim3(K), dim3(256), 0, stream, probs.p, yar.p,
                        alpha.p, logd.p, A64.p, scal.p);
@@ -1072,6 +1051,13 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
     B.hacc.alloc(k * kMaxAcc);
     B.build_tasks();
     B.act_tasks.alloc(std::max(B.n_fill, 1));
+    B.mvpart.alloc((size_t)std::max(B.n_fill, 1) * 4 * kTile);
+    {
+      std::vector<int> bp(B.vtot / kTile);
+      for (int p = 0; p < B.K; ++p)
+        for (long q = B.hp[p].voff / kTile; q < (B.hp[p].voff + B.hp[p].npad) / kTile; ++q) bp[q] = p;
+      upload(B.d_blkprob, bp, s);
+    }
     prepare_dense_kernels();
     GPRX_HIP_CHECK(hipStreamSynchronize(s));
     B.partial.alloc((size_t)B.n_fill * kMaxAcc);
@@ -1120,18 +1106,15 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
       // Mixed: f32 arenas, f64 diagonal blocks, alpha refined with f64 residuals.
       B->fill<float>(B->A32.p);
       B->factor<float>(B->A32.p, B->P32.p);
-      B->tile_mv<float>(B->P32.p, B->yar.p, B->tv.p, 1, 0, 0, 0, 0);
-      B->tile_mv<float>(B->P32.p, B->tv.p, B->alpha.p, 0, 1, 0, 0, 0);
-      // row abs sums of A (first residual pass uses them): symmetric, one pass
-      hipLaunchKernelGGL(k_fill_value, dim3(std::min<long>((B->vtot + 255) / 256, 4096)), dim3(256), 0, s,
-                         B->dlt.p, B->vtot, 1.0);
-      B->tile_mv<double>(B->K64.p, B->dlt.p, B->q1.p, 1, 1, 0, 1, 1);
+      B->tile_mv<float>(B->P32.p, B->yar.p, B->tv.p, kTmvRow);
+      B->tile_mv<float>(B->P32.p, B->tv.p, B->alpha.p, kTmvCol);
       std::vector<int> act(B->K, 1), streak(B->K, 0), conv(B->K, 0);
       std::vector<double> prev(B->K, -1.0);
       int n_act = -1;  // every problem
       for (int it = 0; it < 10; ++it) {
-        // p1 = A alpha over the lower triangle (row + strict transpose)
-        B->tile_mv<double>(B->K64.p, B->alpha.p, B->p1.p, 1, 1, 0, 1, 0, n_act);
+        // p1 = A alpha over the lower triangle; on the first pass also the
+        // absolute row sums (max: ||A||_inf)
+        B->tile_mv<double>(B->K64.p, B->alpha.p, B->p1.p, it == 0 ? kTmvSymAbs : kTmvSym, B->q1.p, n_act);
         hipLaunchKernelGGL(k_problem_residual, dim3(B->K), dim3(256), 0, s, B->probs.p, B->yar.p,
                            B->p1.p, nullptr, B->alpha.p, it == 0 ? B->q1.p : nullptr, nullptr,
                            B->r.p, B->scal.p);
@@ -1170,8 +1153,8 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
         GPRX_HIP_CHECK(hipMemcpyAsync(B->active.p, B->hactive.p, sizeof(int) * B->K, hipMemcpyHostToDevice, s));
         n_act = B->set_active(act);
         // correction d = X^T (X r) on the problems still refining
-        B->tile_mv<float>(B->P32.p, B->r.p, B->tv.p, 1, 0, 0, 0, 0, n_act);
-        B->tile_mv<float>(B->P32.p, B->tv.p, B->dlt.p, 0, 1, 0, 0, 0, n_act);
+        B->tile_mv<float>(B->P32.p, B->r.p, B->tv.p, kTmvRow, nullptr, n_act);
+        B->tile_mv<float>(B->P32.p, B->tv.p, B->dlt.p, kTmvCol, nullptr, n_act);
         hipLaunchKernelGGL(k_masked_axpy, dim3((65536 + 255) / 256, B->K), dim3(256), 0, s,
                            B->probs.p, B->active.p, B->alpha.p, B->dlt.p);
       }

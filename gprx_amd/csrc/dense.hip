@@ -21,6 +21,7 @@
 
 #include "dense.hpp"
 #include "tile_gemm.hpp"
+#include "tile_mv.hpp"
 
 namespace gprx {
 
@@ -763,146 +764,22 @@ void col_sumsq(const T* M, long ld, int rows, int cols, double* out, hipStream_t
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
-// One lower tile (ti, tj) per workgroup, streamed in four 32-column slabs:
-// thread (r, h) holds row r of the slab's columns h, h + 2, ..., forms its
-// row sums from registers and parks the slab in LDS, where thread
-// (c, part) sums 16 rows of column c.  Row partials (row-product tiles of
-// L v, and of A v with the diagonal tile's c <= r) go to slot 0, column
-// partials (L^T v; strict r > c on the diagonal tile for A v) to slot 1,
-// the absolute sums of kTmvSymAbs to slots 2 and 3.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_mv_lower(const T* __restrict__ M, long ld,
                                                        const double* __restrict__ v,
                                                        double* __restrict__ part) {
-  constexpr bool kRow = MODE & 1, kCol = MODE & 2, kSym = MODE == kTmvSym || MODE == kTmvSymAbs;
-  constexpr bool kAbs = MODE == kTmvSymAbs;
-  __shared__ double slab[32][kTile + 1];
-  __shared__ double vr[kTile], vc[kTile];
-  __shared__ double cred[2][8][32];
-  __shared__ double rred[2][2][kTile];
-  const int tid = threadIdx.x;
   int ti, tj;
   tri_index(blockIdx.x, ti, tj);
-  const T* Mt = M + (long)ti * kTile + (long)tj * kTile * ld;
-  if (tid < kTile) {
-    vr[tid] = v[ti * kTile + tid];
-    vc[tid] = v[tj * kTile + tid];
-  }
-  const bool diag = ti == tj;
-  const int r = tid & 127, h = tid >> 7;
-  double racc = 0.0, racc2 = 0.0;
-  T mv[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) mv[j] = Mt[r + (long)(h + 2 * j) * ld];
-#pragma unroll 1
-  for (int sb = 0; sb < 4; ++sb) {
-    double m[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) m[j] = (double)mv[j];
-    if (sb < 3) {
-#pragma unroll
-      for (int j = 0; j < 16; ++j) mv[j] = Mt[r + (long)((sb + 1) * 32 + h + 2 * j) * ld];
-    }
-    __syncthreads();  // vr / vc stored; the previous slab's column sums read
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int cc = h + 2 * j, c = sb * 32 + cc;
-      const bool in = !diag || c <= r;
-      if (kRow) racc += in ? m[j] * vc[c] : 0.0;
-      if (kAbs) racc2 += in ? fabs(m[j]) : 0.0;
-      if (kCol) slab[cc][r] = m[j];
-    }
-    if (kCol) {
-      __syncthreads();
-      const int cc = tid & 31, pt = tid >> 5, c = sb * 32 + cc;
-      double a = 0.0, a2 = 0.0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int rr = pt * 16 + k;
-        const bool in = !diag || (kSym ? rr > c : rr >= c);
-        const double x = slab[cc][rr];
-        a += in ? x * vr[rr] : 0.0;
-        if (kAbs) a2 += in ? fabs(x) : 0.0;
-      }
-      cred[0][pt][cc] = a;
-      if (kAbs) cred[1][pt][cc] = a2;
-      __syncthreads();
-      if (tid < 32) {
-        double sum = 0.0, sum2 = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          sum += cred[0][q][tid];
-          if (kAbs) sum2 += cred[1][q][tid];
-        }
-        double* out = part + (long)blockIdx.x * 4 * kTile;
-        out[kTile + sb * 32 + tid] = sum;
-        if (kAbs) out[3 * kTile + sb * 32 + tid] = sum2;
-      }
-    }
-  }
-  if (kRow) {
-    rred[0][h][r] = racc;
-    if (kAbs) rred[1][h][r] = racc2;
-    __syncthreads();
-    if (tid < kTile) {
-      double* out = part + (long)blockIdx.x * 4 * kTile;
-      out[tid] = rred[0][0][tid] + rred[0][1][tid];
-      if (kAbs) out[2 * kTile + tid] = rred[1][0][tid] + rred[1][1][tid];
-    }
-  }
-}
-
-// out[b * 128 + i] = row partials of tiles (b, 0..b), then column partials
-// of tiles (b..nt-1, b), each list summed in eight interleaved running
-// sums (fixed order) so eight independent loads are in flight per thread.
-__device__ __forceinline__ double sum_strided(const double* __restrict__ part, long first, long step,
-                                              int count) {
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int k = 0;
-  for (; k + 8 <= count; k += 8) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) s[u] += part[first + (long)(k + u) * step];
-  }
-#pragma unroll
-  for (int u = 0; u < 8; ++u)
-    if (k + u < count) s[u] += part[first + (long)(k + u) * step];
-  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+  tile_mv_tile<T, MODE>(M + (long)ti * kTile + (long)tj * kTile * ld, ld, v + ti * kTile, v + tj * kTile,
+                        ti == tj, part + (long)blockIdx.x * 4 * kTile);
 }
 
 __global__ void k_tile_mv_reduce(const double* __restrict__ part, int nt, int mode,
                                  double* __restrict__ out, double* __restrict__ out2) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nt * kTile) return;
-  const int b = i / kTile, ii = i % kTile;
-  double acc = 0.0, acc2 = 0.0;
-  // row partials: tiles (b, j), j = 0..b, consecutive tile indices
-  const long row0 = (long)b * (b + 1) / 2 * 4 * kTile;
-  if (mode & 1) {
-    acc += sum_strided(part, row0 + ii, 4 * kTile, b + 1);
-    if (out2) acc2 += sum_strided(part, row0 + 2 * kTile + ii, 4 * kTile, b + 1);
-  }
-  // column partials: tiles (q, b), q = b..nt-1, tile index q (q + 1) / 2 + b
-  if (mode & 2) {
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int q = b;
-    for (; q + 8 <= nt; q += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double* p = part + ((long)(q + u) * (q + u + 1) / 2 + b) * 4 * kTile;
-        s[u] += p[kTile + ii];
-        if (out2) s2[u] += p[3 * kTile + ii];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (q + u >= nt) continue;
-      const double* p = part + ((long)(q + u) * (q + u + 1) / 2 + b) * 4 * kTile;
-      s[u] += p[kTile + ii];
-      if (out2) s2[u] += p[3 * kTile + ii];
-    }
-    acc += ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-    acc2 += ((s2[0] + s2[1]) + (s2[2] + s2[3])) + ((s2[4] + s2[5]) + (s2[6] + s2[7]));
-  }
+  double acc, acc2;
+  tile_mv_sum(part, nt, i / kTile, i % kTile, mode, out2 != nullptr, acc, acc2);
   out[i] = acc;
   if (out2) out2[i] = acc2;
 }

@@ -756,12 +756,29 @@ __global__ __launch_bounds__(256) void k_gemv_t(const T* __restrict__ M, long ld
   const int r0 = tri ? (strict ? j + 1 : j) : 0;
   const T* col = M + (long)j * ld;
   double acc = 0.0;
-  for (int i = r0 + lane; i < rows; i += 64) {
-    const double m = (double)col[i];
-    if (sumsq)
-      acc += m * m;
-    else
-      acc += (abs_mode ? fabs(m) : m) * v[i];
+  constexpr int VEC = 16 / sizeof(T);
+  using V = typename Vec<T>::V;
+  if (r0 == 0 && rows % (64 * VEC) == 0 && ld % VEC == 0) {
+    // whole 16-byte-aligned columns: one 16-byte load per lane and step
+    for (int i = lane * VEC; i < rows; i += 64 * VEC) {
+      const V mv = *reinterpret_cast<const V*>(col + i);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const double m = (double)mv[e];
+        if (sumsq)
+          acc += m * m;
+        else
+          acc += (abs_mode ? fabs(m) : m) * v[i + e];
+      }
+    }
+  } else {
+    for (int i = r0 + lane; i < rows; i += 64) {
+      const double m = (double)col[i];
+      if (sumsq)
+        acc += m * m;
+      else
+        acc += (abs_mode ? fabs(m) : m) * v[i];
+    }
   }
   acc = wave_sum(acc);
   if (lane == 0) out[j] = accumulate ? out[j] + acc : acc;

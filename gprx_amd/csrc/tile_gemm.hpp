@@ -57,34 +57,36 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Tile-kernel configuration for a TT x TT tile: K step and LDS row stride
-// LS (elements).  A k-contiguous operand is stored transposed, a thread's
-// vector of k = k0..k0+VEC-1 going to rows k0.. of one column, while the
-// lanes of one store instruction differ in k0 by multiples of VEC: LS is
-// chosen so those rows start on different banks (4 * LS = 16 mod 64 dwords
-// for f32 128-tiles, with k0 in {0, 4, 8, 12}), keeping rows 16-byte
-// aligned for the vector stores of the other layout; fragment reads (16
-// consecutive elements per 16-lane group) stay conflict-free.  BK keeps
-// two 16-byte global loads per thread per operand and slice.
+// Tile-kernel configuration for a TT x TT tile: K step BK and the LDS row
+// stride LS (elements) of a row-contiguous operand, staged as BK rows of TT.
+// A k-contiguous operand is staged as TT rows of BK (see kc_offset), so
+// both operands are written to LDS with 16-byte stores.  MFMA step s of a
+// slice gives lane group g = lane / 16 the k = (4 r + g) VEC + c, s = r VEC
+// + c, so one 16-byte read of a k-contiguous row feeds VEC steps; the four
+// groups then read row-contiguous rows VEC apart, and LS puts those rows on
+// disjoint banks (4 LS = 16 mod 64 dwords for f32, 4 LS = 32 mod 64 for
+// f64) with rows 16-byte aligned.  BK keeps two 16-byte global loads per
+// thread per operand and slice.
 template <typename T, int TT = 128>
 struct Cfg;
 // BK 8 (f64) / 16 (f32) with two workgroups per CU measured fastest on
-// gfx950 among BK in {8, 16, 32} and 1-2 workgroups per CU (f32 BK 32, one
-// workgroup per CU by its LDS: X^T X at n = 16384 14.5 ms against 12.0).
+// gfx950 among BK in {8, 16, 32}: f32 BK 32 (two workgroups per CU, two
+// slices prefetched) ties BK 16 (X^T X at n = 16384 11.45 ms against
+// 11.52); f64 BK 16 needs more than 256 registers.
 template <>
 struct Cfg<double, 128> {
   static constexpr int BK = 8;
-  static constexpr int LS = 132;
+  static constexpr int LS = 136;
 };
 template <>
 struct Cfg<float, 128> {
   static constexpr int BK = 16;
-  static constexpr int LS = 164;
+  static constexpr int LS = 132;
 };
 template <>
 struct Cfg<double, 64> {
   static constexpr int BK = 16;
-  static constexpr int LS = 66;
+  static constexpr int LS = 72;
 };
 template <>
 struct Cfg<float, 64> {
@@ -96,13 +98,25 @@ struct Cfg<float, 64> {
 template <>
 struct Cfg<double, 32> {
   static constexpr int BK = 16;
-  static constexpr int LS = 34;
+  static constexpr int LS = 40;
 };
 template <>
 struct Cfg<float, 32> {
   static constexpr int BK = 32;
   static constexpr int LS = 36;
 };
+
+// Element offset of 16-byte vector kv (elements kv VEC ..) of row m in a
+// k-contiguous slice of rows of BK elements.  KV = BK / VEC vectors make a
+// row of 4 KV dwords, so RP = 16 / KV rows share one sweep of the 64 banks;
+// XOR-ing kv with (m / RP) mod KV gives the 16 rows of a fragment read (same
+// kv) and the RP rows of a store (all kv) distinct bank quads.
+template <typename T, int BK>
+__device__ __forceinline__ int kc_offset(int m, int kv) {
+  constexpr int VEC = 16 / (int)sizeof(T), KV = BK / VEC, RP = 16 / KV;
+  static_assert(KV >= 1 && KV <= 16 && 16 % KV == 0, "k-contiguous row of 1..16 vectors");
+  return m * BK + ((kv ^ ((m / RP) % KV)) * VEC);
+}
 
 // Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.
 __device__ __forceinline__ void tri_index(int bid, int& ti, int& tj) {
@@ -127,8 +141,12 @@ __device__ __forceinline__ int xcd_remap(int bid, int total) {
 
 template <typename T, int TT = 128>
 struct TileSmem {
-  __attribute__((aligned(16))) T a[2][Cfg<T, TT>::BK * Cfg<T, TT>::LS];
-  __attribute__((aligned(16))) T b[2][Cfg<T, TT>::BK * Cfg<T, TT>::LS];
+  // one operand slice: BK x LS (row-contiguous) or TT x BK (k-contiguous)
+  static constexpr int kElems = Cfg<T, TT>::BK * Cfg<T, TT>::LS > TT * Cfg<T, TT>::BK
+                                    ? Cfg<T, TT>::BK * Cfg<T, TT>::LS
+                                    : TT * Cfg<T, TT>::BK;
+  __attribute__((aligned(16))) T a[2][kElems];
+  __attribute__((aligned(16))) T b[2][kElems];
 };
 
 // Global-load stages in flight per thread: a slice leaves HBM P
@@ -164,6 +182,8 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   constexpr int FB = TT / 32, WT = TT / 2;
   using V = typename Vec<T>::V;
   constexpr int NV = TT * BK / VEC / 256;  // 16-byte vectors per thread per operand
+  constexpr int KV = BK / VEC;              // 16-byte vectors per k-contiguous row
+  static_assert(BK % (4 * VEC) == 0, "a slice feeds whole 16-byte k-vectors to each lane group");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
@@ -176,14 +196,14 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
         const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
         xa[v] = *reinterpret_cast<const V*>(Ablk + mn + (long)(k0 + k) * lda);
       } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
+        const int mn = e / KV, k = (e % KV) * VEC;
         xa[v] = *reinterpret_cast<const V*>(Ablk + (k0 + k) + (long)mn * lda);
       }
       if (TB) {
         const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
         xb[v] = *reinterpret_cast<const V*>(Bblk + mn + (long)(k0 + k) * ldb);
       } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
+        const int mn = e / KV, k = (e % KV) * VEC;
         xb[v] = *reinterpret_cast<const V*>(Bblk + (k0 + k) + (long)mn * ldb);
       }
     }
@@ -198,35 +218,59 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
         const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
         *reinterpret_cast<V*>(a + k * LS + mn) = xa[v];
       } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) a[(k + c) * LS + mn] = xa[v][c];
+        *reinterpret_cast<V*>(a + kc_offset<T, BK>(e / KV, e % KV)) = xa[v];
       }
       if (TB) {
         const int k = e / (TT / VEC), mn = (e % (TT / VEC)) * VEC;
         *reinterpret_cast<V*>(b + k * LS + mn) = xb[v];
       } else {
-        const int mn = e / (BK / VEC), k = (e % (BK / VEC)) * VEC;
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) b[(k + c) * LS + mn] = xb[v][c];
+        *reinterpret_cast<V*>(b + kc_offset<T, BK>(e / KV, e % KV)) = xb[v];
       }
     }
   };
+  // Fragment rows start at multiples of 16, so a lane's swizzle depends on
+  // lane & 15 alone (kc_offset with m = row base + (lane & 15)).
   auto compute = [&](int buf) {
     const T* a = sm.a[buf];
     const T* b = sm.b[buf];
+    const int g = lane >> 4, l16 = lane & 15;
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T af[FB], bf[FB];
+    for (int r = 0; r < BK / (4 * VEC); ++r) {
+      const int kv = 4 * r + g;
+      V xa[FB], xb[FB];
+      if constexpr (TA) {
 #pragma unroll
-      for (int i = 0; i < FB; ++i) af[i] = a[kr * LS + wm * WT + i * 16 + (lane & 15)];
+        for (int i = 0; i < FB; ++i)
+          xa[i] = *reinterpret_cast<const V*>(a + kc_offset<T, BK>(wm * WT + i * 16 + l16, kv));
+      }
+      if constexpr (!TB) {
 #pragma unroll
-      for (int j = 0; j < FB; ++j) bf[j] = b[kr * LS + wn * WT + j * 16 + (lane & 15)];
+        for (int j = 0; j < FB; ++j)
+          xb[j] = *reinterpret_cast<const V*>(b + kc_offset<T, BK>(wn * WT + j * 16 + l16, kv));
+      }
 #pragma unroll
-      for (int i = 0; i < FB; ++i)
+      for (int c = 0; c < VEC; ++c) {
+        const int kr = kv * VEC + c;
+        T af[FB], bf[FB];
 #pragma unroll
-        for (int j = 0; j < FB; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
+        for (int i = 0; i < FB; ++i) {
+          if constexpr (TA)
+            af[i] = xa[i][c];
+          else
+            af[i] = a[kr * LS + wm * WT + i * 16 + l16];
+        }
+#pragma unroll
+        for (int j = 0; j < FB; ++j) {
+          if constexpr (!TB)
+            bf[j] = xb[j][c];
+          else
+            bf[j] = b[kr * LS + wn * WT + j * 16 + l16];
+        }
+#pragma unroll
+        for (int i = 0; i < FB; ++i)
+#pragma unroll
+          for (int j = 0; j < FB; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
+      }
     }
   };
   if (nk <= 0) return;

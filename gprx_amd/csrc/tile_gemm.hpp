@@ -2,8 +2,11 @@
 // the lane layouts).  A 256-thread workgroup computes one TT x TT tile
 // (TT = 128, or 64 / 32 for launches too small to fill the chip) of op(A) op(B)
 // with four waves, staging BK-deep slices of both operands through
-// double-buffered LDS fed by a register prefetch Prefetch<T>::P slices deep.
+// double-buffered LDS fed by a register prefetch Prefetch<T>::P slices deep,
+// and reading each slice's MFMA fragments one slice ahead.
 #pragma once
+
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -118,6 +121,19 @@ __device__ __forceinline__ int kc_offset(int m, int kv) {
   return m * BK + ((kv ^ ((m / RP) % KV)) * VEC);
 }
 
+// f(std::integral_constant<int, 0>) .. f(integral_constant<N - 1>), unrolled.
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for_from(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for_from<I + 1, N>(f);
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_from<0, N>(f);
+}
+
 // Lower-triangle enumeration by rows: bid = ti (ti + 1) / 2 + tj, ti >= tj.
 __device__ __forceinline__ void tri_index(int bid, int& ti, int& tj) {
   int t = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
@@ -149,15 +165,16 @@ struct TileSmem {
   __attribute__((aligned(16))) T b[2][kElems];
 };
 
-// Global-load stages in flight per thread: a slice leaves HBM P
+// Global-load stages in flight per thread: a slice leaves HBM P + 2
 // iterations before its MFMAs run, which hides the load latency of a launch
 // too small to have a second workgroup per CU to switch to.  Measured on
-// gfx950 (n = 16384 X^T X): f32 11.84 ms at 3 slices, 11.85 at 4, with 4
-// faster in the factor (34.4 against 35.1 ms); f64 would need 4 slices'
-// registers beyond its 256 and spill, so it keeps 3.
+// gfx950 at n = 16384: f32 X^T X 11.84 ms at 3 slices, 11.85 at 4, with 4
+// faster in the factor (34.4 against 35.1 ms); f64 factor 57.8 ms at 2
+// slices against 58.9 at 3 (3 slices and the second fragment set together
+// need more than 256 registers and spill).
 template <typename T>
 struct Prefetch {
-  static constexpr int P = 3;
+  static constexpr int P = 2;
 };
 template <>
 struct Prefetch<float> {
@@ -168,8 +185,10 @@ struct Prefetch<float> {
 // the tile origins: A "row-contiguous" operand has element (mn, k) at
 // P[mn + k * ld], a "k-contiguous" one at P[k + mn * ld].  Four waves in a
 // 2 x 2 grid each own a (TT/2)^2 quarter as FB x FB 16 x 16 MFMA blocks.
-// Slice i sits in register slot i % P until it is copied to LDS buffer
-// i % 2; the iteration loop is unrolled by P so every slot index is static.
+// The fragments of slice i + 1 are read from LDS while slice i's MFMAs run
+// (two fragment register sets), one barrier per slice; against reading a
+// slice's fragments just before its MFMAs this took f64 X^T X at n = 16384
+// from 23.0 to 22.35 ms and left f32 unchanged.
 template <typename T, bool TA, bool TB, int TT = 128>
 __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long lda,
                                               const T* __restrict__ Bblk, long ldb, int kb, int nk,
@@ -184,6 +203,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   constexpr int NV = TT * BK / VEC / 256;  // 16-byte vectors per thread per operand
   constexpr int KV = BK / VEC;              // 16-byte vectors per k-contiguous row
   static_assert(BK % (4 * VEC) == 0, "a slice feeds whole 16-byte k-vectors to each lane group");
+  static_assert(P >= 2, "the prologue fills two register slots");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
@@ -228,83 +248,86 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
       }
     }
   };
-  // Fragment rows start at multiples of 16, so a lane's swizzle depends on
-  // lane & 15 alone (kc_offset with m = row base + (lane & 15)).
-  auto compute = [&](int buf) {
+  if (nk <= 0) return;
+  // fragments of one slice for this lane: step s = r VEC + c of the slice
+  constexpr int NS = BK / 4;
+  struct Frag {
+    T a[NS][FB], b[NS][FB];
+  };
+  const int g = lane >> 4, l16 = lane & 15;
+  auto fread = [&](int buf, Frag& f) {
     const T* a = sm.a[buf];
     const T* b = sm.b[buf];
-    const int g = lane >> 4, l16 = lane & 15;
 #pragma unroll
     for (int r = 0; r < BK / (4 * VEC); ++r) {
       const int kv = 4 * r + g;
-      V xa[FB], xb[FB];
-      if constexpr (TA) {
 #pragma unroll
-        for (int i = 0; i < FB; ++i)
-          xa[i] = *reinterpret_cast<const V*>(a + kc_offset<T, BK>(wm * WT + i * 16 + l16, kv));
-      }
-      if constexpr (!TB) {
+      for (int i = 0; i < FB; ++i) {
+        if constexpr (TA) {
+          const V x = *reinterpret_cast<const V*>(a + kc_offset<T, BK>(wm * WT + i * 16 + l16, kv));
 #pragma unroll
-        for (int j = 0; j < FB; ++j)
-          xb[j] = *reinterpret_cast<const V*>(b + kc_offset<T, BK>(wn * WT + j * 16 + l16, kv));
-      }
+          for (int c = 0; c < VEC; ++c) f.a[r * VEC + c][i] = x[c];
+        } else {
 #pragma unroll
-      for (int c = 0; c < VEC; ++c) {
-        const int kr = kv * VEC + c;
-        T af[FB], bf[FB];
-#pragma unroll
-        for (int i = 0; i < FB; ++i) {
-          if constexpr (TA)
-            af[i] = xa[i][c];
-          else
-            af[i] = a[kr * LS + wm * WT + i * 16 + l16];
+          for (int c = 0; c < VEC; ++c)
+            f.a[r * VEC + c][i] = a[(kv * VEC + c) * LS + wm * WT + i * 16 + l16];
         }
+      }
 #pragma unroll
-        for (int j = 0; j < FB; ++j) {
-          if constexpr (!TB)
-            bf[j] = xb[j][c];
-          else
-            bf[j] = b[kr * LS + wn * WT + j * 16 + l16];
+      for (int j = 0; j < FB; ++j) {
+        if constexpr (!TB) {
+          const V x = *reinterpret_cast<const V*>(b + kc_offset<T, BK>(wn * WT + j * 16 + l16, kv));
+#pragma unroll
+          for (int c = 0; c < VEC; ++c) f.b[r * VEC + c][j] = x[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < VEC; ++c)
+            f.b[r * VEC + c][j] = b[(kv * VEC + c) * LS + wn * WT + j * 16 + l16];
         }
-#pragma unroll
-        for (int i = 0; i < FB; ++i)
-#pragma unroll
-          for (int j = 0; j < FB; ++j) acc[i][j] = mfma16(bf[j], af[i], acc[i][j]);
       }
     }
   };
-  if (nk <= 0) return;
-  // Loads are never conditional: a slice past the end is replaced by the
-  // last one (read again, never used), and the steady-state loop carries no
-  // branch, so the compiler's wait counters track each slot's loads and the
-  // refill of a slot does not wait for the slices still in flight.
+  auto mma = [&](const Frag& f) {
+#pragma unroll
+    for (int st = 0; st < NS; ++st)
+#pragma unroll
+      for (int i = 0; i < FB; ++i)
+#pragma unroll
+        for (int j = 0; j < FB; ++j) acc[i][j] = mfma16(f.b[st][j], f.a[st][i], acc[i][j]);
+  };
+  // Slice i: loaded into register slot i % P at iteration i - 2 - P, stored
+  // to LDS buffer i % 2 at iteration i - 2, read into fragment set i % 2 at
+  // iteration i - 1, multiplied at iteration i.  A buffer is overwritten
+  // one barrier after its fragments were read, a slot refilled right after
+  // its store.  Loads past the end repeat the last slice and are not used.
+  constexpr int U = (P % 2 == 0) ? P : 2 * P;
 #pragma unroll
   for (int u = 0; u < P; ++u) gload(kb + min(u, nk - 1) * BK, ra[u], rb[u]);
   lstore(0, ra[0], rb[0]);
+  gload(kb + min(P, nk - 1) * BK, ra[0], rb[0]);
   __syncthreads();
+  Frag fr[2];
+  fread(0, fr[0]);
+  lstore(1, ra[1], rb[1]);
+  gload(kb + min(P + 1, nk - 1) * BK, ra[1], rb[1]);
+  // publishes slice 1, and every wave has read slice 0 before iteration 0
+  // overwrites buffer 0
+  __syncthreads();
+  auto body = [&](int i, auto uc) {
+    constexpr int u = decltype(uc)::value;
+    fread((u + 1) & 1, fr[(u + 1) & 1]);
+    lstore(u & 1, ra[(u + 2) % P], rb[(u + 2) % P]);
+    gload(kb + min(i + 2 + P, nk - 1) * BK, ra[(u + 2) % P], rb[(u + 2) % P]);
+    mma(fr[u & 1]);
+    __syncthreads();
+  };
   int it = 0;
-  for (; it + P <= nk; it += P) {
-#pragma unroll
-    for (int u = 0; u < P; ++u) {
-      const int i = it + u;
-      // slot u (slice i) is in LDS already: refill it with slice i + P; the
-      // last slice's store below goes to the idle buffer and is not read
-      gload(kb + min(i + P, nk - 1) * BK, ra[u], rb[u]);
-      compute(i & 1);
-      lstore((i + 1) & 1, ra[(u + 1) % P], rb[(u + 1) % P]);
-      __syncthreads();
-    }
+  for (; it + U <= nk; it += U) {
+    static_for<U>([&](auto uc) { body(it + decltype(uc)::value, uc); });
   }
-  // the last nk mod P slices: slot u holds slice it + u
-#pragma unroll
-  for (int u = 0; u < P - 1; ++u) {
-    const int i = it + u;
-    if (i < nk) {
-      compute(i & 1);
-      if (i + 1 < nk) lstore((i + 1) & 1, ra[(u + 1) % P], rb[(u + 1) % P]);
-      __syncthreads();
-    }
-  }
+  static_for<U - 1>([&](auto uc) {
+    if (it + decltype(uc)::value < nk) body(it + decltype(uc)::value, uc);
+  });
 }
 
 // Tile of a launch: (ti, tj) and the K-range its triangular operands allow.

@@ -6,7 +6,7 @@
 // j tiles runs in the same three launches,
 //   1. column update A_ij -= L_i,0:j L_j,0:j^T, i >= j (one tile task each,
 //      reduction length 128 j)
-//   2. diagonal block j: Cholesky + inverse in LDS (f64) -> X_jj
+//   2. diagonal block j: Cholesky + inverse in LDS (storage arithmetic) -> X_jj
 //   3. panel L_ij = A_ij X_jj^T (i > j) and Y_jk = X_jj L_jk (k < j)
 // then the inverse by distance, X_ij = -sum_k Y_ik X_kj (reduction 128 (i-j)),
 // A^-1 = X^T X, alpha and the fused gradient contraction, each as one

@@ -256,7 +256,9 @@ template void gemm<double>(bool, bool, const GemmArgs<double>&, hipStream_t);
 template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// 128 x 128 diagonal block: Cholesky + inverse in LDS, f64 arithmetic.
+// 128 x 128 diagonal block: Cholesky + inverse in LDS, in the arithmetic of
+// the storage type C (f64 for the FP64 factor; f32 for the f32 factor of
+// the mixed form, like every other step of that factor).
 //
 // Layout in LDS (column stride DS): L lower (r >= c) at s[c * DS + r]; the
 // inverse X = L^-1 stored transposed above the diagonal, X[r][c] (r > c) at
@@ -272,14 +274,28 @@ template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 // form a serial chain; everything on it runs on one wave with the pivot
 // and column broadcasts in registers.
 // ---------------------------------------------------------------------------
-constexpr int DS = 129;
+// LDS column stride: odd, so a row read across lanes (stride DS) meets
+// distinct banks; for f32, DS = 17 mod 64 also spreads the four lane groups
+// of an MFMA operand read (rows k .. k + 3) over the banks.
+template <typename C>
+struct DiagCfg;
+template <>
+struct DiagCfg<double> {
+  static constexpr int DS = 129;
+};
+template <>
+struct DiagCfg<float> {
+  static constexpr int DS = 145;
+};
 
 // X[r][c] from the block's LDS layout (xd[kTile] holds 0.0).  The address
 // is selected and one load issued: a selected value would be turned back
 // into exec-mask branches around the loads, which keeps the compiler from
 // issuing the operands of several MFMAs ahead of their use.
-__device__ __forceinline__ double x_at(const double* s, const double* xd, int r, int c) {
-  const double* p = r > c ? s + r * DS + c : (r == c ? xd + r : xd + kTile);
+template <typename C>
+__device__ __forceinline__ C x_at(const C* s, const C* xd, int r, int c) {
+  constexpr int DS = DiagCfg<C>::DS;
+  const C* p = r > c ? s + r * DS + c : (r == c ? xd + r : xd + kTile);
   return *p;
 }
 
@@ -291,6 +307,9 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 
 __device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
   u = 0;
@@ -298,8 +317,14 @@ __device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
   v = t - u * (u + 1) / 2;
 }
 
-// p^-1/2 from the hardware estimate and two Newton steps
-// y <- y + y/2 (1 - p y^2), each squaring the relative error.
+// p^-1/2 from the hardware estimate and Newton steps y <- y + y/2 (1 - p y^2),
+// each squaring the relative error: two for f64 (from an estimate good to
+// about f32 precision), one for f32.
+__device__ __forceinline__ float rsqrt_nr(float p) {
+  const float y = __builtin_amdgcn_rsqf(p);
+  const float e = fmaf(-p * y, y, 1.0f);
+  return fmaf(0.5f * y, e, y);
+}
 __device__ __forceinline__ double rsqrt_nr(double p) {
   double y = __builtin_amdgcn_rsq(p);
 #pragma unroll
@@ -320,14 +345,16 @@ __device__ __forceinline__ double rsqrt_nr(double p) {
 // w * pairs / 4: every wave then has the same reduction lengths in both
 // products (sum over uj of HB - uj, and over ui of ui + 1), and its HB
 // accumulator chains advance together in one K loop.
-template <int HB, int W>
-__device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) {
+template <typename C, int HB, int W>
+__device__ __forceinline__ void inverse_level_wave(C* s, const C* xd) {
+  constexpr int DS = DiagCfg<C>::DS;
+  using AV = typename Acc<C>::V;
   constexpr int kPairs = 8 / (2 * HB);
   constexpr int o = (W * kPairs / 4) * 2 * HB;
   const int lane = threadIdx.x & 63;
-  d4 acc[HB];
+  AV acc[HB];
 #pragma unroll
-  for (int i = 0; i < HB; ++i) acc[i] = d4{};
+  for (int i = 0; i < HB; ++i) acc[i] = AV{};
   // operands of every chain for one k step are loaded before its MFMAs, so
   // one LDS round trip covers up to HB products; every bound is static
 #pragma unroll
@@ -335,7 +362,7 @@ __device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) 
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 4) {
       const int k = (o + kb) * 16 + kk + (lane >> 4);
-      double a[HB], b[HB];
+      C a[HB], b[HB];
 #pragma unroll
       for (int i = 0; i < HB; ++i) {
         const int bi = o + HB + i, bj = o + (W + i) % HB;
@@ -354,17 +381,17 @@ __device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) 
     const int bi = o + HB + i, bj = o + (W + i) % HB;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      s[(bj * 16 + (lane & 15)) * DS + bi * 16 + drow<double>(lane, q)] = acc[i][q];
+      s[(bj * 16 + (lane & 15)) * DS + bi * 16 + drow<C>(lane, q)] = acc[i][q];
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < HB; ++i) acc[i] = d4{};
+  for (int i = 0; i < HB; ++i) acc[i] = AV{};
 #pragma unroll
   for (int kb = 0; kb < HB; ++kb)
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 4) {
       const int k = (o + HB + kb) * 16 + kk + (lane >> 4);
-      double a[HB], b[HB];
+      C a[HB], b[HB];
 #pragma unroll
       for (int i = kb; i < HB; ++i) {  // X22 is lower: k <= row block
         const int bi = o + HB + i, bj = o + (W + i) % HB;
@@ -379,7 +406,7 @@ __device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) 
     const int bi = o + HB + i, bj = o + (W + i) % HB;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      s[(bi * 16 + drow<double>(lane, q)) * DS + bj * 16 + (lane & 15)] = -acc[i][q];
+      s[(bi * 16 + drow<C>(lane, q)) * DS + bj * 16 + (lane & 15)] = -acc[i][q];
   }
   __syncthreads();
 }
@@ -387,20 +414,22 @@ __device__ __forceinline__ void inverse_level_wave(double* s, const double* xd) 
 // Dispatch on the (scalar) wave index, so each wave's reduction bounds are
 // compile-time constants: data-dependent guards on MFMA accumulators make
 // the compiler copy every accumulator at each merge.
-template <int HB>
-__device__ __forceinline__ void inverse_level(double* s, const double* xd) {
+template <typename C, int HB>
+__device__ __forceinline__ void inverse_level(C* s, const C* xd) {
   switch (wave_index()) {
-    case 0: inverse_level_wave<HB, 0>(s, xd); break;
-    case 1: inverse_level_wave<HB, 1>(s, xd); break;
-    case 2: inverse_level_wave<HB, 2>(s, xd); break;
-    default: inverse_level_wave<HB, 3>(s, xd); break;
+    case 0: inverse_level_wave<C, HB, 0>(s, xd); break;
+    case 1: inverse_level_wave<C, HB, 1>(s, xd); break;
+    case 2: inverse_level_wave<C, HB, 2>(s, xd); break;
+    default: inverse_level_wave<C, HB, 3>(s, xd); break;
   }
 }
 
 // Stage (a) of the diagonal block, run by one wave: factors the 16 x 16
 // block at (c0, c0) of s in registers and writes L (lower), X = L^-1
 // (transposed storage above the diagonal, diagonal in xd) and log L_ii.
-__device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag, int* sbad, int c0) {
+template <typename C>
+__device__ __forceinline__ void factor16(C* s, C* xd, double* logdiag, int* sbad, int c0) {
+  constexpr int DS = DiagCfg<C>::DS;
   const int lane = threadIdx.x & 63;
   // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k].  Entries
   // right of the diagonal (k > r) start at zero and collect values no lower
@@ -408,19 +437,20 @@ __device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag,
   // p = A[j][j], rs = p^-1/2; column j of L is d[j] * rs (lane j: p rs =
   // sqrt(p)); rank-1 update of the columns right of j.
   const int r = lane & 15;
-  double d[16];
+  C d[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) d[k] = s[(c0 + k) * DS + c0 + r];
   bool bad = false;
-  double lrr = 1.0, xrr = 1.0;
+  C lrr = 1, xrr = 1;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    double p = lane_bcast(d[j], j);
-    if (!(p > 0.0) || !isfinite(p)) {
+    C p = lane_bcast(d[j], j);
+    // f32: a subnormal pivot fails too (its estimate may be flushed)
+    if (!(p > 0) || !isfinite(p) || (sizeof(C) == 4 && p < (C)1.17549435e-38)) {
       bad = true;
-      p = 1.0;
+      p = 1;
     }
-    const double rs = rsqrt_nr(p);
+    const C rs = rsqrt_nr(p);
     d[j] *= rs;
     // column j + 1 first: it holds the next pivot
 #pragma unroll
@@ -438,7 +468,7 @@ __device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag,
     for (int k = 0; k < 16; ++k)
       if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
     xd[c0 + r] = xrr;
-    logdiag[c0 + r] = log(lrr);
+    logdiag[c0 + r] = log((double)lrr);
   }
 }
 
@@ -446,11 +476,13 @@ __device__ __forceinline__ void factor16(double* s, double* xd, double* logdiag,
 // L_i = A_i L_D^-T, one row per thread, by forward substitution with the
 // columns of L_D read as broadcasts (every lane the same address):
 //   x_j = a_j / L[j][j],  a_c -= x_j L[c][j]  (c > j).
-__device__ __forceinline__ void panel_rows(double* s, const double* xd, int c0, int rows) {
+template <typename C>
+__device__ __forceinline__ void panel_rows(C* s, const C* xd, int c0, int rows) {
+  constexpr int DS = DiagCfg<C>::DS;
   const int t = threadIdx.x;
   if (t >= rows) return;
   const int R = c0 + 16 + t;
-  double a[16];
+  C a[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) a[k] = s[(c0 + k) * DS + R];
 #pragma unroll
@@ -467,13 +499,15 @@ __device__ __forceinline__ void panel_rows(double* s, const double* xd, int c0, 
 // first 128 solves L_b x = e_c by forward substitution (x_k = 0 for k < c
 // without a branch) and stores X[k][c] (k > c) transposed above the
 // diagonal; X[c][c] is in xd already.
-__device__ __forceinline__ void invert_diag16(double* s, const double* xd) {
+template <typename C>
+__device__ __forceinline__ void invert_diag16(C* s, const C* xd) {
+  constexpr int DS = DiagCfg<C>::DS;
   const int t = threadIdx.x;
   if (t >= kTile) return;
   const int o = (t >> 4) * 16, c = t & 15;
-  double b[16];
+  C b[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) b[k] = k == c ? 1.0 : 0.0;
+  for (int k = 0; k < 16; ++k) b[k] = k == c ? 1 : 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     b[k] *= xd[o + k];
@@ -487,29 +521,32 @@ __device__ __forceinline__ void invert_diag16(double* s, const double* xd) {
 
 // A_(bi,bj) -= L_bi L_bj^T (panel columns c0 .. c0+15) for one block, or two
 // when `two` (independent chains interleaved); diagonal blocks keep r >= c.
-__device__ __forceinline__ void update_blocks(double* s, int c0, int bi, int bj, bool two, int bi2,
+template <typename C>
+__device__ __forceinline__ void update_blocks(C* s, int c0, int bi, int bj, bool two, int bi2,
                                               int bj2) {
+  constexpr int DS = DiagCfg<C>::DS;
+  using AV = typename Acc<C>::V;
   const int lane = threadIdx.x & 63;
-  d4 acc = d4{}, acc2 = d4{};
+  AV acc = AV{}, acc2 = AV{};
 #pragma unroll
   for (int kk = 0; kk < 16; kk += 4) {
     const int kr = kk + (lane >> 4);
-    const double a = s[(c0 + kr) * DS + bi * 16 + (lane & 15)];
-    const double b = s[(c0 + kr) * DS + bj * 16 + (lane & 15)];
+    const C a = s[(c0 + kr) * DS + bi * 16 + (lane & 15)];
+    const C b = s[(c0 + kr) * DS + bj * 16 + (lane & 15)];
     acc = mfma16(a, b, acc);
     if (two) {
-      const double a2 = s[(c0 + kr) * DS + bi2 * 16 + (lane & 15)];
-      const double b2 = s[(c0 + kr) * DS + bj2 * 16 + (lane & 15)];
+      const C a2 = s[(c0 + kr) * DS + bi2 * 16 + (lane & 15)];
+      const C b2 = s[(c0 + kr) * DS + bj2 * 16 + (lane & 15)];
       acc2 = mfma16(a2, b2, acc2);
     }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int row = bi * 16 + drow<double>(lane, q);
+    const int row = bi * 16 + drow<C>(lane, q);
     const int col = bj * 16 + (lane & 15);
     if (bi != bj || row >= col) s[col * DS + row] -= acc[q];
     if (two) {
-      const int row2 = bi2 * 16 + drow<double>(lane, q);
+      const int row2 = bi2 * 16 + drow<C>(lane, q);
       const int col2 = bj2 * 16 + (lane & 15);
       if (bi2 != bj2 || row2 >= col2) s[col2 * DS + row2] -= acc2[q];
     }
@@ -518,29 +555,30 @@ __device__ __forceinline__ void update_blocks(double* s, int c0, int bi, int bj,
 
 template <typename T>
 __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
-                                double* __restrict__ logdiag, int* info, double* s, double* xd,
+                                double* __restrict__ logdiag, int* info, T* s, T* xd,
                                 int* sbad) {
+  constexpr int DS = DiagCfg<T>::DS;
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   if (tid == 0) {
     *sbad = 0;
-    xd[kTile] = 0.0;  // the zero x_at reads above the diagonal
+    xd[kTile] = 0;  // the zero x_at reads above the diagonal
   }
   // 16 loads in flight per thread: the whole block lies inside the matrix,
   // so the upper part is read and dropped rather than branched around.
   constexpr int kBatch = 16;
 #pragma unroll
   for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
-    double v[kBatch];
+    T v[kBatch];
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
-      v[i] = (double)A[(e & 127) + (long)(e >> 7) * lda];
+      v[i] = A[(e & 127) + (long)(e >> 7) * lda];
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
       const int r = e & 127, c = e >> 7;
-      s[c * DS + r] = (r >= c) ? v[i] : 0.0;
+      s[c * DS + r] = (r >= c) ? v[i] : (T)0;
     }
   }
   __syncthreads();
@@ -583,21 +621,21 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
   // HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
   invert_diag16(s, xd);
   __syncthreads();
-  inverse_level<1>(s, xd);
-  inverse_level<2>(s, xd);
-  inverse_level<4>(s, xd);
+  inverse_level<T, 1>(s, xd);
+  inverse_level<T, 2>(s, xd);
+  inverse_level<T, 4>(s, xd);
 #pragma unroll
   for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
-    double v[kBatch];
+    T v[kBatch];
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
-      v[i] = x_at(s, xd, e & 127, e >> 7);
+      v[i] = x_at<T>(s, xd, e & 127, e >> 7);
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
-      X[(e & 127) + (long)(e >> 7) * ldx] = (T)v[i];
+      X[(e & 127) + (long)(e >> 7) * ldx] = v[i];
     }
   }
   if (tid == 0 && *sbad) atomicExch(info, 1);
@@ -608,8 +646,8 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const T* A, long lda, T* X, 
                                                     double* logdiag, int* info, long sa, long sx,
                                                     long slog) {
   extern __shared__ double smem[];
-  double* s = smem;
-  double* xd = smem + kTile * DS;
+  T* s = reinterpret_cast<T*>(smem);
+  T* xd = s + kTile * DiagCfg<T>::DS;
   int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const long z = blockIdx.x;
   potrf_block_lds<T>(A + z * sa, lda, X + z * sx, ldx, logdiag + z * slog, info, s, xd, sbad);
@@ -621,24 +659,26 @@ __global__ __launch_bounds__(256) void k_potrf_diag_list(T* A, T* X, const long*
                                                          const long* log_off, double* logdiag,
                                                          int* info, const int* info_idx) {
   extern __shared__ double smem[];
-  double* s = smem;
-  double* xd = smem + kTile * DS;
+  T* s = reinterpret_cast<T*>(smem);
+  T* xd = s + kTile * DiagCfg<T>::DS;
   int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const int b = blockIdx.x;
   potrf_block_lds<T>(A + a_off[b], ld[b], X + x_off[b], ld[b], logdiag + log_off[b],
                      info + info_idx[b], s, xd, sbad);
 }
 
-constexpr size_t kDiagLds = (size_t)kTile * DS * sizeof(double) + (kTile + 1) * sizeof(double) + 16;
+template <typename T>
+constexpr size_t kDiagLds =
+    (size_t)kTile * DiagCfg<T>::DS * sizeof(T) + (kTile + 1) * sizeof(T) + 16;
 
 template <typename T>
 static void set_diag_lds_once() {
   static bool done = false;
   if (!done) {
     GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds));
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
     GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag_list<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds));
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
     done = true;
   }
 }
@@ -652,7 +692,7 @@ template <typename T>
 void potrf_diag(const T* A, long lda, T* X, long ldx, double* logdiag, int* info, int batch,
                 long stride_a, long stride_x, long stride_log, hipStream_t stream) {
   set_diag_lds_once<T>();
-  hipLaunchKernelGGL((k_potrf_diag<T>), dim3(batch), dim3(256), kDiagLds, stream, A, lda, X, ldx,
+  hipLaunchKernelGGL((k_potrf_diag<T>), dim3(batch), dim3(256), kDiagLds<T>, stream, A, lda, X, ldx,
                      logdiag, info, stride_a, stride_x, stride_log);
   GPRX_HIP_CHECK(hipGetLastError());
 }
@@ -663,7 +703,7 @@ void potrf_diag_list(T* A, T* X, const long* a_off, const long* x_off, const lon
                      hipStream_t stream) {
   if (count <= 0) return;
   set_diag_lds_once<T>();
-  hipLaunchKernelGGL((k_potrf_diag_list<T>), dim3(count), dim3(256), kDiagLds, stream, A, X, a_off,
+  hipLaunchKernelGGL((k_potrf_diag_list<T>), dim3(count), dim3(256), kDiagLds<T>, stream, A, X, a_off,
                      x_off, ld, log_off, logdiag, info, info_idx);
   GPRX_HIP_CHECK(hipGetLastError());
 }

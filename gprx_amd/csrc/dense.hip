@@ -51,7 +51,13 @@ __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
     for (int j = 0; j < FB; ++j) acc[i][j] = AV{};
   const T* Ablk = TA ? A + (long)ti * TT * g.lda : A + (long)ti * TT;
   const T* Bblk = TB ? B + (long)tj * TT : B + (long)tj * TT * g.ldb;
-  tile_mainloop<T, TA, TB, TT>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc);
+  // K-ranges that start at the tile's row or column but share their end:
+  // run them from the end down, so the tiles an XCD starts together read
+  // the same operand rows at the same time and share them in its L2 (X^T X
+  // at n = 16384: HBM-side reads -18% in f64, -22% in f32; f32 12.18 to
+  // 11.77 ms).
+  const bool krev = (g.krange & (kKloRow | kKloCol)) && !(g.krange & (kKhiRow | kKhiCol));
+  tile_mainloop<T, TA, TB, TT>(Ablk, g.lda, Bblk, g.ldb, tc.kb, nk, sm, acc, krev);
 
   // Epilogue.  With beta != 0 the old values of a row of FB blocks are read
   // before any of them is stored (the tile lies inside C, so entries above

@@ -188,12 +188,14 @@ struct Prefetch<float> {
 // The fragments of slice i + 1 are read from LDS while slice i's MFMAs run
 // (two fragment register sets), one barrier per slice; against reading a
 // slice's fragments just before its MFMAs this took f64 X^T X at n = 16384
-// from 23.0 to 22.35 ms and left f32 unchanged.
+// from 23.0 to 22.35 ms and left f32 unchanged.  With krev the slices run
+// from the end of the K-range down (the sum's order is then reversed).
 template <typename T, bool TA, bool TB, int TT = 128>
 __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long lda,
                                               const T* __restrict__ Bblk, long ldb, int kb, int nk,
                                               TileSmem<T, TT>& sm,
-                                              typename Acc<T>::V (&acc)[TT / 32][TT / 32]) {
+                                              typename Acc<T>::V (&acc)[TT / 32][TT / 32],
+                                              bool krev = false) {
   constexpr int BK = Cfg<T, TT>::BK;
   constexpr int LS = Cfg<T, TT>::LS;
   constexpr int VEC = 16 / sizeof(T);
@@ -301,15 +303,21 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
   // one barrier after its fragments were read, a slot refilled right after
   // its store.  Loads past the end repeat the last slice and are not used.
   constexpr int U = (P % 2 == 0) ? P : 2 * P;
+  // K offset of slice i (clamped to the last slice), last slice first when
+  // krev
+  auto koff = [&](int i) {
+    i = min(i, nk - 1);
+    return kb + (krev ? nk - 1 - i : i) * BK;
+  };
 #pragma unroll
-  for (int u = 0; u < P; ++u) gload(kb + min(u, nk - 1) * BK, ra[u], rb[u]);
+  for (int u = 0; u < P; ++u) gload(koff(u), ra[u], rb[u]);
   lstore(0, ra[0], rb[0]);
-  gload(kb + min(P, nk - 1) * BK, ra[0], rb[0]);
+  gload(koff(P), ra[0], rb[0]);
   __syncthreads();
   Frag fr[2];
   fread(0, fr[0]);
   lstore(1, ra[1], rb[1]);
-  gload(kb + min(P + 1, nk - 1) * BK, ra[1], rb[1]);
+  gload(koff(P + 1), ra[1], rb[1]);
   // publishes slice 1, and every wave has read slice 0 before iteration 0
   // overwrites buffer 0
   __syncthreads();
@@ -317,7 +325,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
     constexpr int u = decltype(uc)::value;
     fread((u + 1) & 1, fr[(u + 1) & 1]);
     lstore(u & 1, ra[(u + 2) % P], rb[(u + 2) % P]);
-    gload(kb + min(i + 2 + P, nk - 1) * BK, ra[(u + 2) % P], rb[(u + 2) % P]);
+    gload(koff(i + 2 + P), ra[(u + 2) % P], rb[(u + 2) % P]);
     mma(fr[u & 1]);
     __syncthreads();
   };

@@ -66,10 +66,13 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
 // both operands are written to LDS with 16-byte stores.  MFMA step s of a
 // slice gives lane group g = lane / 16 the k = (4 r + g) VEC + c, s = r VEC
 // + c, so one 16-byte read of a k-contiguous row feeds VEC steps; the four
-// groups then read row-contiguous rows VEC apart, and LS puts those rows on
-// disjoint banks (4 LS = 16 mod 64 dwords for f32, 4 LS = 32 mod 64 for
-// f64) with rows 16-byte aligned.  BK keeps two 16-byte global loads per
-// thread per operand and slice.
+// groups then read row-contiguous rows VEC apart.  Bank model (it accounts
+// for every SQ_LDS_BANK_CONFLICT count measured on these kernels): 32 banks
+// of 4 bytes, 128 bytes per pass, so a pass serves 32 lanes of a 4-byte
+// access, 16 of an 8-byte and 8 of a 16-byte one.  f32 rows VEC = 4 apart
+// then need 4 LS = 16 mod 32 dwords; an f64 pass reads one group's 16
+// consecutive elements.  Rows stay 16-byte aligned.  BK keeps two 16-byte
+// global loads per thread per operand and slice.
 template <typename T, int TT = 128>
 struct Cfg;
 // BK 8 (f64) / 16 (f32) with two workgroups per CU measured fastest on
@@ -111,13 +114,13 @@ struct Cfg<float, 32> {
 
 // Element offset of 16-byte vector kv (elements kv VEC ..) of row m in a
 // k-contiguous slice of rows of BK elements.  KV = BK / VEC vectors make a
-// row of 4 KV dwords, so RP = 16 / KV rows share one sweep of the 64 banks;
-// XOR-ing kv with (m / RP) mod KV gives the 16 rows of a fragment read (same
-// kv) and the RP rows of a store (all kv) distinct bank quads.
+// row of 4 KV dwords, so RP = 8 / KV rows share one sweep of the 32 banks;
+// XOR-ing kv with (m / RP) mod KV gives the 8 rows of a fragment-read pass
+// (same kv) and the RP rows of a store pass (all kv) distinct bank quads.
 template <typename T, int BK>
 __device__ __forceinline__ int kc_offset(int m, int kv) {
-  constexpr int VEC = 16 / (int)sizeof(T), KV = BK / VEC, RP = 16 / KV;
-  static_assert(KV >= 1 && KV <= 16 && 16 % KV == 0, "k-contiguous row of 1..16 vectors");
+  constexpr int VEC = 16 / (int)sizeof(T), KV = BK / VEC, RP = 8 / KV;
+  static_assert(KV >= 1 && KV <= 8 && 8 % KV == 0, "k-contiguous row of 1..8 vectors");
   return m * BK + ((kv ^ ((m / RP) % KV)) * VEC);
 }
 

@@ -41,7 +41,7 @@ PEAK_HBM = 8000.0  # GB/s
 # HBM-side bytes per X^T X launch at n = 16384: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on that launch, each scaled by what the counter reads for a 1 GiB
 # device copy (x2.0 and x1.0); the run is summarised under profiles/.
-XTX_TRAFFIC_N16384 = {"mixed": 3.721e+10, "fp64": 8.185e+10}
+XTX_TRAFFIC_N16384 = {"mixed": 2.957e+10, "fp64": 6.610e+10}
 
 
 def exact_flops(n: int, d: int) -> float:

@@ -161,7 +161,7 @@ def bench_exact(args, gx):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mixed (f64 panels, log-det, refinement)",
+        "dtype": "f32-mixed (f64 fill, log-det, refinement)",
         "data": "synthetic (seeded SeededRng, seed 2)",
         "config": {"workload": "exact GPR NLML+grad, Constant x RbfArd", "n": wl.n, "d": wl.d,
                    "params": int(wl.theta().size), "parallelism": "single GPU"},

@@ -5,12 +5,13 @@ ABI of include/gprx_hip.h); this package is the host-side mirror of gprx's
 model API over that ABI.  There is no CPU fallback: calls fail loudly when
 the library or a GPU is missing.
 """
-from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InvalidHyperparameter,
-                     InvalidNoiseVariance, LengthMismatch, NonFiniteInput, NonFiniteKernelValue,
-                     Unsupported)
+from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InvalidConfig,
+                     InvalidHyperparameter, InvalidInterval, InvalidNoiseVariance, LengthMismatch,
+                     NonFiniteInput, NonFiniteKernelValue, OptimizationNotConverged, Unsupported)
 from .gpr import (Fallback, FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision,
                   Prediction, VarianceKind)
 from .batch import GprBatch
+from .optimizer import Fixed, Lbfgs, OptResult
 from .sgpr import FittedSgpr, Sgpr
 from .svgp import FittedSvgp, Svgp, prior_params
 from .kernels import (ConstantKernel, KernelSpec, PeriodicKernel, Product, RbfArdKernel, RbfKernel,
@@ -22,4 +23,5 @@ __all__ = [
     "Unsupported", "Fallback", "FittedGpr", "GaussianLikelihood", "Gpr", "JitterPolicy", "Precision",
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
     "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
+    "Fixed", "Lbfgs", "OptResult", "OptimizationNotConverged", "InvalidConfig", "InvalidInterval",
 ]

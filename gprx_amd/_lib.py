@@ -25,6 +25,9 @@ LENGTH_MISMATCH = 6
 EMPTY_INPUT = 7
 UNSUPPORTED = 8
 DEVICE_ERROR = 9
+NOT_CONVERGED = 10
+INVALID_CONFIG = 11
+INVALID_INTERVAL = 12
 
 FP64 = 0
 MIXED = 1
@@ -76,6 +79,19 @@ class ProblemC(C.Structure):
     ]
 
 
+class LbfgsOptionsC(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_uint64),
+        ("tolerance", C.c_double),
+        ("history_size", C.c_int32),
+        ("restarts", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+# gprx_objective_fn: int (*)(void* ctx, const double* x, size_t n, double* value, double* grad)
+ObjectiveFn = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, C.c_size_t, _dp, _dp)
+
 _err = C.POINTER(GprxError)
 _mp = C.POINTER(C.c_void_p)
 
@@ -115,6 +131,12 @@ SIGNATURES = {
     "gprx_hip_batch_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                           C.POINTER(C.c_double)]),
     "gprx_hip_batch_last_fallback": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_size_t]),
+    "gprx_lbfgs_default_options": (None, [C.POINTER(LbfgsOptionsC)]),
+    "gprx_lbfgs_minimize": (C.c_int, [ObjectiveFn, _vp, _dp, C.c_size_t, C.POINTER(LbfgsOptionsC),
+                                      _dp, _dp, C.POINTER(C.c_uint64), _err]),
+    "gprx_hip_exact_fit": (C.c_int, [_dp, C.c_size_t, C.c_size_t, _dp, C.POINTER(KernelProgram), _dp,
+                                     C.c_size_t, _dp, C.c_int32, C.POINTER(LbfgsOptionsC), _dp, _dp,
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_int32), _err]),
     "gprx_hip_destroy": (None, [_vp]),
     "gprx_hip_last_eval_ms": (C.c_double, [_vp]),
     "gprx_hip_last_dense_ms": (C.c_double, [_vp]),

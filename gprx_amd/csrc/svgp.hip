@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "model.hpp"
+#include "rng.hpp"
 
 namespace gprx {
 int exact_validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err);
@@ -33,42 +34,6 @@ using namespace gprx;
 namespace {
 
 constexpr double kLog2Pi = 1.8378770664093453;
-
-// SplitMix64-seeded Xoshiro256++ (src/rng.rs:15-63), for the epoch shuffle.
-struct Rng {
-  uint64_t s[4];
-  explicit Rng(uint64_t seed) {
-    uint64_t x = seed;
-    for (auto& v : s) {
-      x += 0x9e3779b97f4a7c15ULL;
-      uint64_t z = x;
-      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-      v = z ^ (z >> 31);
-    }
-  }
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  uint64_t next() {
-    const uint64_t result = rotl(s[0] + s[3], 23) + s[0];
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return result;
-  }
-  uint64_t up_to(uint64_t mx) {
-    const uint64_t range = mx + 1;
-    if (range == 0) return next();
-    const uint64_t threshold = (0 - range) % range;
-    for (;;) {
-      const unsigned __int128 wide = (unsigned __int128)next() * range;
-      if ((uint64_t)wide >= threshold) return (uint64_t)(wide >> 64);
-    }
-  }
-};
 
 template <typename F>
 int guarded(gprx_error* err, F&& f) {

@@ -48,9 +48,22 @@ class DeviceError(GprError):
     status = L.DEVICE_ERROR
 
 
+class OptimizationNotConverged(GprError):
+    status = L.NOT_CONVERGED
+
+
+class InvalidConfig(GprError):
+    status = L.INVALID_CONFIG
+
+
+class InvalidInterval(GprError):
+    status = L.INVALID_INTERVAL
+
+
 _BY_STATUS = {c.status: c for c in (CholeskyFailed, NonFiniteKernelValue, NonFiniteInput,
                                       InvalidHyperparameter, InvalidNoiseVariance, LengthMismatch,
-                                      EmptyInput, Unsupported, DeviceError)}
+                                      EmptyInput, Unsupported, DeviceError, OptimizationNotConverged,
+                                      InvalidConfig, InvalidInterval)}
 
 
 def raise_for(rc: int, err) -> None:

@@ -89,6 +89,52 @@ class Gpr:
         self.jitter = JitterPolicy()
         self.precision = Precision.FP64
         self.mixed_jitter_bound = 0.0
+        self.optimizer = None
+        self.bounds = None
+
+    def with_optimizer(self, optimizer) -> "Gpr":
+        """Lbfgs (with or without restarts) or Fixed, used by fit()."""
+        self.optimizer = optimizer
+        return self
+
+    def with_bounds(self, bounds) -> "Gpr":
+        """User-unit (lo, hi) of every parameter in get_params order (kernel
+        leaves, then the noise variance); default (1e-5, 1e5) each."""
+        self.bounds = None if bounds is None else L.f64(bounds)
+        return self
+
+    def fit(self, x, n: int, d: int, y) -> "FittedGpr":
+        """Gpr::fit (src/gpr/trainer.rs): minimizes the NLML with the
+        optimizer (default Lbfgs()), then factors at the best parameters.
+        The fitted model's fit_result holds the optimizer's OptResult."""
+        from .optimizer import Fixed, Lbfgs, OptResult
+        opt = self.optimizer if self.optimizer is not None else Lbfgs()
+        if isinstance(opt, Fixed):
+            return self.factor(x, n, d, y)
+        x = L.f64(x)
+        y = L.f64(y)
+        if x.size != n * d or y.size != n:
+            raise_for(L.LENGTH_MISMATCH, "x must hold n*d values and y n values")
+        theta = np.concatenate([self.kernel.get_params(), [self.likelihood.log_param()]])
+        if self.bounds is not None and self.bounds.size != 2 * theta.size:
+            raise_for(L.LENGTH_MISMATCH, "bounds must hold (lo, hi) for every parameter")
+        prog, ops = program_c(self.kernel)
+        best = np.empty(theta.size)
+        val = C.c_double()
+        it = C.c_uint64()
+        start = C.c_int32()
+        err = L.GprxError()
+        oc = opt.options_c()
+        bptr = L.dptr(self.bounds) if self.bounds is not None else None
+        rc = L.lib().gprx_hip_exact_fit(L.dptr(x), n, d, L.dptr(y), C.byref(prog), L.dptr(theta),
+                                        theta.size, bptr, self.precision, C.byref(oc),
+                                        L.dptr(best), C.byref(val), C.byref(it), C.byref(start),
+                                        C.byref(err))
+        raise_for(rc, err)
+        fitted = self._create(x, n, d, y, best)
+        fitted.fit_result = OptResult(best.copy(), val.value, it.value, start.value)
+        fitted.neg_log_marginal_likelihood()
+        return fitted
 
     def with_jitter_policy(self, policy: JitterPolicy) -> "Gpr":
         self.jitter = policy
@@ -117,6 +163,9 @@ class Gpr:
         if x.size != n * d or y.size != n:
             raise_for(L.LENGTH_MISMATCH, "x must hold n*d values and y n values")
         theta = np.concatenate([self.kernel.get_params(), [self.likelihood.log_param()]])
+        return self._create(x, n, d, y, theta)
+
+    def _create(self, x, n: int, d: int, y, theta: np.ndarray) -> "FittedGpr":
         prog, ops = program_c(self.kernel)
         err = L.GprxError()
         handle = C.c_void_p()

@@ -40,7 +40,10 @@ typedef enum gprx_status {
   GPRX_LENGTH_MISMATCH = 6,          /* GprError::LengthMismatch */
   GPRX_EMPTY_INPUT = 7,              /* GprError::EmptyInput */
   GPRX_UNSUPPORTED = 8,              /* a program or size this backend does not take */
-  GPRX_DEVICE_ERROR = 9              /* a HIP runtime failure */
+  GPRX_DEVICE_ERROR = 9,             /* a HIP runtime failure */
+  GPRX_NOT_CONVERGED = 10,           /* GprError::OptimizationNotConverged */
+  GPRX_INVALID_CONFIG = 11,          /* GprError::InvalidConfig */
+  GPRX_INVALID_INTERVAL = 12         /* GprError::InvalidInterval (a start outside its bounds) */
 } gprx_status;
 
 /* CholeskyStage of GprError::CholeskyFailed (src/error.rs). */
@@ -257,6 +260,55 @@ int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy,
  * reasons[k] receives each problem's gprx_fallback for the last call.  No
  * counterpart in the reference (it evaluates problems one by one). */
 int gprx_hip_batch_last_fallback(const gprx_hip_model* model, int32_t* reasons, size_t k);
+
+/* ------------------------------------------------------------------ fit */
+
+/* Lbfgs settings (src/optimizer/lbfgs.rs:19-75): max_iterations 100,
+ * gradient tolerance sqrt(eps) (on the gradient in logit coordinates),
+ * history 10; Lbfgs::with_restarts(n, seed) adds n starts drawn
+ * log-uniformly inside the bounds from SeededRng(seed)
+ * (src/optimizer/mod.rs:182-227, src/optimizer/logit.rs:243-265). */
+typedef struct gprx_lbfgs_options {
+  uint64_t max_iterations;
+  double tolerance;
+  int32_t history_size;
+  uint32_t restarts;
+  uint64_t seed;
+} gprx_lbfgs_options;
+
+/* Fills the defaults of Lbfgs::new(), no restarts. */
+void gprx_lbfgs_default_options(gprx_lbfgs_options* opt);
+
+/* An objective on the host: value and gradient at x[n]; a nonzero return
+ * (or a non-finite value) counts as a failed point. */
+typedef int (*gprx_objective_fn)(void* ctx, const double* x, size_t n, double* value,
+                                 double* grad);
+
+/* Replaces the unconstrained L-BFGS run (src/optimizer/lbfgs.rs:118-142,
+ * argmin 0.11's LBFGS + MoreThuenteLineSearch restated): minimizes fn from
+ * x0, writing the best point, its value and the iterations taken.
+ * GPRX_NOT_CONVERGED when no point had a finite cost.  Host only. */
+int gprx_lbfgs_minimize(gprx_objective_fn fn, void* ctx, const double* x0, size_t n,
+                        const gprx_lbfgs_options* opt, double* x_out, double* value,
+                        uint64_t* iterations, gprx_error* err);
+
+/* Replaces Gpr::with_optimizer(Lbfgs::..).fit (src/gpr/trainer.rs, the
+ * Optimizer impl of src/optimizer/lbfgs.rs:101-116): minimizes the exact
+ * NLML over log theta (p = kernel params + 1) through the logit map of
+ * `bounds` (2p user-unit values lo, hi per parameter, or NULL for the
+ * default (1e-5, 1e5) of every parameter; src/optimizer/logit.rs:75-230)
+ * from log_theta and opt->restarts further starts.  All starts run together:
+ * every round evaluates each live start's next point in one batched GPU
+ * call (gprx_hip_batch_*, jitter policy fixed(0)); a failed point costs
+ * 1e300 (src/optimizer/adapter.rs:23,116-150).  Writes the best start's
+ * best point, its NLML, its iterations and its index (0 = log_theta).
+ * The first start failing fails the fit; a failed restart is skipped
+ * (src/optimizer/mod.rs:210-226). */
+int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
+                       const gprx_kernel_program* kernel, const double* log_theta, size_t p,
+                       const double* bounds, int32_t precision, const gprx_lbfgs_options* opt,
+                       double* best_log_theta, double* best_value, uint64_t* iterations,
+                       int32_t* best_start, gprx_error* err);
 
 /* ------------------------------------------------------------------ all */
 

@@ -347,6 +347,92 @@ def bench_batch(args, gx):
     }
 
 
+def bench_fit(args, gx):
+    """Gpr.fit with Lbfgs and R restarts at config 1 (n=1024, d=4, isotropic
+    RBF, FP64): every start's L-BFGS evaluated together by the batched path,
+    against the same starts fitted one after another."""
+    import copy
+
+    from gprx_amd.configs import exact_rbf_small
+    from gprx_amd.rng import SeededRng
+
+    wl = exact_rbf_small()
+    R, seed = args.restarts, 0
+    th0 = wl.theta()
+    lo, hi = math.log(1e-5), math.log(1e5)
+    r = SeededRng(seed)  # the restart draws gprx_hip_exact_fit makes
+    starts = [th0] + [np.array([lo + r.open_unit() * (hi - lo) for _ in th0]) for _ in range(R)]
+
+    def gpr_at(theta):
+        spec = copy.deepcopy(wl.kernel())
+        spec.set_params(theta[:-1])
+        return gx.Gpr(spec, gx.GaussianLikelihood(math.exp(theta[-1])))
+
+    def batched():
+        return gpr_at(th0).with_optimizer(gx.Lbfgs().with_restarts(R, seed)).fit(
+            wl.x, wl.n, wl.d, wl.y)
+
+    def looped():
+        return [gpr_at(t).with_optimizer(gx.Lbfgs()).fit(wl.x, wl.n, wl.d, wl.y) for t in starts]
+
+    print(f"[bench] fit: 1+{R} starts batched", file=sys.stderr, flush=True)
+    fb, wall_b = _timed(batched, args.steps, args.warmup)
+    best_b = fb.neg_log_marginal_likelihood()
+    res_b = fb.fit_result
+    fb.close()
+    print(f"[bench] fit: 1+{R} starts looped", file=sys.stderr, flush=True)
+    fl, wall_l = _timed(looped, args.steps, args.warmup)
+    best_l = min(f.neg_log_marginal_likelihood() for f in fl)
+    for f in fl:
+        f.close()
+    # one round of the batched fit: a batched NLML+grad of all starts
+    probs = [(gpr_at(t).kernel, gpr_at(t).likelihood, wl.x, wl.n, wl.d, wl.y) for t in starts]
+    bt = gx.GprBatch(probs, precision=gx.Precision.FP64)
+    _, wall_r = _timed(lambda: bt.value_and_gradient(starts), 5, 1)
+    bt.close()
+    F = len(starts) * exact_flops(wl.n, wl.d)
+    ach = F / (wall_r / 5) / 1e12
+    out = {
+        "metric": f"L-BFGS starts fitted/s, exact GPR n=1024 d=4 RBF (config 1), 1+{R} starts "
+                  "batched vs looped",
+        "value": len(starts) / (wall_b / args.steps), "unit": "starts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_b / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SeededRng, seeds 0 and 1)",
+        "config": {"workload": "exact GPR fit, Lbfgs with restarts", "n": wl.n, "d": wl.d,
+                   "starts": len(starts), "parallelism": "single GPU"},
+        "batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
+        "speedup": wall_l / wall_b, "best_nlml_batched": best_b, "best_nlml_looped": best_l,
+        "best_start": res_b.best_start, "best_start_iterations": res_b.iterations,
+        "round_ms": wall_r / 5 * 1e3,
+        "roofline": {"bound": "mfma", "kernel": "one round: batched NLML+grad of every start",
+                     "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP64, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+
+        print("[bench] fit: cpu baseline (oracle objective, first start)", file=sys.stderr,
+              flush=True)
+        ops = wl.kernel().program()
+        span = hi - lo
+
+        def obj(z):
+            s_ = 1.0 / (1.0 + np.exp(-z / (span / 4)))
+            th = lo + span * s_
+            v, gr, _, _ = oracle.exact_value_grad(ops, th, wl.x, wl.n, wl.d, wl.y)
+            return v, gr * span * s_ * (1 - s_) / (span / 4)
+
+        u = (th0 - lo) / span
+        t0 = time.perf_counter()
+        gx.Lbfgs().minimize_unconstrained(obj, np.log(u / (1 - u)) * (span / 4))
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / t, "unit": "starts/s", "cores": oracle.threads(),
+                               "kind": "port",
+                               "sample": f"the first start's fit on the oracle's objective ({t:.2f} s)"}
+    return out
+
+
 def bench_copy(args, gx):
     """One 1 GiB device-to-device copy per step (torch's copy kernel): known
     HBM traffic of 1 GiB read + 1 GiB written per launch."""
@@ -376,7 +462,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "copy"], default="exact")
+    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "copy"],
+                    default="exact")
+    ap.add_argument("--restarts", type=int, default=16)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--cpu-n", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -387,12 +475,13 @@ def main():
               "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "copy": 5}[args.workload]
+        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1,
+                      "copy": 5}[args.workload]
 
     import gprx_amd as gx
 
     fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch,
-          "copy": bench_copy}[args.workload]
+          "fit": bench_fit, "copy": bench_copy}[args.workload]
     print(json.dumps(fn(args, gx)))
 
 

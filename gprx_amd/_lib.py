@@ -102,6 +102,7 @@ SIGNATURES = {
                                         _dp, C.c_size_t, C.c_int32, C.POINTER(JitterPolicyC), _mp,
                                         _err]),
     "gprx_hip_exact_value_and_grad": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _err]),
+    "gprx_hip_exact_hessian": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _dp, _err]),
     "gprx_hip_exact_predict": (C.c_int, [_vp, _dp, C.c_size_t, C.c_size_t, C.c_int32, _dp, _dp,
                                          _err]),
     "gprx_hip_get_params": (C.c_int, [_vp, _dp, C.c_size_t]),

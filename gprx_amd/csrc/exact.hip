@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 
+#include "hessian.hpp"
 #include "model.hpp"
 
 using namespace gprx;
@@ -263,6 +264,84 @@ struct ExactModel : gprx_hip_model {
       if (!std::isfinite(next)) remaining = 0;
     }
   }
+
+  // NLML, gradient and Hessian at th in FP64 (ExactFit::fill_hessian_from_factor,
+  // src/gpr/exact_fit.rs:484-596; the pieces are described in hessian.hip).
+  // H is p x p, row-major.  Works on buffers of its own: the model's factor,
+  // alpha and inverse stay those of th afterwards.
+  int hessian(const double* th, double* nlml, double* grad, double* H, gprx_error* err) {
+    const int saved = precision;
+    precision = GPRX_FP64;
+    int rc = evaluate(th, true, nlml, grad, err);
+    precision = saved;
+    if (rc != GPRX_OK) return rc;
+    hipStream_t s = stream;
+    const int nk = hp.n_params, np = nk + 1;
+    const double sigma2 = std::exp(th[nk]);
+    // second order: kernel pairs by contraction, the noise pair from the
+    // noise trace (d2A/dlog sigma^2 2 = sigma^2 I), no kernel-noise cross term
+    std::vector<double> h2((size_t)np * np, 0.0);
+    const int tiles = lower_tiles(npad);
+    DevBuf<double> part2;
+    HostBuf<double> hpart;
+    part2.alloc((size_t)tiles);
+    hpart.alloc((size_t)std::max<long>(tiles, npad));
+    const HessSlots hs = hess_slots(hp);
+    for (int i = 0; i < nk; ++i)
+      for (int j = i; j < nk; ++j) {
+        contract2(dp, prog.p, hs, pts(), A64.p, npad, alpha.p, i, j, part2.p, s);
+        GPRX_HIP_CHECK(hipMemcpyAsync(hpart.p, part2.p, sizeof(double) * tiles,
+                                      hipMemcpyDeviceToHost, s));
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (int t = 0; t < tiles; ++t) acc += hpart.p[t];
+        h2[(size_t)i * np + j] = h2[(size_t)j * np + i] = -0.5 * acc;
+      }
+    h2[(size_t)nk * np + nk] = -0.5 * sigma2 * hscal.p[2];
+    // first order: S_i = X A_i X^T (lower), v_i = X (A_i alpha)
+    const size_t nn = (size_t)npad * npad;
+    DevBuf<double> D, Y, S, u, v, gp;
+    D.alloc(nn);
+    Y.alloc(nn);
+    S.alloc(nn * np);
+    u.alloc((size_t)npad);
+    v.alloc((size_t)npad * np);
+    gp.alloc((size_t)npad * ((npad + 255) / 256));
+    GPRX_HIP_CHECK(hipMemsetAsync(D.p, 0, sizeof(double) * nn, s));
+    for (int i = 0; i < np; ++i) {
+      if (i < nk) {
+        dfill(dp, prog.p, hs, pts(), i, D.p, npad, s);
+      } else {
+        GPRX_HIP_CHECK(hipMemsetAsync(D.p, 0, sizeof(double) * nn, s));
+        set_diag(D.p, npad, n, sigma2, s);
+      }
+      gemv_n<double>(D.p, npad, (int)npad, (int)npad, alpha.p, u.p, gp.p, 0, 0, 0, s);
+      gemv_n<double>(X64.p, npad, (int)npad, (int)npad, u.p, v.p + (size_t)i * npad, gp.p, 1, 0, 0,
+                     s);
+      gemm_call<double>(false, false, X64.p, npad, D.p, npad, Y.p, npad, (int)npad, (int)npad,
+                        (int)npad, 1.0, 0.0, kKhiRow, 0, kOrderRowFast, s);
+      gemm_call<double>(false, true, Y.p, npad, X64.p, npad, S.p + (size_t)i * nn, npad, (int)npad,
+                        (int)npad, (int)npad, 1.0, 0.0, kKhiCol, 1, kOrderRowFast, s);
+    }
+    HostBuf<double> hv;
+    hv.alloc((size_t)npad * np);
+    GPRX_HIP_CHECK(hipMemcpyAsync(hv.p, v.p, sizeof(double) * npad * np, hipMemcpyDeviceToHost, s));
+    DevBuf<double> pd;
+    pd.alloc((size_t)npad);
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j <= i; ++j) {
+        dot_lower(S.p + (size_t)i * nn, S.p + (size_t)j * nn, npad, n, pd.p, s);
+        GPRX_HIP_CHECK(hipMemcpyAsync(hpart.p, pd.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+        double tr = 0.0, vv = 0.0;
+        for (int c = 0; c < n; ++c) tr += hpart.p[c];
+        for (int r = 0; r < n; ++r) vv += hv.p[(size_t)i * npad + r] * hv.p[(size_t)j * npad + r];
+        const double h = h2[(size_t)i * np + j] - 0.5 * tr + vv;
+        H[(size_t)i * np + j] = H[(size_t)j * np + i] = h;
+      }
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    return GPRX_OK;
+  }
 };
 
 int validate_theta(const HostProgram& hp, const double* th, size_t p, gprx_error* err) {
@@ -448,6 +527,29 @@ int gprx_hip_exact_value_and_grad(gprx_hip_model* model, const double* log_theta
     m->theta.assign(log_theta, log_theta + p);
     *nlml = v;
     if (grad) std::memcpy(grad, g.data(), sizeof(double) * p);
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_t p,
+                           double* nlml, double* grad, double* hess, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !log_theta || !nlml || !grad ||
+        !hess) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    int rc = validate_theta(m->hp, log_theta, p, err);
+    if (rc != GPRX_OK) return rc;
+    std::vector<double> g(p, 0.0), h(p * p, 0.0);
+    double v = 0.0;
+    rc = m->hessian(log_theta, &v, g.data(), h.data(), err);
+    if (rc != GPRX_OK) return rc;
+    m->theta.assign(log_theta, log_theta + p);
+    *nlml = v;
+    std::memcpy(grad, g.data(), sizeof(double) * p);
+    std::memcpy(hess, h.data(), sizeof(double) * p * p);
     return GPRX_OK;
   });
 }

@@ -219,6 +219,23 @@ class FittedGpr:
         out[:] = g
         return val.value
 
+    def value_gradient_hessian_into(self, params, grad, hess) -> float:
+        """TwiceDifferentiable::value_gradient_hessian_into: NLML, gradient and
+        the p x p Hessian (FP64) at params; the model moves to params."""
+        th = L.f64(params)
+        if grad.size != th.size or hess.size != th.size * th.size:
+            raise_for(L.LENGTH_MISMATCH, "grad must hold p values and hess p*p")
+        g = np.empty(th.size)
+        h = np.empty(th.size * th.size)
+        v = C.c_double()
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_hessian(self._h, L.dptr(th), th.size, C.byref(v), L.dptr(g),
+                                            L.dptr(h), C.byref(err))
+        raise_for(rc, err)
+        grad.reshape(-1)[:] = g
+        hess.reshape(-1)[:] = h
+        return v.value
+
     def value(self, params) -> float:
         p = L.f64(params)
         val = C.c_double()

@@ -138,6 +138,18 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
 int gprx_hip_exact_value_and_grad(gprx_hip_model* model, const double* log_theta, size_t p,
                                   double* nlml, double* grad, gprx_error* err);
 
+/* Replaces TwiceDifferentiable::value_gradient_hessian_into for the exact
+ * model (src/objective.rs; ExactFit::fill_hessian_from_factor,
+ * src/gpr/exact_fit.rs:484-596), the Hessian TrustRegion uses: NLML,
+ * gradient[p] and the p x p Hessian (row-major, symmetric) with respect to
+ * log_theta, always formed in FP64 whatever the model's precision:
+ *   H_ij = -1/2 <W, d2A_ij> - 1/2 tr(S_i S_j) + v_i . v_j,
+ *   W = alpha alpha^T - A^-1, S_i = L^-1 A_i L^-T, v_i = L^-1 A_i alpha.
+ * Needs (p + 2) n_pad^2 doubles of device memory beyond the model's.  On
+ * success the model's parameters become log_theta. */
+int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_t p,
+                           double* nlml, double* grad, double* hess, gprx_error* err);
+
 /* Replaces FittedGpr::predict_with (src/gpr/fitted.rs:854-960,
  * src/gpr/shared.rs:754-816) at the model's current parameters.  var may be
  * NULL (mean only). */

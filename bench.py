@@ -433,6 +433,41 @@ def bench_fit(args, gx):
     return out
 
 
+def bench_hessian(args, gx):
+    """NLML, gradient and Hessian (FP64) at config 2's kernel and size."""
+    from gprx_amd.configs import exact_ard
+
+    wl = exact_ard(n=args.n)
+    th = wl.theta()
+    p = th.size
+    model = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).prepare(wl.x, wl.n, wl.d, wl.y)
+    grad, hess = np.empty(p), np.empty(p * p)
+    print(f"[bench] hessian n={wl.n} p={p}", file=sys.stderr, flush=True)
+    _, wall = _timed(lambda: model.value_gradient_hessian_into(th, grad, hess), args.steps,
+                     args.warmup)
+    model.close()
+    ms = wall / args.steps * 1e3
+    # S_i = (X A_i) X^T per parameter: X A_i with X lower (n^3), then the
+    # lower half of (X A_i) X^T with X^T upper (n^3 / 3); plus the evaluation
+    F = p * (4.0 / 3.0) * wl.n ** 3 + exact_flops(wl.n, wl.d)
+    ach = F / (ms * 1e-3) / 1e12
+    H = hess.reshape(p, p)
+    return {
+        "metric": f"NLML+grad+Hessian evals/s, exact GPR n={wl.n} d={wl.d} Constant x RbfArd (p={p}), FP64",
+        "value": 1e3 / ms, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SeededRng, seed 2)",
+        "config": {"workload": "exact GPR Hessian (TrustRegion)", "n": wl.n, "d": wl.d,
+                   "params": p, "parallelism": "single GPU"},
+        "hessian_symmetric": bool(np.array_equal(H, H.T)),
+        "hessian_diag": [float(v) for v in np.diag(H)],
+        "algorithmic_flop": F,
+        "roofline": {"bound": "mfma", "kernel": "whole Hessian evaluation", "achieved": ach,
+                     "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach / PEAK_FP64,
+                     "traffic": None},
+    }
+
+
 def bench_copy(args, gx):
     """One 1 GiB device-to-device copy per step (torch's copy kernel): known
     HBM traffic of 1 GiB read + 1 GiB written per launch."""
@@ -462,7 +497,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "copy"],
+    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "hessian", "copy"],
                     default="exact")
     ap.add_argument("--restarts", type=int, default=16)
     ap.add_argument("--n", type=int, default=16384)
@@ -475,13 +510,13 @@ def main():
               "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1,
+        args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1, "hessian": 2,
                       "copy": 5}[args.workload]
 
     import gprx_amd as gx
 
     fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch,
-          "fit": bench_fit, "copy": bench_copy}[args.workload]
+          "fit": bench_fit, "hessian": bench_hessian, "copy": bench_copy}[args.workload]
     print(json.dumps(fn(args, gx)))
 
 

@@ -60,6 +60,10 @@ struct ExactModel : gprx_hip_model {
   DevBuf<int> info;
   HostBuf<double> hscal, hacc;
   HostBuf<int> hinfo;
+  // online insertion: host copies of the data (column-major n x d, n), and
+  // device work vectors
+  std::vector<double> hx, hy;
+  DevBuf<double> zv, kcol, lv, wv, xnew, gpart1;
 
   DevPoints pts() const { return DevPoints{x.p, npad, n}; }
 
@@ -412,6 +416,166 @@ int gprx_hip_device_count(void) {
   return c;
 }
 
+// Buffers sized by npad, beyond the points and targets (set-up, and again
+// when online insertion outgrows the padding).
+void size_buffers(ExactModel* m) {
+  hipStream_t s = m->stream;
+  m->prog.alloc(1);
+  m->hprog.alloc(1);
+  const long np = m->npad;
+  m->logdiag.alloc(np);
+  m->alpha.alloc(np);
+  m->tv.alloc(np);
+  m->r.alloc(np);
+  m->dlt.alloc(np);
+  m->p1.alloc(np);
+  m->q1.alloc(np);
+  m->partials.alloc((size_t)lower_tiles(np) * kMaxAcc);
+  m->tpart.alloc((size_t)lower_tiles(np) * 4 * kTile);
+  m->accs.alloc(kMaxAcc);
+  m->scal.alloc(16);
+  m->info.alloc(1);
+  m->hscal.alloc(16);
+  m->hacc.alloc(kMaxAcc);
+  m->hinfo.alloc(1);
+  GPRX_HIP_CHECK(hipMemsetAsync(m->r.p, 0, sizeof(double) * np, s));
+  GPRX_HIP_CHECK(hipMemsetAsync(m->alpha.p, 0, sizeof(double) * np, s));
+  GPRX_HIP_CHECK(hipMemsetAsync(m->logdiag.p, 0, sizeof(double) * np, s));
+  GPRX_HIP_CHECK(hipMemsetAsync(m->scal.p, 0, sizeof(double) * 16, s));
+  if (m->precision == GPRX_MIXED)
+    m->alloc_mixed();
+  else
+    m->alloc_fp64();
+}
+
+// Row n of X (the new point's): X[n][j] = -w_j / d for j < n, X[n][n] = 1 / d.
+__global__ void k_insert_row(double* X, long ld, int n, const double* __restrict__ w, double inv_d) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) X[n + (long)j * ld] = -w[j] * inv_d;
+  if (j == n) X[n + (long)n * ld] = inv_d;
+}
+
+// Re-pads the model for n_new points: every npad-sized buffer again, the
+// points and targets from the host copies, no factor.
+void regrow(ExactModel* m, long npad_new) {
+  hipStream_t s = m->stream;
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  m->npad = npad_new;
+  m->nt = (int)(npad_new / kTile);
+  m->ldt = (long)((m->nt + 1) / 2) * kTile;
+  for (DevBuf<double>* b : {&m->A64, &m->X64, &m->T64, &m->K64}) b->release();
+  for (DevBuf<float>* b : {&m->A32, &m->X32, &m->T32}) b->release();
+  size_buffers(m);
+  upload_points(m->x, m->hx.data(), m->n, m->d, m->npad, m->dp, s);
+  std::vector<double> yh(m->npad, 0.0);
+  std::memcpy(yh.data(), m->hy.data(), sizeof(double) * m->n);
+  m->y.alloc(m->npad);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  m->factored = false;
+}
+
+// OnlineGpr::insert (src/gpr/online.rs) on the FP64 factor, O(n^2): with
+// k = K(X, x_new), kappa = k(x_new, x_new) + noise and l = L^-1 k = X k,
+// the new row of L is [l^T, d], d = sqrt(kappa - l^T l); the new row of
+// X = L^-1 is [-l^T X / d, 1 / d]; z = X y gains (y_new - l^T z) / d, and
+// alpha = X^T z, NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  When the
+// padding is full the model is re-padded by 128 rows and factored again.
+int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_error* err) {
+  if (m->precision != GPRX_FP64) {
+    set_error(err, GPRX_UNSUPPORTED, "online insertion runs on FP64 models");
+    return GPRX_UNSUPPORTED;
+  }
+  if (!all_finite(xn, (size_t)m->d) || !std::isfinite(yn)) {
+    set_error(err, GPRX_NON_FINITE_INPUT, "non-finite point");
+    return GPRX_NON_FINITE_INPUT;
+  }
+  hipStream_t s = m->stream;
+  const int n = m->n, d = m->d;
+  const double* th = m->theta.data();
+  // host copies with the new point appended (column-major n x d)
+  std::vector<double> hx_new((size_t)(n + 1) * d);
+  for (int q = 0; q < d; ++q) {
+    std::memcpy(hx_new.data() + (size_t)q * (n + 1), m->hx.data() + (size_t)q * n, sizeof(double) * n);
+    hx_new[(size_t)q * (n + 1) + n] = xn[q];
+  }
+  if (n + 1 > m->npad) {
+    std::vector<double> hx_old = m->hx;
+    m->hx = hx_new;
+    m->hy.push_back(yn);
+    m->n = n + 1;
+    regrow(m, m->npad + kTile);
+    double v = 0.0;
+    int rc = m->evaluate(th, false, &v, nullptr, err);
+    if (rc != GPRX_OK) {  // back to the model before the insert
+      m->hx = hx_old;
+      m->hy.pop_back();
+      m->n = n;
+      regrow(m, m->npad - kTile);
+      gprx_error e2{};
+      double v2 = 0.0;
+      (void)m->evaluate(th, false, &v2, nullptr, &e2);
+      return rc;
+    }
+    if (nlml) *nlml = v;
+    return GPRX_OK;
+  }
+  if (!m->factored || m->cur_form != GPRX_FP64) {
+    double v = 0.0;
+    int rc = m->evaluate(th, false, &v, nullptr, err);
+    if (rc != GPRX_OK) return rc;
+  }
+  const long np = m->npad;
+  m->zv.alloc(np);
+  m->kcol.alloc((size_t)np * kTile);
+  m->lv.alloc(np);
+  m->wv.alloc(np);
+  m->xnew.alloc((size_t)kTile * m->dp);
+  m->gpart1.alloc((size_t)np * ((np + 255) / 256));
+  // the new point as a one-row point set, and z = X y of the current factor
+  GPRX_HIP_CHECK(hipMemsetAsync(m->xnew.p, 0, sizeof(double) * kTile * m->dp, s));
+  GPRX_HIP_CHECK(hipMemcpy2DAsync(m->xnew.p, sizeof(double) * kTile, xn, sizeof(double),
+                                  sizeof(double), d, hipMemcpyHostToDevice, s));
+  const DevPoints pn{m->xnew.p, kTile, 1};
+  gemv_n<double>(m->X64.p, np, n, n, m->y.p, m->zv.p, m->gpart1.p, 1, 0, 0, s);
+  fill_cross<double>(m->dp, m->prog.p, m->pts(), pn, m->kcol.p, np, s);
+  fill_diag(m->dp, m->prog.p, pn, m->scal.p + 12, s);
+  gemv_n<double>(m->X64.p, np, n, n, m->kcol.p, m->lv.p, m->gpart1.p, 1, 0, 0, s);
+  dot_and_sum(m->lv.p, m->lv.p, m->lv.p, n, m->scal.p + 8, s);
+  dot_and_sum(m->lv.p, m->zv.p, m->lv.p, n, m->scal.p + 10, s);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  const double kappa = m->hscal.p[12] + std::exp(th[m->hp.n_params]) + m->last_jitter;
+  const double piv2 = kappa - m->hscal.p[8];
+  if (!(piv2 > 0.0) || !std::isfinite(piv2)) {
+    set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky factor failed on insertion", GPRX_STAGE_FIT,
+              m->last_jitter, (size_t)(n + 1));
+    return GPRX_CHOLESKY_FAILED;
+  }
+  const double dd = std::sqrt(piv2);
+  const double zn = (yn - m->hscal.p[10]) / dd, ld = std::log(dd);
+  // X's new row, the point, the target, z_n and log L_nn
+  gemv_t<double>(m->X64.p, np, n, n, m->lv.p, m->wv.p, 1, 0, 0, 0, s);
+  hipLaunchKernelGGL(k_insert_row, dim3((n + 1 + 255) / 256), dim3(256), 0, s, m->X64.p, np, n,
+                     m->wv.p, 1.0 / dd);
+  GPRX_HIP_CHECK(hipGetLastError());
+  GPRX_HIP_CHECK(hipMemcpy2DAsync(m->x.p + n, sizeof(double) * np, xn, sizeof(double),
+                                  sizeof(double), d, hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p + n, &yn, sizeof(double), hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->zv.p + n, &zn, sizeof(double), hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->logdiag.p + n, &ld, sizeof(double), hipMemcpyHostToDevice, s));
+  m->n = n + 1;
+  m->hx = hx_new;
+  m->hy.push_back(yn);
+  // alpha = X^T z and the NLML
+  gemv_t<double>(m->X64.p, np, n + 1, n + 1, m->zv.p, m->alpha.p, 1, 0, 0, 0, s);
+  dot_and_sum(m->zv.p, m->zv.p, m->logdiag.p, n + 1, m->scal.p, s);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  if (nlml) *nlml = 0.5 * m->hscal.p[0] + m->hscal.p[1] + 0.5 * (double)(n + 1) * kLog2Pi;
+  return GPRX_OK;
+}
+
 int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
                           const gprx_kernel_program* kernel, const double* log_theta, size_t p,
                           int32_t precision, const gprx_jitter_policy* jitter,
@@ -439,6 +603,8 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
       return GPRX_UNSUPPORTED;
     }
     auto m = std::make_unique<ExactModel>();
+    m->hx.assign(x, x + n * d);
+    m->hy.assign(y, y + n);
     std::string why;
     int rc = parse_program(kernel, (int)d, &m->hp, &why);
     if (rc != GPRX_OK) {
@@ -478,32 +644,7 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
       GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * m->npad, hipMemcpyHostToDevice, s));
       GPRX_HIP_CHECK(hipStreamSynchronize(s));
     }
-    m->prog.alloc(1);
-    m->hprog.alloc(1);
-    const long np = m->npad;
-    m->logdiag.alloc(np);
-    m->alpha.alloc(np);
-    m->tv.alloc(np);
-    m->r.alloc(np);
-    m->dlt.alloc(np);
-    m->p1.alloc(np);
-    m->q1.alloc(np);
-    m->partials.alloc((size_t)lower_tiles(np) * kMaxAcc);
-    m->tpart.alloc((size_t)lower_tiles(np) * 4 * kTile);
-    m->accs.alloc(kMaxAcc);
-    m->scal.alloc(16);
-    m->info.alloc(1);
-    m->hscal.alloc(16);
-    m->hacc.alloc(kMaxAcc);
-    m->hinfo.alloc(1);
-    GPRX_HIP_CHECK(hipMemsetAsync(m->r.p, 0, sizeof(double) * np, s));
-    GPRX_HIP_CHECK(hipMemsetAsync(m->alpha.p, 0, sizeof(double) * np, s));
-    GPRX_HIP_CHECK(hipMemsetAsync(m->logdiag.p, 0, sizeof(double) * np, s));
-    GPRX_HIP_CHECK(hipMemsetAsync(m->scal.p, 0, sizeof(double) * 16, s));
-    if (precision == GPRX_MIXED)
-      m->alloc_mixed();
-    else
-      m->alloc_fp64();
+    size_buffers(m.get());
     GPRX_HIP_CHECK(hipStreamSynchronize(s));
     *out = m.release();
     return GPRX_OK;
@@ -551,6 +692,26 @@ int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_
     std::memcpy(grad, g.data(), sizeof(double) * p);
     std::memcpy(hess, h.data(), sizeof(double) * p * p);
     return GPRX_OK;
+  });
+}
+
+int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, double y_new,
+                          double* nlml, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !x_new) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null point");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    if (d != (size_t)m->d) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "point dimension differs from training");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    if (m->n >= (1 << 20)) {
+      set_error(err, GPRX_UNSUPPORTED, "exact GPR above 2^20 points");
+      return GPRX_UNSUPPORTED;
+    }
+    return exact_insert(m, x_new, y_new, nlml, err);
   });
 }
 

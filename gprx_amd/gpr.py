@@ -236,6 +236,18 @@ class FittedGpr:
         hess.reshape(-1)[:] = h
         return v.value
 
+    def insert(self, x_new, y_new: float) -> float:
+        """OnlineGpr::insert: appends one point and its target, updating the
+        FP64 factor in O(n^2); returns the new NLML."""
+        xn = L.f64(x_new)
+        v = C.c_double()
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_insert(self._h, L.dptr(xn), xn.size, float(y_new), C.byref(v),
+                                           C.byref(err))
+        raise_for(rc, err)
+        self.n += 1
+        return v.value
+
     def value(self, params) -> float:
         p = L.f64(params)
         val = C.c_double()

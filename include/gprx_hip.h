@@ -150,6 +150,15 @@ int gprx_hip_exact_value_and_grad(gprx_hip_model* model, const double* log_theta
 int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_t p,
                            double* nlml, double* grad, double* hess, gprx_error* err);
 
+/* Replaces OnlineGpr::insert (src/gpr/online.rs; the factor update of
+ * src/linalg/ldlt.rs) for an FP64 model: appends one point (x_new[d]) and
+ * its target, extends the factor and its inverse by one row in O(n^2) on
+ * the device, and writes the new NLML (nullable).  When the 128-row padding
+ * is full the model is re-padded and factored again.  GPRX_CHOLESKY_FAILED
+ * leaves the model as it was. */
+int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, double y_new,
+                          double* nlml, gprx_error* err);
+
 /* Replaces FittedGpr::predict_with (src/gpr/fitted.rs:854-960,
  * src/gpr/shared.rs:754-816) at the model's current parameters.  var may be
  * NULL (mean only). */

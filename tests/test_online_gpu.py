@@ -1,0 +1,85 @@
+"""Online exact GPR on the GPU: FittedGpr.insert (gprx_hip_exact_insert,
+OnlineGpr::insert in src/gpr/online.rs).  The reference's libgp goldens
+(tests/online_libgp.rs: from start_n points, insert the rest one by one and
+check the NLML, predictive mean and observation variance after each step,
+1e-8), and insertions across the 128-row padding against a model factored
+from scratch at every step."""
+import numpy as np
+import pytest
+
+import gprx_amd as g
+from conftest import load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def point(x, n, d, i):
+    return np.array([x[q * n + i] for q in range(d)])
+
+
+@pytest.mark.parametrize("name", ["online_libgp_forrester", "online_libgp_sphere"])
+def test_online_libgp_goldens(gpu, name):
+    gd = load_golden(name)
+    n, d, start = gd["n_rows"], gd["n_cols"], gd["start_n"]
+    x, y = np.asarray(gd["x"]), np.asarray(gd["y"])
+    xs, q = np.asarray(gd["xs"]), gd["xs_n_rows"]
+    spec = g.RbfArdKernel(gd["lengthscales"]) if gd["ard"] else g.RbfKernel(gd["lengthscales"][0])
+    x0 = np.concatenate([x[k * n:k * n + start] for k in range(d)])
+    print(f"[online] {name}: start {start}, {n - start} insertions", flush=True)
+    m = g.Gpr(spec, g.GaussianLikelihood(gd["noise_variance"])).factor(x0, start, d, y[:start])
+
+    def check(step, nlml):
+        assert step["n"] == m.n
+        assert rel_err(nlml, step["neg_log_marginal_likelihood"]) < 1e-8, (step["n"], nlml)
+        p = m.predict(xs, q, d)
+        assert rel_err(p.mean, step["mean"]) < 1e-8, step["n"]
+        assert rel_err(p.variance, step["observation_variance"]) < 1e-8, step["n"]
+
+    check(gd["steps"][0], m.neg_log_marginal_likelihood())
+    for k, i in enumerate(range(start, n), start=1):
+        nlml = m.insert(point(x, n, d, i), y[i])
+        check(gd["steps"][k], nlml)
+    assert k + 1 == len(gd["steps"])
+
+
+def test_insertions_across_the_padding_match_a_fresh_factor(gpu):
+    rng = np.random.default_rng(4)
+    n, d, start = 150, 3, 120
+    x = rng.uniform(0, 1, n * d)
+    y = np.sin(3 * x[:n]) + 0.1 * rng.standard_normal(n)
+    spec = g.ConstantKernel(1.2) * g.RbfArdKernel([0.4, 0.6, 0.8])
+    lik = g.GaussianLikelihood(0.05)
+    xs = rng.uniform(0, 1, 4 * d)
+
+    def cols(m_):
+        return np.concatenate([x[k * n:k * n + m_] for k in range(d)])
+
+    m = g.Gpr(spec, lik).factor(cols(start), start, d, y[:start])
+    print("[online] 120 -> 150 points across the 128-row padding", flush=True)
+    for i in range(start, n):
+        nlml = m.insert(point(x, n, d, i), y[i])
+        if i in (start, 126, 127, 128, 140, n - 1):
+            ref = g.Gpr(spec, lik).factor(cols(i + 1), i + 1, d, y[:i + 1])
+            assert rel_err(nlml, ref.neg_log_marginal_likelihood()) < 1e-10, i
+            a, b = m.predict(xs, 4, d), ref.predict(xs, 4, d)
+            assert rel_err(a.mean, b.mean) < 1e-9 and rel_err(a.variance, b.variance) < 1e-9, i
+            ref.close()
+    # the gradient at the grown model is the fresh model's too
+    th = m.get_params()
+    ga, gb = np.empty(th.size), np.empty(th.size)
+    va = m.value_and_gradient_into(th, ga)
+    ref = g.Gpr(spec, lik).prepare(cols(n), n, d, y)
+    vb = ref.value_and_gradient_into(th, gb)
+    assert rel_err(va, vb) < 1e-10 and rel_err(ga, gb) < 1e-10
+
+
+def test_insert_rejects_a_wrong_dimension_and_mixed_models(gpu):
+    x = np.linspace(0, 1, 20)
+    y = np.sin(4 * x)
+    m = g.Gpr(g.RbfKernel(0.3), g.GaussianLikelihood(0.1)).factor(x, 20, 1, y)
+    with pytest.raises(g.LengthMismatch):
+        m.insert([0.1, 0.2], 0.0)
+    mm = g.Gpr(g.RbfKernel(0.3), g.GaussianLikelihood(0.1)).with_precision(
+        g.Precision.MIXED).factor(x, 20, 1, y)
+    with pytest.raises(g.Unsupported):
+        mm.insert([0.5], 0.0)

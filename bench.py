@@ -468,6 +468,51 @@ def bench_hessian(args, gx):
     }
 
 
+def bench_online(args, gx):
+    """OnlineGpr::insert at config 2's kernel: from n = 16384 - K points,
+    K insertions inside the 128-row padding (FP64), each O(n^2)."""
+    from gprx_amd.configs import exact_ard
+
+    K = args.steps
+    wl = exact_ard(n=args.n)
+    n0 = wl.n - K - args.warmup
+    cols = np.concatenate([wl.x[q * wl.n:q * wl.n + n0] for q in range(wl.d)])
+    model = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).factor(cols, n0, wl.d, wl.y[:n0])
+
+    def pt(i):
+        return np.array([wl.x[q * wl.n + i] for q in range(wl.d)])
+
+    print(f"[bench] online: {n0} points, {args.warmup} + {K} insertions", file=sys.stderr,
+          flush=True)
+    for i in range(n0, n0 + args.warmup):
+        model.insert(pt(i), wl.y[i])
+    t0 = time.perf_counter()
+    for i in range(n0 + args.warmup, wl.n):
+        nlml = model.insert(pt(i), wl.y[i])
+    wall = time.perf_counter() - t0
+    ref = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).factor(wl.x, wl.n, wl.d, wl.y)
+    err = abs(nlml - ref.neg_log_marginal_likelihood()) / abs(ref.neg_log_marginal_likelihood())
+    ref.close()
+    model.close()
+    ms = wall / K * 1e3
+    n_avg = wl.n - K / 2
+    # three passes over the lower triangle of X (X k, X^T l, X^T z)
+    bytes_per = 3 * (n_avg * n_avg / 2) * 8
+    ach = bytes_per / (ms * 1e-3) / 1e9
+    return {
+        "metric": f"online insertions/s, exact GPR n~{wl.n} d={wl.d} Constant x RbfArd, FP64",
+        "value": 1e3 / ms, "unit": "insertions/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded SeededRng, seed 2)",
+        "config": {"workload": "exact GPR online insertion", "n_final": wl.n, "d": wl.d,
+                   "parallelism": "single GPU"},
+        "final_nlml_rel_err_vs_fresh_factor": err,
+        "roofline": {"bound": "hbm", "kernel": "one insertion (three triangular matrix-vector passes)",
+                     "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s", "frac": ach / PEAK_HBM,
+                     "traffic": None},
+    }
+
+
 def bench_copy(args, gx):
     """One 1 GiB device-to-device copy per step (torch's copy kernel): known
     HBM traffic of 1 GiB read + 1 GiB written per launch."""
@@ -497,7 +542,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "hessian", "copy"],
+    ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "hessian", "online",
+                                           "copy"],
                     default="exact")
     ap.add_argument("--restarts", type=int, default=16)
     ap.add_argument("--n", type=int, default=16384)
@@ -511,12 +557,13 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1, "hessian": 2,
-                      "copy": 5}[args.workload]
+                      "online": 50, "copy": 5}[args.workload]
 
     import gprx_amd as gx
 
     fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch,
-          "fit": bench_fit, "hessian": bench_hessian, "copy": bench_copy}[args.workload]
+          "fit": bench_fit, "hessian": bench_hessian, "online": bench_online,
+          "copy": bench_copy}[args.workload]
     print(json.dumps(fn(args, gx)))
 
 

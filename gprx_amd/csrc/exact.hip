@@ -64,6 +64,7 @@ struct ExactModel : gprx_hip_model {
   // device work vectors
   std::vector<double> hx, hy;
   DevBuf<double> zv, kcol, lv, wv, xnew, gpart1;
+  bool z_valid = false;  // zv = X y for the factor in X64
 
   DevPoints pts() const { return DevPoints{x.p, npad, n}; }
 
@@ -236,6 +237,7 @@ struct ExactModel : gprx_hip_model {
   // mixed_jitter_bound (or the policy's end) is formed again in FP64 at the
   // same jitter, and the loop goes on in FP64.
   int evaluate(const double* th, bool want_grad, double* nlml, double* grad, gprx_error* err) {
+    z_valid = false;  // X changes
     double j = 0.0, next = jit.initial;
     int remaining = jit.max_retries;
     bool fp64 = precision != GPRX_MIXED;
@@ -479,8 +481,10 @@ void regrow(ExactModel* m, long npad_new) {
 // k = K(X, x_new), kappa = k(x_new, x_new) + noise and l = L^-1 k = X k,
 // the new row of L is [l^T, d], d = sqrt(kappa - l^T l); the new row of
 // X = L^-1 is [-l^T X / d, 1 / d]; z = X y gains (y_new - l^T z) / d, and
-// alpha = X^T z, NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  When the
-// padding is full the model is re-padded by 128 rows and factored again.
+// alpha = X^T z, NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  z is kept
+// from one insertion to the next (recomputed after a full evaluation), so an
+// insertion reads X's lower triangle three times.  When the padding is full
+// the model is re-padded by 128 rows and factored again.
 int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_error* err) {
   if (m->precision != GPRX_FP64) {
     set_error(err, GPRX_UNSUPPORTED, "online insertion runs on FP64 models");
@@ -526,6 +530,7 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
     if (rc != GPRX_OK) return rc;
   }
   const long np = m->npad;
+  if (m->zv.n != (size_t)np) m->z_valid = false;
   m->zv.alloc(np);
   m->kcol.alloc((size_t)np * kTile);
   m->lv.alloc(np);
@@ -537,7 +542,7 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   GPRX_HIP_CHECK(hipMemcpy2DAsync(m->xnew.p, sizeof(double) * kTile, xn, sizeof(double),
                                   sizeof(double), d, hipMemcpyHostToDevice, s));
   const DevPoints pn{m->xnew.p, kTile, 1};
-  gemv_n<double>(m->X64.p, np, n, n, m->y.p, m->zv.p, m->gpart1.p, 1, 0, 0, s);
+  if (!m->z_valid) gemv_n<double>(m->X64.p, np, n, n, m->y.p, m->zv.p, m->gpart1.p, 1, 0, 0, s);
   fill_cross<double>(m->dp, m->prog.p, m->pts(), pn, m->kcol.p, np, s);
   fill_diag(m->dp, m->prog.p, pn, m->scal.p + 12, s);
   gemv_n<double>(m->X64.p, np, n, n, m->kcol.p, m->lv.p, m->gpart1.p, 1, 0, 0, s);
@@ -567,6 +572,7 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   m->n = n + 1;
   m->hx = hx_new;
   m->hy.push_back(yn);
+  m->z_valid = true;
   // alpha = X^T z and the NLML
   gemv_t<double>(m->X64.p, np, n + 1, n + 1, m->zv.p, m->alpha.p, 1, 0, 0, 0, s);
   dot_and_sum(m->zv.p, m->zv.p, m->logdiag.p, n + 1, m->scal.p, s);

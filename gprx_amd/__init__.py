@@ -5,7 +5,7 @@ ABI of include/gprx_hip.h); this package is the host-side mirror of gprx's
 model API over that ABI.  There is no CPU fallback: calls fail loudly when
 the library or a GPU is missing.
 """
-from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InvalidConfig,
+from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, InsufficientData, InvalidConfig,
                      InvalidHyperparameter, InvalidInterval, InvalidNoiseVariance, LengthMismatch,
                      NonFiniteInput, NonFiniteKernelValue, OptimizationNotConverged, Unsupported)
 from .gpr import (Fallback, FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision,
@@ -24,4 +24,5 @@ __all__ = [
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
     "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
     "Fixed", "Lbfgs", "OptResult", "OptimizationNotConverged", "InvalidConfig", "InvalidInterval",
+    "InsufficientData",
 ]

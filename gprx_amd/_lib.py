@@ -28,6 +28,7 @@ DEVICE_ERROR = 9
 NOT_CONVERGED = 10
 INVALID_CONFIG = 11
 INVALID_INTERVAL = 12
+INSUFFICIENT_DATA = 13
 
 FP64 = 0
 MIXED = 1
@@ -103,6 +104,7 @@ SIGNATURES = {
                                         _err]),
     "gprx_hip_exact_value_and_grad": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _err]),
     "gprx_hip_exact_hessian": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _dp, _err]),
+    "gprx_hip_exact_delete": (C.c_int, [_vp, C.c_size_t, _dp, _err]),
     "gprx_hip_exact_insert": (C.c_int, [_vp, _dp, C.c_size_t, C.c_double, _dp, _err]),
     "gprx_hip_exact_predict": (C.c_int, [_vp, _dp, C.c_size_t, C.c_size_t, C.c_int32, _dp, _dp,
                                          _err]),

@@ -701,6 +701,56 @@ int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_
   });
 }
 
+// OnlineGpr::delete (src/gpr/online.rs:564-620): drops the point at row
+// `index` and factors the model again (O(n^3); the reference downdates an
+// LDL^T factor, which this model, holding L^-1 rather than L, does not have).
+int exact_delete(ExactModel* m, size_t index, double* nlml, gprx_error* err) {
+  const int n = m->n, d = m->d;
+  std::vector<double> hx_new((size_t)(n - 1) * d);
+  for (int q = 0; q < d; ++q)
+    for (int i = 0, k = 0; i < n; ++i)
+      if ((size_t)i != index) hx_new[(size_t)q * (n - 1) + k++] = m->hx[(size_t)q * n + i];
+  std::vector<double> hx_old = m->hx, hy_old = m->hy;
+  const long npad_old = m->npad;
+  m->hx = hx_new;
+  m->hy.erase(m->hy.begin() + (long)index);
+  m->n = n - 1;
+  regrow(m, (long)round_up((size_t)(n - 1), kTile));
+  double v = 0.0;
+  int rc = m->evaluate(m->theta.data(), false, &v, nullptr, err);
+  if (rc != GPRX_OK) {  // back to the model before the deletion
+    m->hx = hx_old;
+    m->hy = hy_old;
+    m->n = n;
+    regrow(m, npad_old);
+    gprx_error e2{};
+    double v2 = 0.0;
+    (void)m->evaluate(m->theta.data(), false, &v2, nullptr, &e2);
+    return rc;
+  }
+  if (nlml) *nlml = v;
+  return GPRX_OK;
+}
+
+int gprx_hip_exact_delete(gprx_hip_model* model, size_t index, double* nlml, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    if (m->n <= 1) {
+      set_error(err, GPRX_INSUFFICIENT_DATA, "deleting would leave fewer than one point");
+      return GPRX_INSUFFICIENT_DATA;
+    }
+    if (index >= (size_t)m->n) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "no point at that index");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    return exact_delete(m, index, nlml, err);
+  });
+}
+
 int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, double y_new,
                           double* nlml, gprx_error* err) {
   return guarded(err, [&]() -> int {

@@ -60,10 +60,14 @@ class InvalidInterval(GprError):
     status = L.INVALID_INTERVAL
 
 
+class InsufficientData(GprError):
+    status = L.INSUFFICIENT_DATA
+
+
 _BY_STATUS = {c.status: c for c in (CholeskyFailed, NonFiniteKernelValue, NonFiniteInput,
                                       InvalidHyperparameter, InvalidNoiseVariance, LengthMismatch,
                                       EmptyInput, Unsupported, DeviceError, OptimizationNotConverged,
-                                      InvalidConfig, InvalidInterval)}
+                                      InvalidConfig, InvalidInterval, InsufficientData)}
 
 
 def raise_for(rc: int, err) -> None:

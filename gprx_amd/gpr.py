@@ -248,6 +248,16 @@ class FittedGpr:
         self.n += 1
         return v.value
 
+    def delete(self, index: int) -> float:
+        """OnlineGpr::delete: drops the point at row `index` (later rows move
+        up by one) and factors again; returns the new NLML."""
+        v = C.c_double()
+        err = L.GprxError()
+        rc = L.lib().gprx_hip_exact_delete(self._h, int(index), C.byref(v), C.byref(err))
+        raise_for(rc, err)
+        self.n -= 1
+        return v.value
+
     def value(self, params) -> float:
         p = L.f64(params)
         val = C.c_double()

@@ -43,7 +43,8 @@ typedef enum gprx_status {
   GPRX_DEVICE_ERROR = 9,             /* a HIP runtime failure */
   GPRX_NOT_CONVERGED = 10,           /* GprError::OptimizationNotConverged */
   GPRX_INVALID_CONFIG = 11,          /* GprError::InvalidConfig */
-  GPRX_INVALID_INTERVAL = 12         /* GprError::InvalidInterval (a start outside its bounds) */
+  GPRX_INVALID_INTERVAL = 12,        /* GprError::InvalidInterval (a start outside its bounds) */
+  GPRX_INSUFFICIENT_DATA = 13        /* GprError::InsufficientData (a deletion leaving < 1 point) */
 } gprx_status;
 
 /* CholeskyStage of GprError::CholeskyFailed (src/error.rs). */
@@ -158,6 +159,12 @@ int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_
  * leaves the model as it was. */
 int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, double y_new,
                           double* nlml, gprx_error* err);
+
+/* Replaces OnlineGpr::delete (src/gpr/online.rs:564-620): drops the point
+ * at row `index` (rows keep their order) and factors the model again at its
+ * parameters, O(n^3), writing the new NLML (nullable).  A model of one
+ * point refuses (GPRX_INSUFFICIENT_DATA). */
+int gprx_hip_exact_delete(gprx_hip_model* model, size_t index, double* nlml, gprx_error* err);
 
 /* Replaces FittedGpr::predict_with (src/gpr/fitted.rs:854-960,
  * src/gpr/shared.rs:754-816) at the model's current parameters.  var may be

@@ -83,3 +83,45 @@ def test_insert_rejects_a_wrong_dimension_and_mixed_models(gpu):
         g.Precision.MIXED).factor(x, 20, 1, y)
     with pytest.raises(g.Unsupported):
         mm.insert([0.5], 0.0)
+
+
+def test_deletions_match_a_fresh_factor_of_the_remaining_points(gpu):
+    rng = np.random.default_rng(9)
+    n, d = 140, 2
+    x = rng.uniform(0, 1, n * d)
+    y = np.cos(5 * x[:n]) + 0.1 * rng.standard_normal(n)
+    spec = g.RbfArdKernel([0.3, 0.5])
+    lik = g.GaussianLikelihood(0.02)
+    xs = rng.uniform(0, 1, 3 * d)
+    keep = list(range(n))
+    m = g.Gpr(spec, lik).factor(x, n, d, y)
+    print("[online] deletions from 140 points (across the 128-row padding) and an insertion", flush=True)
+    for idx in (0, 70, 11, len(keep) - 4, 5, 3, 2, 1, 0, 100, 50, 20, 7):
+        nlml = m.delete(idx)
+        keep.pop(idx)
+        k = len(keep)
+        xk = np.concatenate([x[q * n + np.array(keep)] for q in range(d)])
+        ref = g.Gpr(spec, lik).factor(xk, k, d, y[keep])
+        assert m.n == k
+        assert rel_err(nlml, ref.neg_log_marginal_likelihood()) < 1e-10, idx
+        a, b = m.predict(xs, 3, d), ref.predict(xs, 3, d)
+        assert rel_err(a.mean, b.mean) < 1e-9 and rel_err(a.variance, b.variance) < 1e-9, idx
+        ref.close()
+    # an insertion after deletions extends the refactored model
+    nlml = m.insert(point(x, n, d, 70), y[70])
+    keep.append(70)
+    xk = np.concatenate([x[q * n + np.array(keep)] for q in range(d)])
+    ref = g.Gpr(spec, lik).factor(xk, len(keep), d, y[keep])
+    assert rel_err(nlml, ref.neg_log_marginal_likelihood()) < 1e-10
+
+
+def test_delete_refuses_the_last_point_and_bad_indices(gpu):
+    x = np.array([0.1, 0.7])
+    y = np.array([0.3, -0.2])
+    m = g.Gpr(g.RbfKernel(0.5), g.GaussianLikelihood(0.1)).factor(x, 2, 1, y)
+    with pytest.raises(g.LengthMismatch):
+        m.delete(2)
+    m.delete(0)
+    assert m.n == 1
+    with pytest.raises(g.InsufficientData):
+        m.delete(0)

@@ -450,11 +450,44 @@ void size_buffers(ExactModel* m) {
     m->alloc_fp64();
 }
 
-// Row n of X (the new point's): X[n][j] = -w_j / d for j < n, X[n][n] = 1 / d.
-__global__ void k_insert_row(double* X, long ld, int n, const double* __restrict__ w, double inv_d) {
+// Scalars of one insertion from sc[8] = l^T l, sc[10] = l^T z and
+// sc[12] = k(x_new, x_new): sc[13] = 1 / d, sc[14] = z_n, sc[15] = 1 when
+// the pivot d^2 = kappa - l^T l is positive (0 otherwise, and nothing below
+// changes the model); z[n] = z_n and logdiag[n] = log d.
+__global__ void k_insert_scalars(double* sc, double noise, double y_new, double* z, double* logdiag,
+                                 int n) {
+  const double piv2 = sc[12] + noise - sc[8];
+  const bool ok = piv2 > 0.0 && isfinite(piv2);
+  const double d = ok ? sqrt(piv2) : 1.0;
+  sc[13] = 1.0 / d;
+  sc[14] = ok ? (y_new - sc[10]) / d : 0.0;
+  sc[15] = ok ? 1.0 : 0.0;
+  if (ok) {
+    z[n] = sc[14];
+    logdiag[n] = log(d);
+  }
+}
+
+// Row n of X (the new point's): X[n][j] = -w_j / d for j < n, X[n][n] = 1 / d
+// (1 / d = sc[13]; only when sc[15] says the pivot held).
+__global__ void k_insert_row(double* X, long ld, int n, const double* __restrict__ w,
+                             const double* __restrict__ sc) {
+  if (sc[15] == 0.0) return;
+  const double inv_d = sc[13];
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) X[n + (long)j * ld] = -w[j] * inv_d;
   if (j == n) X[n + (long)n * ld] = inv_d;
+}
+
+// alpha after a one-row extension of X: alpha_j -= c w_j (j < n), alpha_n = c,
+// with c = z_n / d and w = X^T l (the new row of X is -w / d, 1 / d).
+__global__ void k_alpha_extend(double* alpha, const double* __restrict__ w, int n,
+                               const double* __restrict__ sc) {
+  if (sc[15] == 0.0) return;
+  const double c = sc[14] * sc[13];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) alpha[j] -= c * w[j];
+  if (j == n) alpha[n] = c;
 }
 
 // Re-pads the model for n_new points: every npad-sized buffer again, the
@@ -481,10 +514,11 @@ void regrow(ExactModel* m, long npad_new) {
 // k = K(X, x_new), kappa = k(x_new, x_new) + noise and l = L^-1 k = X k,
 // the new row of L is [l^T, d], d = sqrt(kappa - l^T l); the new row of
 // X = L^-1 is [-l^T X / d, 1 / d]; z = X y gains (y_new - l^T z) / d, and
-// alpha = X^T z, NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  z is kept
-// from one insertion to the next (recomputed after a full evaluation), so an
-// insertion reads X's lower triangle three times.  When the padding is full
-// the model is re-padded by 128 rows and factored again.
+// alpha = X^T z = alpha_old - (z_n / d) X^T l (plus alpha_n = z_n / d),
+// NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  z is kept from one
+// insertion to the next (recomputed after a full evaluation), so an
+// insertion reads X's lower triangle twice (X k, X^T l).  When the padding
+// is full the model is re-padded by 128 rows and factored again.
 int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_error* err) {
   if (m->precision != GPRX_FP64) {
     set_error(err, GPRX_UNSUPPORTED, "online insertion runs on FP64 models");
@@ -548,36 +582,43 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   gemv_n<double>(m->X64.p, np, n, n, m->kcol.p, m->lv.p, m->gpart1.p, 1, 0, 0, s);
   dot_and_sum(m->lv.p, m->lv.p, m->lv.p, n, m->scal.p + 8, s);
   dot_and_sum(m->lv.p, m->zv.p, m->lv.p, n, m->scal.p + 10, s);
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
-  GPRX_HIP_CHECK(hipStreamSynchronize(s));
-  const double kappa = m->hscal.p[12] + std::exp(th[m->hp.n_params]) + m->last_jitter;
-  const double piv2 = kappa - m->hscal.p[8];
-  if (!(piv2 > 0.0) || !std::isfinite(piv2)) {
-    set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky factor failed on insertion", GPRX_STAGE_FIT,
-              m->last_jitter, (size_t)(n + 1));
-    return GPRX_CHOLESKY_FAILED;
-  }
-  const double dd = std::sqrt(piv2);
-  const double zn = (yn - m->hscal.p[10]) / dd, ld = std::log(dd);
-  // X's new row, the point, the target, z_n and log L_nn
+  // the pivot, z_n and log d on the device; every update below is skipped
+  // there when the pivot fails, so the host reads back once
+  const double noise = std::exp(th[m->hp.n_params]) + m->last_jitter;
+  hipLaunchKernelGGL(k_insert_scalars, dim3(1), dim3(1), 0, s, m->scal.p, noise, yn, m->zv.p,
+                     m->logdiag.p, n);
+  GPRX_HIP_CHECK(hipGetLastError());
+  // X's new row, then alpha = X^T z: the old alpha (= X^T z of the old
+  // rows) plus the new row's term, from w = X^T l; the NLML sums
   gemv_t<double>(m->X64.p, np, n, n, m->lv.p, m->wv.p, 1, 0, 0, 0, s);
   hipLaunchKernelGGL(k_insert_row, dim3((n + 1 + 255) / 256), dim3(256), 0, s, m->X64.p, np, n,
-                     m->wv.p, 1.0 / dd);
+                     m->wv.p, m->scal.p);
+  GPRX_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_alpha_extend, dim3((n + 1 + 255) / 256), dim3(256), 0, s, m->alpha.p,
+                     m->wv.p, n, m->scal.p);
   GPRX_HIP_CHECK(hipGetLastError());
   GPRX_HIP_CHECK(hipMemcpy2DAsync(m->x.p + n, sizeof(double) * np, xn, sizeof(double),
                                   sizeof(double), d, hipMemcpyHostToDevice, s));
   GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p + n, &yn, sizeof(double), hipMemcpyHostToDevice, s));
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->zv.p + n, &zn, sizeof(double), hipMemcpyHostToDevice, s));
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->logdiag.p + n, &ld, sizeof(double), hipMemcpyHostToDevice, s));
+  dot_and_sum(m->zv.p, m->zv.p, m->logdiag.p, n + 1, m->scal.p, s);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  if (m->hscal.p[15] == 0.0) {
+    // the point's row of x and its target lie in the padding: clear them
+    const std::vector<double> zero((size_t)d, 0.0);
+    GPRX_HIP_CHECK(hipMemcpy2DAsync(m->x.p + n, sizeof(double) * np, zero.data(), sizeof(double),
+                                    sizeof(double), d, hipMemcpyHostToDevice, s));
+    const double z0 = 0.0;
+    GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p + n, &z0, sizeof(double), hipMemcpyHostToDevice, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    set_error(err, GPRX_CHOLESKY_FAILED, "Cholesky factor failed on insertion", GPRX_STAGE_FIT,
+              m->last_jitter, (size_t)(n + 1));
+    return GPRX_CHOLESKY_FAILED;
+  }
   m->n = n + 1;
   m->hx = hx_new;
   m->hy.push_back(yn);
   m->z_valid = true;
-  // alpha = X^T z and the NLML
-  gemv_t<double>(m->X64.p, np, n + 1, n + 1, m->zv.p, m->alpha.p, 1, 0, 0, 0, s);
-  dot_and_sum(m->zv.p, m->zv.p, m->logdiag.p, n + 1, m->scal.p, s);
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
-  GPRX_HIP_CHECK(hipStreamSynchronize(s));
   if (nlml) *nlml = 0.5 * m->hscal.p[0] + m->hscal.p[1] + 0.5 * (double)(n + 1) * kLog2Pi;
   return GPRX_OK;
 }

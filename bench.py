@@ -553,7 +553,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this path runs on one GPU; exact GPR at n=16384 does not shard "
-              "across GPUs here (see DESIGN.md, sharding).", file=sys.stderr)
+              "across GPUs in this backend; run it on one GPU.", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
         args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1, "hessian": 2,

@@ -11,7 +11,7 @@ from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, Insuffic
 from .gpr import (Fallback, FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision,
                   Prediction, VarianceKind)
 from .batch import GprBatch
-from .optimizer import Fixed, Lbfgs, OptResult
+from .optimizer import Fixed, Lbfgs, OptResult, TrustRegion
 from .sgpr import FittedSgpr, Sgpr
 from .svgp import FittedSvgp, Svgp, prior_params
 from .kernels import (ConstantKernel, KernelSpec, PeriodicKernel, Product, RbfArdKernel, RbfKernel,
@@ -23,6 +23,6 @@ __all__ = [
     "Unsupported", "Fallback", "FittedGpr", "GaussianLikelihood", "Gpr", "JitterPolicy", "Precision",
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
     "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
-    "Fixed", "Lbfgs", "OptResult", "OptimizationNotConverged", "InvalidConfig", "InvalidInterval",
+    "Fixed", "Lbfgs", "OptResult", "TrustRegion", "OptimizationNotConverged", "InvalidConfig", "InvalidInterval",
     "InsufficientData",
 ]

@@ -92,6 +92,19 @@ class LbfgsOptionsC(C.Structure):
 
 # gprx_objective_fn: int (*)(void* ctx, const double* x, size_t n, double* value, double* grad)
 ObjectiveFn = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, C.c_size_t, _dp, _dp)
+# gprx_objective2_fn: the same with the Hessian (double* hess) last
+Objective2Fn = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, C.c_size_t, _dp, _dp, _dp)
+
+
+class TrOptionsC(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_uint64),
+        ("tolerance", C.c_double),
+        ("initial_radius", C.c_double),
+        ("max_radius", C.c_double),
+        ("restarts", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
 
 _err = C.POINTER(GprxError)
 _mp = C.POINTER(C.c_void_p)
@@ -141,6 +154,12 @@ SIGNATURES = {
     "gprx_hip_exact_fit": (C.c_int, [_dp, C.c_size_t, C.c_size_t, _dp, C.POINTER(KernelProgram), _dp,
                                      C.c_size_t, _dp, C.c_int32, C.POINTER(LbfgsOptionsC), _dp, _dp,
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_int32), _err]),
+    "gprx_tr_default_options": (None, [C.POINTER(TrOptionsC)]),
+    "gprx_trust_region_minimize": (C.c_int, [Objective2Fn, _vp, _dp, C.c_size_t, C.POINTER(TrOptionsC),
+                                             _dp, _dp, C.POINTER(C.c_uint64), _err]),
+    "gprx_hip_exact_fit_trust_region": (C.c_int, [_vp, _dp, C.c_size_t, _dp, C.POINTER(TrOptionsC), _dp,
+                                                  _dp, C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                                  _err]),
     "gprx_hip_destroy": (None, [_vp]),
     "gprx_hip_last_eval_ms": (C.c_double, [_vp]),
     "gprx_hip_last_dense_ms": (C.c_double, [_vp]),

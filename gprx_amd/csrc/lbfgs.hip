@@ -29,9 +29,11 @@
 #include <vector>
 
 #include "../../include/gprx_hip.h"
-#include "rng.hpp"
+#include "logit.hpp"
 
 namespace {
+
+using namespace gprx;
 
 constexpr double kBarrierCost = 1.0e300;  // src/optimizer/adapter.rs:23
 constexpr double kEps = 2.220446049250313e-16;
@@ -42,62 +44,6 @@ void put_error(gprx_error* err, int status, const char* msg) {
   std::memset(err, 0, sizeof(*err));
   err->status = status;
   std::strncpy(err->message, msg, sizeof(err->message) - 1);
-}
-
-// ------------------------------------------------------------- logit map
-// One parameter's interval in user units: positive intervals (lo > 0) map
-// log theta, others theta itself (src/optimizer/logit.rs:75-265).
-struct Interval {
-  double lo, hi;
-  bool positive() const { return lo > 0.0; }
-  double span() const { return positive() ? std::log(hi) - std::log(lo) : hi - lo; }
-  double scale() const { return span() / 4.0; }
-  bool contains(double v) const { return std::isfinite(v) && v > lo && v < hi; }
-};
-
-double sigmoid(double z) {
-  if (z >= 0.0) {
-    const double e = std::exp(-z);
-    return 1.0 / (1.0 + e);
-  }
-  const double e = std::exp(z);
-  return e / (1.0 + e);
-}
-
-// The model parameter of z: log theta on a positive interval, theta otherwise.
-double z_to_param(double z, const Interval& iv) {
-  const double u = sigmoid(z / iv.scale());
-  if (iv.positive()) {
-    const double ln_lo = std::log(iv.lo), ln_hi = std::log(iv.hi);
-    return ln_lo + u * (ln_hi - ln_lo);
-  }
-  return iv.lo + u * (iv.hi - iv.lo);
-}
-
-// d param / dz
-double dparam_dz(double z, const Interval& iv) {
-  const double s = sigmoid(z / iv.scale());
-  return iv.span() * s * (1.0 - s) / iv.scale();
-}
-
-// z of a user value x inside the interval (false outside)
-bool user_to_z(double x, const Interval& iv, double* z) {
-  if (!iv.contains(x)) return false;
-  double u = iv.positive() ? (std::log(x) - std::log(iv.lo)) / (std::log(iv.hi) - std::log(iv.lo))
-                           : (x - iv.lo) / (iv.hi - iv.lo);
-  u = std::fmin(std::fmax(u, kEps), 1.0 - kEps);
-  *z = std::log(u / (1.0 - u)) * iv.scale();
-  return true;
-}
-
-// Log-uniform (positive) or uniform draw strictly inside the interval.
-double sample_user(gprx::Rng& rng, const Interval& iv) {
-  const double u = rng.open_unit();
-  if (iv.positive()) {
-    const double ln_lo = std::log(iv.lo), ln_hi = std::log(iv.hi);
-    return std::exp(ln_lo + u * (ln_hi - ln_lo));
-  }
-  return iv.lo + u * (iv.hi - iv.lo);
 }
 
 // ------------------------------------------------------- Moré–Thuente
@@ -492,7 +438,7 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
       return GPRX_INVALID_INTERVAL;
     }
   }
-  gprx::Rng rng(o.seed);
+  Rng rng(o.seed);
   for (int r = 1; r < k; ++r)
     for (int i = 0; i < np; ++i) user_to_z(sample_user(rng, iv[i]), iv[i], &z0[r][i]);
 

@@ -93,7 +93,7 @@ class Gpr:
         self.bounds = None
 
     def with_optimizer(self, optimizer) -> "Gpr":
-        """Lbfgs (with or without restarts) or Fixed, used by fit()."""
+        """Lbfgs or TrustRegion (with or without restarts), or Fixed, used by fit()."""
         self.optimizer = optimizer
         return self
 
@@ -107,10 +107,29 @@ class Gpr:
         """Gpr::fit (src/gpr/trainer.rs): minimizes the NLML with the
         optimizer (default Lbfgs()), then factors at the best parameters.
         The fitted model's fit_result holds the optimizer's OptResult."""
-        from .optimizer import Fixed, Lbfgs, OptResult
+        from .optimizer import Fixed, Lbfgs, OptResult, TrustRegion
         opt = self.optimizer if self.optimizer is not None else Lbfgs()
         if isinstance(opt, Fixed):
             return self.factor(x, n, d, y)
+        if isinstance(opt, TrustRegion):
+            fitted = self.prepare(x, n, d, y)
+            th0 = fitted.get_params()
+            if self.bounds is not None and self.bounds.size != 2 * th0.size:
+                raise_for(L.LENGTH_MISMATCH, "bounds must hold (lo, hi) for every parameter")
+            best = np.empty(th0.size)
+            val = C.c_double()
+            it = C.c_uint64()
+            start = C.c_int32()
+            err = L.GprxError()
+            oc = opt.options_c()
+            bptr = L.dptr(self.bounds) if self.bounds is not None else None
+            rc = L.lib().gprx_hip_exact_fit_trust_region(fitted._h, L.dptr(th0), th0.size, bptr,
+                                                         C.byref(oc), L.dptr(best), C.byref(val),
+                                                         C.byref(it), C.byref(start),
+                                                         C.byref(err))
+            raise_for(rc, err)
+            fitted.fit_result = OptResult(best.copy(), val.value, it.value, start.value)
+            return fitted
         x = L.f64(x)
         y = L.f64(y)
         if x.size != n * d or y.size != n:

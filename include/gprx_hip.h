@@ -338,6 +338,43 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
                        double* best_log_theta, double* best_value, uint64_t* iterations,
                        int32_t* best_start, gprx_error* err);
 
+/* TrustRegion settings (src/optimizer/trust_region.rs:19-80): 100
+ * iterations, gradient tolerance sqrt(eps), radius 1, largest radius 100;
+ * with_restarts(n, seed) as for Lbfgs. */
+typedef struct gprx_tr_options {
+  uint64_t max_iterations;
+  double tolerance;
+  double initial_radius;
+  double max_radius;
+  uint32_t restarts;
+  uint64_t seed;
+} gprx_tr_options;
+
+/* Fills the defaults of TrustRegion::new(), no restarts. */
+void gprx_tr_default_options(gprx_tr_options* opt);
+
+/* An objective with its Hessian on the host: value, gradient[n] and the
+ * n x n Hessian (row-major) at x; nonzero (or non-finite) = failed point. */
+typedef int (*gprx_objective2_fn)(void* ctx, const double* x, size_t n, double* value,
+                                  double* grad, double* hess);
+
+/* Replaces the trust-region run (src/optimizer/trust_region.rs:107-148,
+ * argmin 0.11's TrustRegion + Steihaug restated) over a host callback.
+ * Host only. */
+int gprx_trust_region_minimize(gprx_objective2_fn fn, void* ctx, const double* x0, size_t n,
+                               const gprx_tr_options* opt, double* x_out, double* value,
+                               uint64_t* iterations, gprx_error* err);
+
+/* Replaces Gpr::with_optimizer(TrustRegion::..).fit for an exact model:
+ * trust-region steps on the NLML through the logit map of `bounds` (as
+ * gprx_hip_exact_fit), each point's Hessian formed on the GPU
+ * (gprx_hip_exact_hessian), from log_theta and then each restart draw in
+ * turn.  The model is left at the best point. */
+int gprx_hip_exact_fit_trust_region(gprx_hip_model* model, const double* log_theta, size_t p,
+                                    const double* bounds, const gprx_tr_options* opt,
+                                    double* best_log_theta, double* best_value,
+                                    uint64_t* iterations, int32_t* best_start, gprx_error* err);
+
 /* ------------------------------------------------------------------ all */
 
 /* Frees a handle of any kind (NULL is ignored). */

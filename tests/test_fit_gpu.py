@@ -138,3 +138,39 @@ def test_fit_goldens(gpu, name):
 def test_start_outside_bounds_is_an_interval_error(gpu):
     with pytest.raises(g.InvalidInterval):
         rbf_gpr(2.0, 0.2).with_bounds([0.1, 1.0, 1e-5, 1e5]).fit(X4, 4, 1, Y4)
+
+
+def test_trust_region_fit_reaches_the_lbfgs_optimum(gpu):
+    """TrustRegion (each point's Hessian from the GPU) and Lbfgs from the
+    same start land on the same NLML."""
+    rng = np.random.default_rng(12)
+    n, d = 160, 2
+    x = rng.uniform(0, 1, n * d)
+    y = np.sin(4 * x[:n]) * np.cos(3 * x[n:]) + 0.05 * rng.standard_normal(n)
+    spec = g.ConstantKernel(1.0) * g.RbfArdKernel([0.5, 0.5])
+    print("[fit] trust region vs L-BFGS", flush=True)
+    tr = g.Gpr(spec, g.GaussianLikelihood(0.05)).with_optimizer(g.TrustRegion()).fit(x, n, d, y)
+    lb = g.Gpr(spec, g.GaussianLikelihood(0.05)).with_optimizer(
+        g.Lbfgs().with_max_iterations(200)).fit(x, n, d, y)
+    a, b = tr.neg_log_marginal_likelihood(), lb.neg_log_marginal_likelihood()
+    print(f"[fit] trust region {a} in {tr.fit_result.iterations} iterations, L-BFGS {b}", flush=True)
+    assert abs(a - b) <= 1e-7 * max(1.0, abs(b))
+    assert a == pytest.approx(tr.fit_result.value, rel=1e-12)
+
+
+@pytest.mark.parametrize("name", ["forrester_rbf", "sphere_rbf_ard"])
+def test_trust_region_fit_goldens(gpu, name):
+    gd = load_golden(name)
+    n, d = gd["n_rows"], gd["n_cols"]
+    y = np.asarray(gd["y"])
+    yn = (y - y.mean()) / y.std()
+    ls0 = gd["lengthscales_init"]
+    spec = g.RbfArdKernel(ls0) if gd["kernel"] == "rbf_ard" else g.RbfKernel(ls0[0])
+    f = g.Gpr(spec, g.GaussianLikelihood(gd["noise_variance_init"])).with_optimizer(
+        g.TrustRegion()).fit(gd["x"], n, d, yn)
+    lml = -f.neg_log_marginal_likelihood()
+    print(f"[fit] trust region {name}: lml {lml} against sklearn's {gd['log_marginal_likelihood']}",
+          flush=True)
+    # a local method from sklearn's start: at sklearn's optimum or a better one
+    assert lml >= gd["log_marginal_likelihood"] - 1e-3 * abs(gd["log_marginal_likelihood"])
+    assert np.all(np.isfinite(f.get_params()))

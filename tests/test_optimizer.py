@@ -94,3 +94,51 @@ def test_invalid_settings_are_config_errors():
         g.Lbfgs().with_tolerance(math.inf)
     with pytest.raises(g.InvalidConfig):
         g.Lbfgs().with_history_size(0)
+
+
+def test_trust_region_quadratic_bowl_in_one_step():
+    # a strictly convex quadratic inside the region: Newton's step, then |g| = 0
+    A = np.array([[3.0, 1.0], [1.0, 2.0]])
+    b = np.array([1.0, -1.0])
+
+    def q(x):
+        return 0.5 * x @ A @ x - b @ x, A @ x - b, A
+
+    r = g.TrustRegion().minimize_unconstrained(q, [0.2, 0.1])
+    assert np.max(np.abs(r.params - np.linalg.solve(A, b))) < 1e-12
+    assert r.iterations <= 2
+
+
+def test_trust_region_rosenbrock_with_its_hessian():
+    def rosen(x):
+        a, bb = x
+        f = (1 - a) ** 2 + 100 * (bb - a * a) ** 2
+        gr = np.array([-2 * (1 - a) - 400 * a * (bb - a * a), 200 * (bb - a * a)])
+        H = np.array([[2 - 400 * (bb - a * a) + 800 * a * a, -400 * a], [-400 * a, 200.0]])
+        return f, gr, H
+
+    r = g.TrustRegion().with_tolerance(1e-10).minimize_unconstrained(rosen, [-1.2, 1.0])
+    assert np.max(np.abs(r.params - 1.0)) < 1e-8, r
+    assert r.iterations < 60
+
+
+def test_trust_region_leaves_a_saddle_along_negative_curvature():
+    # f = x0^2 - x1^2 + x1^4 / 4: a saddle at 0, minima at x1 = +-sqrt(2)
+    def f(x):
+        v = x[0] ** 2 - x[1] ** 2 + x[1] ** 4 / 4
+        gr = np.array([2 * x[0], -2 * x[1] + x[1] ** 3])
+        H = np.array([[2.0, 0.0], [0.0, -2.0 + 3 * x[1] ** 2]])
+        return v, gr, H
+
+    r = g.TrustRegion().minimize_unconstrained(f, [0.5, 1e-3])
+    assert abs(abs(r.params[1]) - math.sqrt(2)) < 1e-8 and abs(r.params[0]) < 1e-8
+
+
+def test_trust_region_zero_iterations_and_bad_settings():
+    r = g.TrustRegion().with_max_iterations(0).minimize_unconstrained(
+        lambda x: (float(x @ x), 2 * x, 2 * np.eye(2)), [0.3, -0.7])
+    assert list(r.params) == [0.3, -0.7] and r.iterations == 0
+    with pytest.raises(g.InvalidConfig):
+        g.TrustRegion().with_radii(2.0, 1.0)
+    with pytest.raises(g.InvalidConfig):
+        g.TrustRegion().with_tolerance(-1.0)

@@ -13,7 +13,8 @@
 //   - the Moré–Thuente line search (Moré & Thuente, ACM TOMS 20, 1994: the
 //     safeguarded step of their dcstep and the interval logic of dcsrch),
 //     c1 = 1e-4, c2 = 0.9, relative width tolerance 1e-10, steps in
-//     [sqrt(eps), inf), extrapolation bounds 1.1 and 4, first trial step 1;
+//     [sqrt(eps), inf), extrapolation bounds 1.1 and 4, first trial step 1,
+//     at most 20 evaluations (then the last trial is taken);
 //   - stopping when |g| < tolerance, when the cost changes by less than
 //     eps between iterations, after max_iterations iterations, or when the
 //     direction is not one of descent; the result is the best iterate.

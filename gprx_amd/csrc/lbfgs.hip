@@ -22,7 +22,7 @@
 // one point and is told its cost and gradient.  The exact-GPR driver asks
 // all live starts, evaluates their points in one batched call and tells
 // each its result, so a round costs one batched evaluation however many
-// starts there are.
+// starts there are; the batch shrinks to the live starts as they finish.
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -443,48 +443,76 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
   for (int r = 1; r < k; ++r)
     for (int i = 0; i < np; ++i) user_to_z(sample_user(rng, iv[i]), iv[i], &z0[r][i]);
 
-  // One batch problem per start, all on the same data.
-  std::vector<gprx_problem> probs(k);
-  for (int r = 0; r < k; ++r) probs[r] = gprx_problem{x, n, d, y, kernel, log_theta, p};
-  gprx_hip_model* model = nullptr;
-  int rc = gprx_hip_batch_create(probs.data(), (size_t)k, precision, &model, err);
-  if (rc != GPRX_OK) return rc;
-  std::unique_ptr<gprx_hip_model, void (*)(gprx_hip_model*)> hold(model, gprx_hip_destroy);
-
   std::vector<LbfgsRun> runs;
   runs.reserve(k);
   for (int r = 0; r < k; ++r) runs.emplace_back(z0[r].data(), np, o);
+  // The batch holds one problem (the same data) per start in `members`.
+  // When the live starts fall to half the batch, it is made again over
+  // them alone, with the launch schedule the first batch tuned, so the
+  // long tail of a few slow starts does not pay for the finished ones.
+  std::vector<int> members;
+  gprx_hip_model* model = nullptr;
+  std::unique_ptr<gprx_hip_model, void (*)(gprx_hip_model*)> hold(nullptr, gprx_hip_destroy);
+  int sched_policy = -1, sched_small = 0;
+  auto make_batch = [&](const std::vector<int>& who) -> int {
+    std::vector<gprx_problem> probs(who.size(), gprx_problem{x, n, d, y, kernel, log_theta, p});
+    gprx_hip_model* mnew = nullptr;
+    int rc2 = gprx_hip_batch_create(probs.data(), probs.size(), precision, &mnew, err);
+    if (rc2 != GPRX_OK) return rc2;
+    hold.reset(mnew);
+    model = mnew;
+    members = who;
+    if (sched_policy >= 0) {
+      rc2 = gprx_hip_batch_set_schedule(model, sched_policy, sched_small, err);
+      if (rc2 != GPRX_OK) return rc2;
+    }
+    return GPRX_OK;
+  };
+  std::vector<int> all(k);
+  for (int r = 0; r < k; ++r) all[r] = r;
+  int rc = make_batch(all);
+  if (rc != GPRX_OK) return rc;
   std::vector<std::vector<double>> th(k, std::vector<double>(np)), gr(k, std::vector<double>(np));
   std::vector<const double*> thp(k);
   std::vector<double*> grp(k);
   std::vector<double> nlml(k);
   std::vector<gprx_error> errs(k);
-  for (int r = 0; r < k; ++r) {
-    thp[r] = th[r].data();
-    grp[r] = gr[r].data();
-  }
   for (;;) {
-    bool live = false;
-    for (int r = 0; r < k; ++r) {
-      // a finished start is evaluated again at its best point (its result
-      // unused): the batch evaluates every problem each call
-      const double* z = runs[r].done() ? runs[r].best_x.data() : runs[r].ask();
-      live |= !runs[r].done();
-      for (int i = 0; i < np; ++i) th[r][i] = z_to_param(z[i], iv[i]);
+    std::vector<int> live;
+    for (int r = 0; r < k; ++r)
+      if (!runs[r].done()) live.push_back(r);
+    if (live.empty()) break;
+    if (members.size() > 1 && live.size() * 2 <= members.size()) {
+      if (sched_policy < 0) {
+        double tuned = 0.0;
+        (void)gprx_hip_batch_schedule(model, &sched_policy, &sched_small, &tuned);
+      }
+      rc = make_batch(live);
+      if (rc != GPRX_OK) return rc;
     }
-    if (!live) break;
+    const int kb = (int)members.size();
+    for (int s = 0; s < kb; ++s) {
+      // a member that has finished is evaluated again at its best point
+      // (the result unused): a batch evaluates all of its problems
+      const LbfgsRun& run = runs[members[s]];
+      const double* z = run.done() ? run.best_x.data() : run.ask();
+      for (int i = 0; i < np; ++i) th[s][i] = z_to_param(z[i], iv[i]);
+      thp[s] = th[s].data();
+      grp[s] = gr[s].data();
+    }
     rc = gprx_hip_batch_value_and_grad(model, thp.data(), nlml.data(), grp.data(), errs.data());
     if (rc != GPRX_OK) {
       put_error(err, rc, errs[0].message[0] ? errs[0].message : "batched evaluation failed");
       return rc;
     }
-    for (int r = 0; r < k; ++r) {
-      if (runs[r].done()) continue;
-      const double* z = runs[r].ask();
-      for (int i = 0; i < np; ++i) gr[r][i] *= dparam_dz(z[i], iv[i]);
-      double f = nlml[r];
-      barrier_if_bad(errs[r].status, &f, gr[r].data(), np);
-      runs[r].tell(f, gr[r].data());
+    for (int s = 0; s < kb; ++s) {
+      LbfgsRun& run = runs[members[s]];
+      if (run.done()) continue;
+      const double* z = run.ask();
+      for (int i = 0; i < np; ++i) gr[s][i] *= dparam_dz(z[i], iv[i]);
+      double f = nlml[s];
+      barrier_if_bad(errs[s].status, &f, gr[s].data(), np);
+      run.tell(f, gr[s].data());
     }
   }
   // keep_better over the runs in order; the first start's failure is the

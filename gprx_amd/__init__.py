@@ -11,6 +11,7 @@ from .errors import (CholeskyFailed, DeviceError, EmptyInput, GprError, Insuffic
 from .gpr import (Fallback, FittedGpr, GaussianLikelihood, Gpr, JitterPolicy, Precision,
                   Prediction, VarianceKind)
 from .batch import GprBatch
+from .persist import FORMAT_VERSION, PersistFailed, UnsupportedPersistVersion
 from .optimizer import Fixed, Lbfgs, OptResult, TrustRegion
 from .sgpr import FittedSgpr, Sgpr
 from .svgp import FittedSvgp, Svgp, prior_params
@@ -24,5 +25,5 @@ __all__ = [
     "Prediction", "VarianceKind", "ConstantKernel", "KernelSpec", "PeriodicKernel", "Product",
     "RbfArdKernel", "RbfKernel", "Sum", "Sgpr", "FittedSgpr", "GprBatch", "Svgp", "FittedSvgp", "prior_params",
     "Fixed", "Lbfgs", "OptResult", "TrustRegion", "OptimizationNotConverged", "InvalidConfig", "InvalidInterval",
-    "InsufficientData",
+    "InsufficientData", "FORMAT_VERSION", "PersistFailed", "UnsupportedPersistVersion",
 ]

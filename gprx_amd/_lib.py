@@ -118,6 +118,7 @@ SIGNATURES = {
     "gprx_hip_exact_value_and_grad": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _err]),
     "gprx_hip_exact_hessian": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _dp, _err]),
     "gprx_hip_exact_delete": (C.c_int, [_vp, C.c_size_t, _dp, _err]),
+    "gprx_hip_exact_export_factor": (C.c_int, [_vp, _dp, C.c_size_t, _dp, _dp, _err]),
     "gprx_hip_exact_insert": (C.c_int, [_vp, _dp, C.c_size_t, C.c_double, _dp, _err]),
     "gprx_hip_exact_predict": (C.c_int, [_vp, _dp, C.c_size_t, C.c_size_t, C.c_int32, _dp, _dp,
                                          _err]),

@@ -812,6 +812,54 @@ int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, 
   });
 }
 
+int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, double* alpha,
+                                 double* jitter, gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !l || !alpha || !jitter) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    if (n != (size_t)m->n) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "n differs from the model's point count");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    // The FP64 evaluation leaves X64 = L^-1 (the factor itself is never
+    // stored); L^T = L^-1 (Kc + s I) with Kc the kernel matrix and s the
+    // noise variance plus the jitter the factor took, one triangular GEMM.
+    const int saved = m->precision;
+    m->precision = GPRX_FP64;
+    double v = 0.0;
+    const int rc = m->evaluate(m->theta.data(), false, &v, nullptr, err);
+    m->precision = saved;
+    if (rc != GPRX_OK) return rc;
+    hipStream_t s = m->stream;
+    const long np2 = m->npad * m->npad;
+    const double shift = std::exp(m->theta.back()) + m->last_jitter;
+    fill_cross<double>(m->dp, m->prog.p, m->pts(), m->pts(), m->A64.p, m->npad, s);
+    DevBuf<double> lt;
+    lt.alloc((size_t)np2);
+    GPRX_HIP_CHECK(hipMemcpyAsync(lt.p, m->X64.p, sizeof(double) * np2, hipMemcpyDeviceToDevice, s));
+    gemm_call<double>(false, false, m->X64.p, m->npad, m->A64.p, m->npad, lt.p, m->npad,
+                      (int)m->npad, (int)m->npad, (int)m->npad, 1.0, shift, kKhiRow, 0,
+                      kOrderColFastRev, s);
+    std::vector<double> h(n * n);
+    GPRX_HIP_CHECK(hipMemcpy2DAsync(h.data(), sizeof(double) * n, lt.p, sizeof(double) * m->npad,
+                                    sizeof(double) * n, n, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(alpha, m->alpha.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
+    // h holds L^T column-major: L(r, c) = h[c + r n]; l is L column-major
+    for (size_t c = 0; c < n; ++c)
+      for (size_t r = 0; r < n; ++r) l[c * n + r] = r >= c ? h[c + r * n] : 0.0;
+    if (saved != GPRX_FP64) {  // the model predicts in its own form again
+      const int rc2 = m->evaluate(m->theta.data(), false, &v, nullptr, err);
+      if (rc2 != GPRX_OK) return rc2;
+    }
+    *jitter = m->last_jitter;
+    return GPRX_OK;
+  });
+}
+
 int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
                            int32_t variance_kind, double* mean, double* var, gprx_error* err) {
   return guarded(err, [&]() -> int {

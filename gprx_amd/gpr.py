@@ -10,6 +10,7 @@ FittedGpr::predict / predict_with (src/gpr/fitted.rs:854-960).
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import math
 from dataclasses import dataclass
@@ -194,6 +195,11 @@ class Gpr:
                                            C.byref(jit), C.byref(handle), C.byref(err))
         raise_for(rc, err)
         fitted = FittedGpr(handle, theta, n, d)
+        # what a save writes: this configuration and the training data
+        # (x column-major, kept as d rows of n values)
+        fitted._gpr = copy.copy(self)
+        fitted._x = np.array(x, dtype=np.float64).reshape(d, n)
+        fitted._y = np.array(y, dtype=np.float64).reshape(n)
         if self.mixed_jitter_bound != 0.0:
             rc = L.lib().gprx_hip_set_mixed_jitter_bound(handle, self.mixed_jitter_bound,
                                                          C.byref(err))
@@ -265,6 +271,8 @@ class FittedGpr:
                                            C.byref(err))
         raise_for(rc, err)
         self.n += 1
+        self._x = np.hstack([self._x, xn.reshape(self.d, 1)])
+        self._y = np.append(self._y, float(y_new))
         return v.value
 
     def delete(self, index: int) -> float:
@@ -275,7 +283,28 @@ class FittedGpr:
         rc = L.lib().gprx_hip_exact_delete(self._h, int(index), C.byref(v), C.byref(err))
         raise_for(rc, err)
         self.n -= 1
+        self._x = np.delete(self._x, int(index), axis=1)
+        self._y = np.delete(self._y, int(index))
         return v.value
+
+    def save(self, directory) -> None:
+        """FittedGpr::save (src/gpr/fitted.rs:810-812): config.json and the
+        training x, y in model.safetensors, written atomically."""
+        from .persist import save_fitted
+        save_fitted(self, directory, False)
+
+    def save_with_factor(self, directory) -> None:
+        """FittedGpr::save_with_factor (src/gpr/fitted.rs:814-830): as save,
+        with the Cholesky factor l (column-major) and alpha, formed on the
+        device in FP64 (f32 for a mixed model)."""
+        from .persist import save_fitted
+        save_fitted(self, directory, True)
+
+    @staticmethod
+    def load(directory) -> "FittedGpr":
+        """LoadedGpr::load (src/persist/mod.rs:308-310) of an exact model."""
+        from .persist import load
+        return load(directory)
 
     def value(self, params) -> float:
         p = L.f64(params)

@@ -160,6 +160,17 @@ int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_
 int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, double y_new,
                           double* nlml, gprx_error* err);
 
+/* The factor behind FittedGpr::save_with_factor (src/gpr/fitted.rs:810-830,
+ * packed by src/persist/tensors.rs:287-298): forms the model's Cholesky
+ * factor L of K + (sigma^2 + jitter) I in FP64 at its current parameters
+ * (as L^T = L^-1 K, the device model keeping only L^-1) and writes it
+ * column-major into l[n*n] (zeros above the diagonal), with alpha[n] =
+ * K^-1 y and the jitter the factor took.  Needs n_pad^2 doubles of device
+ * memory beyond the model's.  The model keeps its parameters, and a mixed
+ * model is formed again in its own precision afterwards. */
+int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, double* alpha,
+                                 double* jitter, gprx_error* err);
+
 /* Replaces OnlineGpr::delete (src/gpr/online.rs:564-620): drops the point
  * at row `index` (rows keep their order) and factors the model again at its
  * parameters, O(n^3), writing the new NLML (nullable).  A model of one

@@ -1,0 +1,146 @@
+"""save / save_with_factor / load of exact models on the GPU (src/persist/,
+FittedGpr::save{,_with_factor} at src/gpr/fitted.rs:810-830,
+LoadedGpr::load at src/persist/mod.rs:308-310).  A loaded model must predict
+as the saved one did (the parameters pass through exp and log once, so to
+1e-12); the factor a save writes must be the Cholesky factor of the oracle's
+K + (sigma^2 + jitter) I and alpha its solve with y."""
+import json
+
+import numpy as np
+import pytest
+import safetensors.numpy as stnp
+
+import gprx_amd as g
+import oracle
+from gprx_amd import persist as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, d, seed):
+    r = np.random.default_rng(seed)
+    x = r.uniform(-2.0, 2.0, size=n * d)  # column-major n x d
+    cols = x.reshape(d, n)
+    y = np.sin(cols.sum(axis=0)) + 0.1 * r.normal(size=n)
+    xs = r.uniform(-2.0, 2.0, size=40 * d)
+    return x, y, xs
+
+
+def _spec(d):
+    return g.ConstantKernel(1.3) * g.RbfArdKernel([0.8 + 0.2 * q for q in range(d)])
+
+
+def _assert_same_predictions(a, b, xs, d, rtol):
+    pa, pb = a.predict(xs, xs.size // d, d), b.predict(xs, xs.size // d, d)
+    np.testing.assert_allclose(pb.mean, pa.mean, rtol=rtol, atol=rtol)
+    np.testing.assert_allclose(pb.variance, pa.variance, rtol=rtol, atol=rtol)
+
+
+@pytest.mark.parametrize("precision", [g.Precision.FP64, g.Precision.MIXED])
+def test_save_load_round_trip(gpu, tmp_path, precision):
+    n, d = 300, 3
+    x, y, xs = _data(n, d, 0)
+    print(f"[persist] round trip n={n} d={d} precision={precision}", flush=True)
+    m = g.Gpr(_spec(d), g.GaussianLikelihood(0.05)).with_precision(precision).factor(x, n, d, y)
+    m.save(tmp_path)
+    cfg = json.loads((tmp_path / "config.json").read_text())
+    assert cfg["format_version"] == 1 and cfg["has_factor"] is False
+    assert cfg["factor_kind"] == "llt" and (cfg["n"], cfg["d"]) == (n, d)
+    assert cfg.get("precision", "double") == ("mixed" if precision == g.Precision.MIXED else "double")
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    assert set(t) == {"x", "y"}
+    np.testing.assert_array_equal(t["x"].reshape(-1), x)
+    np.testing.assert_array_equal(t["y"], y)
+    back = g.FittedGpr.load(tmp_path)
+    np.testing.assert_allclose(back.get_params(), m.get_params(), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(back.neg_log_marginal_likelihood(), m.neg_log_marginal_likelihood(),
+                               rtol=1e-12)
+    _assert_same_predictions(m, back, xs, d, 1e-10 if precision == g.Precision.MIXED else 1e-12)
+
+
+def test_save_with_factor_writes_the_cholesky_factor(gpu, tmp_path):
+    n, d = 260, 2
+    x, y, xs = _data(n, d, 1)
+    spec = _spec(d)
+    noise = 0.02
+    print(f"[persist] save_with_factor n={n} d={d}", flush=True)
+    m = g.Gpr(spec, g.GaussianLikelihood(noise)).factor(x, n, d, y)
+    m.save_with_factor(tmp_path)
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    assert t["l"].dtype == np.float64 and t["l"].shape == (n, n)
+    lower = t["l"].reshape(-1).reshape(n, n).T  # column-major bytes: [col][row]
+    assert np.all(np.triu(lower, 1) == 0.0)
+    K = oracle.cross(spec.program(), spec.get_params(), x, n, x, n, d) + noise * np.eye(n)
+    scale = np.abs(K).max()
+    np.testing.assert_allclose(lower @ lower.T, K, rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(lower, np.linalg.cholesky(K), rtol=0, atol=1e-11 * np.sqrt(scale))
+    np.testing.assert_allclose(t["alpha"], np.linalg.solve(K, y), rtol=1e-9,
+                               atol=1e-9 * np.abs(t["alpha"]).max())
+    back = g.FittedGpr.load(tmp_path)
+    _assert_same_predictions(m, back, xs, d, 1e-12)
+
+
+def test_mixed_save_with_factor_writes_f32(gpu, tmp_path):
+    n, d = 200, 2
+    x, y, xs = _data(n, d, 2)
+    m = g.Gpr(_spec(d), g.GaussianLikelihood(0.05)).with_precision(g.Precision.MIXED) \
+        .factor(x, n, d, y)
+    print(f"[persist] mixed save_with_factor n={n}", flush=True)
+    m.save_with_factor(tmp_path)
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    assert t["l"].dtype == np.float32 and t["alpha"].dtype == np.float64
+    back = g.FittedGpr.load(tmp_path)
+    assert back._gpr.precision == g.Precision.MIXED
+    _assert_same_predictions(m, back, xs, d, 1e-10)
+
+
+def test_jitter_and_bounds_are_kept(gpu, tmp_path):
+    n, d = 64, 1
+    x = np.repeat(np.linspace(0.0, 1.0, n // 2), 2)  # every point twice
+    y = np.cos(3.0 * x)
+    pol = g.JitterPolicy.adaptive(1e-9, 10.0, 8, 1e-2)
+    bounds = [1e-3, 1e3, 1e-13, 1e2]
+    print("[persist] duplicated points, adaptive jitter, tiny noise", flush=True)
+    m = g.Gpr(g.RbfKernel(0.3), g.GaussianLikelihood(1e-12)).with_jitter_policy(pol) \
+        .with_bounds(bounds).factor(x, n, d, y)
+    m.save(tmp_path)
+    cfg = json.loads((tmp_path / "config.json").read_text())
+    assert cfg["jitter"] == {"adaptive": {"initial": 1e-9, "multiplier": 10.0, "max_retries": 8,
+                                          "max_jitter": 1e-2}}
+    assert cfg.get("factor_jitter", 0.0) == m.last_jitter()
+    assert cfg["kernel"]["rbf"]["lengthscale"]["lo"] == 1e-3
+    back = g.FittedGpr.load(tmp_path)
+    assert back.last_jitter() == m.last_jitter()
+    assert back._gpr.jitter == pol and list(back._gpr.bounds) == bounds
+    _assert_same_predictions(m, back, np.linspace(-0.2, 1.2, 30), d, 1e-12)
+
+
+def test_online_model_saves_its_current_points(gpu, tmp_path):
+    n, d = 150, 2
+    x, y, xs = _data(n + 6, d, 3)
+    cols = x.reshape(d, n + 6)
+    m = g.Gpr(_spec(d), g.GaussianLikelihood(0.05)).factor(cols[:, :n].reshape(-1), n, d, y[:n])
+    print("[persist] 6 insertions and 2 deletions, then save", flush=True)
+    for i in range(n, n + 6):
+        m.insert(cols[:, i], y[i])
+    m.delete(3)
+    m.delete(n + 2)
+    keep = [i for i in range(n + 6) if i not in (3, n + 3)]
+    m.save(tmp_path)
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    np.testing.assert_array_equal(t["x"].reshape(-1), cols[:, keep].reshape(-1))
+    np.testing.assert_array_equal(t["y"], y[keep])
+    back = g.FittedGpr.load(tmp_path)
+    assert back.n == len(keep)
+    _assert_same_predictions(m, back, xs, d, 1e-9)
+
+
+def test_load_refuses_before_touching_the_device(gpu, tmp_path):
+    n, d = 40, 1
+    x, y, _ = _data(n, d, 4)
+    g.Gpr(g.RbfKernel(1.0), g.GaussianLikelihood(0.1)).factor(x, n, d, y).save(tmp_path)
+    cfg = json.loads((tmp_path / "config.json").read_text())
+    cfg["format_version"] = 2
+    (tmp_path / "config.json").write_text(json.dumps(cfg))
+    with pytest.raises(g.UnsupportedPersistVersion):
+        g.FittedGpr.load(tmp_path)

@@ -28,6 +28,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -513,6 +514,73 @@ def bench_online(args, gx):
     }
 
 
+def bench_persist(args, gx):
+    """FittedGpr::save / save_with_factor / LoadedGpr::load at config 2's
+    kernel and size (FP64).  The device part of a save with the factor is
+    gprx_hip_exact_export_factor: an FP64 factor and inverse, then
+    L^T = L^-1 K as one triangular GEMM; a load reads the files and factors
+    again on the device."""
+    import ctypes as C
+    import shutil
+    import tempfile
+
+    from gprx_amd import _lib as L
+    from gprx_amd.configs import exact_ard
+
+    wl = exact_ard(n=args.n)
+    n = wl.n
+    model = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).factor(wl.x, n, wl.d, wl.y)
+    l, alpha, jit, err = np.empty(n * n), np.empty(n), C.c_double(), L.GprxError()
+
+    def export():
+        rc = L.lib().gprx_hip_exact_export_factor(model._h, L.dptr(l), n, L.dptr(alpha),
+                                                   C.byref(jit), C.byref(err))
+        if rc != L.OK:
+            raise RuntimeError(f"export failed: {rc}")
+
+    print(f"[bench] persist n={n}: factor export x {args.steps}", file=sys.stderr, flush=True)
+    _, wall = _timed(export, args.steps, args.warmup)
+    ms = wall / args.steps * 1e3
+    root = Path(tempfile.mkdtemp(prefix="gprx-persist-"))
+    try:
+        t0 = time.perf_counter()
+        model.save(root / "plain")
+        save_ms = (time.perf_counter() - t0) * 1e3
+        print("[bench] persist: save_with_factor", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        model.save_with_factor(root / "factor")
+        save_f_ms = (time.perf_counter() - t0) * 1e3
+        file_gib = (root / "factor" / "model.safetensors").stat().st_size / 2 ** 30
+        print("[bench] persist: load", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        back = gx.FittedGpr.load(root / "factor")
+        load_ms = (time.perf_counter() - t0) * 1e3
+        xs = wl.x[:64 * wl.d].copy()
+        pa, pb = model.predict(xs, 64, wl.d), back.predict(xs, 64, wl.d)
+        pred_err = float(max(np.max(np.abs(pa.mean - pb.mean)), np.max(np.abs(pa.variance - pb.variance))))
+        back.close()
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    model.close()
+    # Cholesky n^3/3, triangular inverse n^3/3, L^-1 K (triangular x full) n^3
+    F = (5.0 / 3.0) * float(n) ** 3
+    ach = F / (ms * 1e-3) / 1e12
+    return {
+        "metric": f"factor exports/s (save_with_factor's device part), exact GPR n={n} d={wl.d} "
+                  "Constant x RbfArd, FP64",
+        "value": 1e3 / ms, "unit": "exports/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SeededRng, seed 2)",
+        "config": {"workload": "exact GPR save / load", "n": n, "d": wl.d,
+                   "parallelism": "single GPU"},
+        "save_ms": save_ms, "save_with_factor_ms": save_f_ms, "load_ms": load_ms,
+        "factor_file_gib": file_gib, "loaded_prediction_max_abs_diff": pred_err,
+        "roofline": {"bound": "mfma", "kernel": "factor export (factor, inverse, L^-1 K; "
+                     "including the n^2 readback)", "achieved": ach, "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None},
+    }
+
+
 def bench_copy(args, gx):
     """One 1 GiB device-to-device copy per step (torch's copy kernel): known
     HBM traffic of 1 GiB read + 1 GiB written per launch."""
@@ -543,7 +611,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["exact", "sgpr", "svgp", "batch", "fit", "hessian", "online",
-                                           "copy"],
+                                           "persist", "copy"],
                     default="exact")
     ap.add_argument("--restarts", type=int, default=16)
     ap.add_argument("--n", type=int, default=16384)
@@ -557,13 +625,13 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"exact": 5, "sgpr": 5, "svgp": 2000, "batch": 3, "fit": 1, "hessian": 2,
-                      "online": 50, "copy": 5}[args.workload]
+                      "online": 50, "persist": 3, "copy": 5}[args.workload]
 
     import gprx_amd as gx
 
     fn = {"exact": bench_exact, "sgpr": bench_sgpr, "svgp": bench_svgp, "batch": bench_batch,
           "fit": bench_fit, "hessian": bench_hessian, "online": bench_online,
-          "copy": bench_copy}[args.workload]
+          "persist": bench_persist, "copy": bench_copy}[args.workload]
     print(json.dumps(fn(args, gx)))
 
 

@@ -454,6 +454,23 @@ void size_buffers(ExactModel* m) {
 // sc[12] = k(x_new, x_new): sc[13] = 1 / d, sc[14] = z_n, sc[15] = 1 when
 // the pivot d^2 = kappa - l^T l is positive (0 otherwise, and nothing below
 // changes the model); z[n] = z_n and logdiag[n] = log d.
+// out = the lower triangle of in^T, zeros above the diagonal (both ld x ld
+// column-major, ld a multiple of 64): one 64 x 64 tile per block, read and
+// written along columns through a padded LDS tile.
+__global__ __launch_bounds__(256) void k_transpose_lower(const double* __restrict__ in,
+                                                         double* __restrict__ out, long ld) {
+  __shared__ double t[64][65];
+  const long i0 = (long)blockIdx.x * 64, j0 = (long)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) t[k][tx] = in[(i0 + tx) + (j0 + k) * ld];
+  __syncthreads();
+  // out(j, i) = in(i, j) = t[j - j0][i - i0]
+  for (int k = ty; k < 64; k += 4) {
+    const long i = i0 + k, j = j0 + tx;
+    out[j + i * ld] = j >= i ? t[tx][k] : 0.0;
+  }
+}
+
 __global__ void k_insert_scalars(double* sc, double noise, double y_new, double* z, double* logdiag,
                                  int n) {
   const double piv2 = sc[12] + noise - sc[8];
@@ -835,7 +852,8 @@ int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, dou
     if (rc != GPRX_OK) return rc;
     hipStream_t s = m->stream;
     const long np2 = m->npad * m->npad;
-    const double shift = std::exp(m->theta.back()) + m->last_jitter;
+    const double factor_jitter = m->last_jitter;
+    const double shift = std::exp(m->theta.back()) + factor_jitter;
     fill_cross<double>(m->dp, m->prog.p, m->pts(), m->pts(), m->A64.p, m->npad, s);
     DevBuf<double> lt;
     lt.alloc((size_t)np2);
@@ -843,19 +861,20 @@ int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, dou
     gemm_call<double>(false, false, m->X64.p, m->npad, m->A64.p, m->npad, lt.p, m->npad,
                       (int)m->npad, (int)m->npad, (int)m->npad, 1.0, shift, kKhiRow, 0,
                       kOrderColFastRev, s);
-    std::vector<double> h(n * n);
-    GPRX_HIP_CHECK(hipMemcpy2DAsync(h.data(), sizeof(double) * n, lt.p, sizeof(double) * m->npad,
+    // L = (L^T)^T into A64 (free until the next evaluation), then straight
+    // into the caller's column-major l
+    hipLaunchKernelGGL(k_transpose_lower, dim3(m->npad / 64, m->npad / 64), dim3(256), 0, s, lt.p,
+                       m->A64.p, m->npad);
+    GPRX_HIP_CHECK(hipGetLastError());
+    GPRX_HIP_CHECK(hipMemcpy2DAsync(l, sizeof(double) * n, m->A64.p, sizeof(double) * m->npad,
                                     sizeof(double) * n, n, hipMemcpyDeviceToHost, s));
     GPRX_HIP_CHECK(hipMemcpyAsync(alpha, m->alpha.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     GPRX_HIP_CHECK(hipStreamSynchronize(s));
-    // h holds L^T column-major: L(r, c) = h[c + r n]; l is L column-major
-    for (size_t c = 0; c < n; ++c)
-      for (size_t r = 0; r < n; ++r) l[c * n + r] = r >= c ? h[c + r * n] : 0.0;
     if (saved != GPRX_FP64) {  // the model predicts in its own form again
       const int rc2 = m->evaluate(m->theta.data(), false, &v, nullptr, err);
       if (rc2 != GPRX_OK) return rc2;
     }
-    *jitter = m->last_jitter;
+    *jitter = factor_jitter;
     return GPRX_OK;
   });
 }

@@ -30,6 +30,7 @@ import copy
 import ctypes as C
 import json
 import math
+import mmap
 import os
 import struct
 import tempfile
@@ -79,46 +80,54 @@ def _dtype_name(a: np.ndarray) -> str:
     raise PersistFailed("tensor", f"dtype {a.dtype} is not f32 or f64")
 
 
-def safetensors_bytes(tensors: dict) -> bytes:
-    """The safetensors file of {name: (array, shape)}: an 8-byte little-endian
-    header length, the compact JSON header padded with spaces to a multiple
-    of 8, then each tensor's little-endian bytes in the header's order
-    (descending dtype, then name), as src/persist/tensors.rs:321-359 lays it
-    out."""
+def safetensors_parts(tensors: dict) -> list:
+    """The safetensors file of {name: (array, shape)} as a list of buffers:
+    an 8-byte little-endian header length and the compact JSON header padded
+    with spaces to a multiple of 8, then each tensor's little-endian values
+    in the header's order (descending dtype, then name), as
+    src/persist/tensors.rs:321-359 lays it out."""
     items = sorted(tensors.items(), key=lambda kv: (-_DTYPE_RANK[_dtype_name(kv[1][0])], kv[0]))
     header, blobs, off = {}, [], 0
     for name, (arr, shape) in items:
         dt = _dtype_name(arr)
-        raw = np.ascontiguousarray(arr, dtype=_NP[dt]).tobytes()
-        if len(raw) != int(np.prod(shape, dtype=np.int64)) * _NP[dt].itemsize:
-            raise PersistFailed("tensor", f"{name} holds {arr.size} values, not shape {shape}")
-        header[name] = {"dtype": dt, "shape": list(shape), "data_offsets": [off, off + len(raw)]}
-        blobs.append(raw)
-        off += len(raw)
+        a = np.ascontiguousarray(arr, dtype=_NP[dt]).reshape(-1)
+        if a.nbytes != int(np.prod(shape, dtype=np.int64)) * _NP[dt].itemsize:
+            raise PersistFailed("tensor", f"{name} holds {a.size} values, not shape {shape}")
+        header[name] = {"dtype": dt, "shape": list(shape), "data_offsets": [off, off + a.nbytes]}
+        blobs.append(memoryview(a).cast("B"))
+        off += a.nbytes
     js = json.dumps(header, separators=(",", ":")).encode()
     js += b" " * (-len(js) % 8)
-    return struct.pack("<Q", len(js)) + js + b"".join(blobs)
+    return [struct.pack("<Q", len(js)) + js] + blobs
+
+
+def safetensors_bytes(tensors: dict) -> bytes:
+    """safetensors_parts joined into one buffer."""
+    return b"".join(bytes(p) for p in safetensors_parts(tensors))
 
 
 def read_safetensors(path: Path) -> dict:
-    """{name: (dtype, shape, bytes)} of a safetensors file, with the checks
-    a load needs (header inside the file, offsets in order and in bounds)."""
+    """{name: (dtype, shape, buffer)} of a safetensors file, memory-mapped
+    (only the pages a load touches are read), with the checks a load needs
+    (header inside the file, offsets in bounds and matching the shapes)."""
     try:
-        data = path.read_bytes()
+        with open(path, "rb") as f:
+            size = os.fstat(f.fileno()).st_size
+            if size < 8:
+                raise PersistFailed("tensor", f"{path} is too short for a safetensors header")
+            data = memoryview(mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ))
     except OSError as e:
         raise PersistFailed("io", f"read {path}: {e}") from None
-    if len(data) < 8:
-        raise PersistFailed("tensor", f"{path} is too short for a safetensors header")
     (hlen,) = struct.unpack("<Q", data[:8])
     if hlen > len(data) - 8:
         raise PersistFailed("tensor", f"{path}: header length {hlen} passes the end of the file")
     try:
-        header = json.loads(data[8:8 + hlen].decode())
+        header = json.loads(bytes(data[8:8 + hlen]).decode())
     except (UnicodeDecodeError, json.JSONDecodeError) as e:
         raise PersistFailed("tensor", f"{path}: header is not JSON: {e}") from None
     if not isinstance(header, dict):
         raise PersistFailed("tensor", f"{path}: header is not a JSON object")
-    body = memoryview(data)[8 + hlen:]
+    body = data[8 + hlen:]
     out = {}
     for name, info in header.items():
         if name == "__metadata__":
@@ -131,7 +140,7 @@ def read_safetensors(path: Path) -> dict:
             raise PersistFailed("tensor", f"{path}: tensor {name} has dtype {dt}, not F32 or F64")
         if not (0 <= a <= b <= len(body)) or b - a != int(np.prod(shape)) * _NP[dt].itemsize:
             raise PersistFailed("tensor", f"{path}: tensor {name} offsets {a}..{b} do not fit")
-        out[name] = (dt, shape, bytes(body[a:b]))
+        out[name] = (dt, shape, body[a:b])
     return out
 
 
@@ -143,19 +152,22 @@ def _tensor(tensors: dict, name: str, shape, dtypes=("F64",)) -> np.ndarray:
         raise PersistFailed("tensor", f"tensor {name} is {dt}, expected {' or '.join(dtypes)}")
     if got != list(shape):
         raise PersistFailed("tensor", f"tensor {name} has shape {got}, expected {list(shape)}")
-    a = np.frombuffer(raw, dtype=_NP[dt]).astype(np.float64)
-    if not np.all(np.isfinite(a)):
+    a = np.frombuffer(raw, dtype=_NP[dt])
+    step = 1 << 22  # checked in slices: no full-size temporaries for a factor
+    if not all(np.isfinite(a[i:i + step]).all() for i in range(0, a.size, step)):
         raise PersistFailed("tensor", f"tensor {name} holds a non-finite value")
     return a
 
 
-def _write_atomic(path: Path, data: bytes) -> None:
-    """Writes a temporary file beside `path` and renames it over `path`
-    (src/persist/atomic.rs), so a reader never sees half a file."""
+def _write_atomic(path: Path, parts) -> None:
+    """Writes the buffers into a temporary file beside `path` and renames it
+    over `path` (src/persist/atomic.rs): a reader never sees half a file, and
+    the file is never built whole in memory."""
     fd, tmp = tempfile.mkstemp(prefix=f".{path.name}.", dir=path.parent)
     try:
         with os.fdopen(fd, "wb") as f:
-            f.write(data)
+            for part in parts:
+                f.write(part)
             f.flush()
             os.fsync(f.fileno())
         os.replace(tmp, path)
@@ -361,8 +373,8 @@ def save_fitted(fitted, directory, with_factor: bool) -> None:
         tensors["l"] = (l, (n, n))
         tensors["alpha"] = (alpha, (n,))
     cfg = model_config(fitted, with_factor)
-    _write_atomic(dirp / TENSOR_FILE, safetensors_bytes(tensors))
-    _write_atomic(dirp / CONFIG_FILE, json.dumps(cfg, indent=2).encode())
+    _write_atomic(dirp / TENSOR_FILE, safetensors_parts(tensors))
+    _write_atomic(dirp / CONFIG_FILE, [json.dumps(cfg, indent=2).encode()])
 
 
 def read_model(directory):
@@ -390,8 +402,8 @@ def read_model(directory):
     if any(tuple(b) != DEFAULT_BOUNDS for b in bounds):
         gpr = gpr.with_bounds(np.asarray(bounds, dtype=np.float64).reshape(-1))
     tensors = read_safetensors(dirp / TENSOR_FILE)
-    x = _tensor(tensors, "x", (n, d))
-    y = _tensor(tensors, "y", (n,))
+    x = _tensor(tensors, "x", (n, d)).copy()
+    y = _tensor(tensors, "y", (n,)).copy()
     if has_factor:
         _tensor(tensors, "l", (n, n), ("F64", "F32"))
         _tensor(tensors, "alpha", (n,), ("F64", "F32"))

@@ -320,6 +320,27 @@ def model_config(fitted, has_factor: bool) -> dict:
     return cfg
 
 
+# serde_json's default nesting limit, past which the reference refuses a
+# config (src/persist/mod.rs:1370-1420)
+MAX_JSON_DEPTH = 128
+
+
+def _json_depth(obj) -> int:
+    """Deepest nesting of objects and arrays in a parsed JSON value."""
+    deepest, stack = 0, [(obj, 1)]
+    while stack:
+        v, depth = stack.pop()
+        if isinstance(v, dict):
+            items = v.values()
+        elif isinstance(v, list):
+            items = v
+        else:
+            continue
+        deepest = max(deepest, depth)
+        stack.extend((c, depth + 1) for c in items)
+    return deepest
+
+
 def _check_config(cfg: dict):
     """Validates what this backend can hold; returns (n, d, has_factor)."""
     if not isinstance(cfg, dict):
@@ -387,8 +408,11 @@ def read_model(directory):
         raise PersistFailed("io", f"read {dirp / CONFIG_FILE}: {e}") from None
     try:
         cfg = json.loads(raw)
-    except (UnicodeDecodeError, json.JSONDecodeError) as e:
+    except (UnicodeDecodeError, json.JSONDecodeError, RecursionError) as e:
         raise PersistFailed("config", f"config.json is not JSON: {e}") from None
+    if _json_depth(cfg) > MAX_JSON_DEPTH:
+        raise PersistFailed("config", f"config.json nests past the recursion limit of "
+                                      f"{MAX_JSON_DEPTH}")
     n, d, has_factor = _check_config(cfg)
     bounds: list = []
     kernel = decode_kernel(cfg.get("kernel"), bounds)

@@ -142,3 +142,31 @@ def test_missing_and_bad_tensor_files(tmp_path):
     with pytest.raises(P.PersistFailed) as e:
         P.read_model(tmp_path)
     assert "no tensor l" in str(e.value)
+
+
+def test_leaf_and_sum_encoding_match_the_reference_literal():
+    # the literal of src/persist/mod.rs:1387-1390 (an rbf leaf at the default
+    # bounds, a binary sum)
+    leaf = {"rbf": {"lengthscale": {"value": 1.0, "lo": 1e-5, "hi": 1e5}}}
+    assert P.encode_kernel(g.RbfKernel(1.0), iter([P.DEFAULT_BOUNDS])) == leaf
+    assert P.encode_kernel(g.RbfKernel(1.0) + g.RbfKernel(1.0), iter([P.DEFAULT_BOUNDS] * 2)) == \
+        {"sum": {"left": leaf, "right": leaf}}
+
+
+def test_deeply_nested_config_trees_are_rejected(tmp_path):
+    # deeply_nested_config_trees_are_rejected_without_overflow
+    # (src/persist/mod.rs:1370-1420): 300 nested sums are refused as a config
+    # error naming the recursion limit; 20 load
+    leaf = {"rbf": {"lengthscale": {"value": 1.0, "lo": 1e-5, "hi": 1e5}}}
+
+    def nested(depth):
+        k = leaf
+        for _ in range(depth):
+            k = {"sum": {"left": k, "right": leaf}}
+        return k
+
+    with pytest.raises(P.PersistFailed) as e:
+        P.read_model(_write(tmp_path, _config(kernel=nested(300))))
+    assert e.value.kind == "config" and "recursion limit" in str(e.value)
+    gpr, *_ = P.read_model(_write(tmp_path, _config(kernel=nested(20))))
+    assert gpr.kernel.num_params() == 21

@@ -170,3 +170,26 @@ def test_deeply_nested_config_trees_are_rejected(tmp_path):
     assert e.value.kind == "config" and "recursion limit" in str(e.value)
     gpr, *_ = P.read_model(_write(tmp_path, _config(kernel=nested(20))))
     assert gpr.kernel.num_params() == 21
+
+
+@pytest.mark.parametrize("name", ["x", "y", "l", "alpha"])
+def test_non_finite_stored_tensors_fail_to_load(tmp_path, name):
+    # non_finite_stored_tensors_fail_to_load (src/persist/mod.rs:1223-1251):
+    # the first value of each tensor set to NaN is a tensor error naming it
+    t = {"x": (np.array([0.0, 1.0, 2.0]), (3, 1)), "y": (np.array([0.5, -1.0, 2.0]), (3,)),
+         "l": (np.tril(np.ones((3, 3))).T.reshape(-1), (3, 3)), "alpha": (np.ones(3), (3,))}
+    t[name] = (t[name][0].copy(), t[name][1])
+    t[name][0].reshape(-1)[0] = np.nan
+    (tmp_path / "config.json").write_text(json.dumps(_config(n=3, has_factor=True)))
+    (tmp_path / "model.safetensors").write_bytes(P.safetensors_bytes(t))
+    with pytest.raises(P.PersistFailed) as e:
+        P.read_model(tmp_path)
+    assert e.value.kind == "tensor"
+    assert "non-finite" in str(e.value) and f"tensor {name} " in str(e.value)
+
+
+def test_missing_directory_is_an_io_failure(tmp_path):
+    # missing_directory_is_an_io_failure (src/persist/mod.rs:1681-1690)
+    with pytest.raises(P.PersistFailed) as e:
+        P.read_model(tmp_path / "does-not-exist")
+    assert e.value.kind == "io"

@@ -144,3 +144,33 @@ def test_load_refuses_before_touching_the_device(gpu, tmp_path):
     (tmp_path / "config.json").write_text(json.dumps(cfg))
     with pytest.raises(g.UnsupportedPersistVersion):
         g.FittedGpr.load(tmp_path)
+
+
+@pytest.mark.parametrize("with_factor", [False, True])
+def test_rbf_save_load_matches_predict(gpu, tmp_path, with_factor):
+    # rbf_save_load_matches_predict / rbf_save_with_factor_matches_predict
+    # (src/persist/mod.rs:1166-1208): the two-point fit, saved and loaded,
+    # predicts at 0.5 as before
+    print(f"[persist] two-point rbf fit, with_factor={with_factor}", flush=True)
+    fitted = g.Gpr(g.RbfKernel(1.0), g.GaussianLikelihood(0.1)).fit([0.0, 1.0], 2, 1, [0.0, 1.0])
+    want = fitted.predict([0.5], 1, 1)
+    (fitted.save_with_factor if with_factor else fitted.save)(tmp_path)
+    got = g.FittedGpr.load(tmp_path).predict([0.5], 1, 1)
+    np.testing.assert_allclose(got.mean, want.mean, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(got.variance, want.variance, rtol=1e-12, atol=1e-12)
+
+
+def test_failed_tensor_write_leaves_the_previous_config(gpu, tmp_path):
+    # failed_tensor_write_leaves_the_previous_config (src/persist/mod.rs:1448-1483)
+    def fixed_rbf(x, y):
+        return g.Gpr(g.RbfKernel(1.0), g.GaussianLikelihood(0.1)).factor(x, len(x), 1, y)
+
+    print("[persist] tensor write blocked by a directory", flush=True)
+    fixed_rbf([0.0, 1.0], [0.0, 1.0]).save(tmp_path)
+    before = (tmp_path / "config.json").read_bytes()
+    (tmp_path / "model.safetensors").unlink()
+    (tmp_path / "model.safetensors" / "blocker").mkdir(parents=True)
+    with pytest.raises(P.PersistFailed):
+        fixed_rbf([0.0, 1.0, 2.0], [0.0, 1.0, 0.5]).save(tmp_path)
+    assert (tmp_path / "config.json").read_bytes() == before
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["config.json", "model.safetensors"]

@@ -358,15 +358,52 @@ static int chol_lower_blocked(double* a, size_t n) {
       double inv = 1.0 / lj;
       for (size_t i = j + 1; i < n; i++) cj[i] *= inv;
     }
-    /* trailing update A22 -= L21 L21^T (lower), parallel over columns */
+    /* trailing update A22 -= L21 L21^T (lower), in chunks of rows so the
+     * panel's rows of a chunk stay in cache while every column of the chunk
+     * is updated.  Each entry still takes the panel's columns in order. */
     size_t t0 = k0 + kb;
-#pragma omp parallel for schedule(dynamic, 8)
-    for (long jj = (long)t0; jj < (long)n; jj++) {
-      double* cj = a + (size_t)jj * n;
-      for (size_t k = k0; k < k0 + kb; k++) {
-        const double ljk = a[(size_t)jj + k * n];
-        const double* ck = a + k * n;
-        for (size_t i = (size_t)jj; i < n; i++) cj[i] -= ck[i] * ljk;
+    const size_t RC = 256;
+    const long nchunks = (long)((n - t0 + RC - 1) / RC);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long ch = 0; ch < nchunks; ch++) {
+      const size_t r0 = t0 + (size_t)ch * RC;
+      const size_t r1 = (r0 + RC < n) ? r0 + RC : n;
+      size_t jj = t0;
+      /* four columns at a time share each load of the panel's rows; rows
+       * above the group's last diagonal are done column by column first */
+      for (; jj + 4 <= r1; jj += 4) {
+        double* c0 = a + jj * n;
+        double* c1 = c0 + n;
+        double* c2 = c1 + n;
+        double* c3 = c2 + n;
+        const size_t i3 = (jj + 3 > r0) ? jj + 3 : r0;
+        for (size_t k = k0; k < k0 + kb; k++) {
+          const double* ck = a + k * n;
+          const double l0 = a[jj + k * n], l1 = a[jj + 1 + k * n];
+          const double l2 = a[jj + 2 + k * n], l3 = a[jj + 3 + k * n];
+          for (size_t q = 0; q < 3; q++) {
+            double* cq = a + (jj + q) * n;
+            const double lq = a[jj + q + k * n];
+            const size_t iq = (jj + q > r0) ? jj + q : r0;
+            for (size_t i = iq; i < i3 && i < r1; i++) cq[i] -= ck[i] * lq;
+          }
+          for (size_t i = i3; i < r1; i++) {
+            const double v = ck[i];
+            c0[i] -= v * l0;
+            c1[i] -= v * l1;
+            c2[i] -= v * l2;
+            c3[i] -= v * l3;
+          }
+        }
+      }
+      for (; jj < r1; jj++) {
+        double* cj = a + jj * n;
+        const size_t i0 = (jj > r0) ? jj : r0;
+        for (size_t k = k0; k < k0 + kb; k++) {
+          const double ljk = a[jj + k * n];
+          const double* ck = a + k * n;
+          for (size_t i = i0; i < r1; i++) cj[i] -= ck[i] * ljk;
+        }
       }
     }
   }
@@ -376,30 +413,121 @@ static int chol_lower_blocked(double* a, size_t n) {
 }
 
 /* Solve L x = b in place (forward), then L^T x = b (backward), for nrhs
- * columns of B (ld = n).  Parallel over right-hand sides. */
-static void trsm_lower(const double* L, size_t n, double* B, size_t nrhs) {
-#pragma omp parallel for schedule(dynamic, 4)
-  for (long c = 0; c < (long)nrhs; c++) {
-    double* x = B + (size_t)c * n;
-    for (size_t j = 0; j < n; j++) {
-      double v = x[j] / L[j + j * n];
-      x[j] = v;
+ * columns of B (ld = n).  Parallel over blocks of ORC_RHS right-hand sides:
+ * a block sweeps L once, column by column, so L is read from memory once
+ * per block rather than once per right-hand side.  Each column still sees
+ * exactly the operations of the one-column substitution, in the same order.
+ * `first` (forward only) is the first row that may be nonzero in any column
+ * of B: rows above it are zero and stay zero (0 / L_jj = 0 subtracts
+ * nothing), so starting there changes no bit of the result. */
+#define ORC_RHS 8
+static void trsm_lower_from(const double* L, size_t n, double* B, size_t nrhs, size_t first,
+                            int ident) {
+  const long nblk = (long)((nrhs + ORC_RHS - 1) / ORC_RHS);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long blk = 0; blk < nblk; blk++) {
+    const size_t c0 = (size_t)blk * ORC_RHS;
+    const size_t nc = (nrhs - c0 < ORC_RHS) ? nrhs - c0 : ORC_RHS;
+    /* For B = I the block's columns are zero above row c0. */
+    const size_t j0 = ident ? c0 : first;
+    double* x[ORC_RHS];
+    for (size_t c = 0; c < nc; c++) x[c] = B + (c0 + c) * n;
+    for (size_t j = j0; j < n; j++) {
       const double* cl = L + j * n;
-      for (size_t i = j + 1; i < n; i++) x[i] -= cl[i] * v;
+      const double ljj = cl[j];
+      double v[ORC_RHS];
+      for (size_t c = 0; c < nc; c++) {
+        v[c] = x[c][j] / ljj;
+        x[c][j] = v[c];
+      }
+      if (nc == ORC_RHS) {
+        /* one pass over L's column for all eight right-hand sides */
+        for (size_t i = j + 1; i < n; i++) {
+          const double l = cl[i];
+          x[0][i] -= l * v[0];
+          x[1][i] -= l * v[1];
+          x[2][i] -= l * v[2];
+          x[3][i] -= l * v[3];
+          x[4][i] -= l * v[4];
+          x[5][i] -= l * v[5];
+          x[6][i] -= l * v[6];
+          x[7][i] -= l * v[7];
+        }
+      } else {
+        for (size_t c = 0; c < nc; c++)
+          for (size_t i = j + 1; i < n; i++) x[c][i] -= cl[i] * v[c];
+      }
     }
   }
 }
+static void trsm_lower(const double* L, size_t n, double* B, size_t nrhs) {
+  trsm_lower_from(L, n, B, nrhs, 0, 0);
+}
 static void trsm_lower_t(const double* L, size_t n, double* B, size_t nrhs) {
-#pragma omp parallel for schedule(dynamic, 4)
-  for (long c = 0; c < (long)nrhs; c++) {
-    double* x = B + (size_t)c * n;
+  const long nblk = (long)((nrhs + ORC_RHS - 1) / ORC_RHS);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long blk = 0; blk < nblk; blk++) {
+    const size_t c0 = (size_t)blk * ORC_RHS;
+    const size_t nc = (nrhs - c0 < ORC_RHS) ? nrhs - c0 : ORC_RHS;
+    double* x[ORC_RHS];
+    for (size_t c = 0; c < nc; c++) x[c] = B + (c0 + c) * n;
     for (size_t jj = n; jj-- > 0;) {
       const double* cl = L + jj * n;
-      double s = x[jj];
-      for (size_t i = jj + 1; i < n; i++) s -= cl[i] * x[i];
-      x[jj] = s / L[jj + jj * n];
+      double s[ORC_RHS];
+      for (size_t c = 0; c < nc; c++) s[c] = x[c][jj];
+      if (nc == ORC_RHS) {
+        for (size_t i = jj + 1; i < n; i++) {
+          const double l = cl[i];
+          s[0] -= l * x[0][i];
+          s[1] -= l * x[1][i];
+          s[2] -= l * x[2][i];
+          s[3] -= l * x[3][i];
+          s[4] -= l * x[4][i];
+          s[5] -= l * x[5][i];
+          s[6] -= l * x[6][i];
+          s[7] -= l * x[7][i];
+        }
+      } else {
+        for (size_t c = 0; c < nc; c++)
+          for (size_t i = jj + 1; i < n; i++) s[c] -= cl[i] * x[c][i];
+      }
+      for (size_t c = 0; c < nc; c++) x[c][jj] = s[c] / cl[jj];
     }
   }
+}
+
+/* out(i, j) = sum_c P(i, c) Q(j, c) for the m x n column-major P and Q
+ * (ld = m), each sum running over c = 0, 1, ..., n-1 in order (the order of
+ * the plain triple loop).  Tiles of ORC_T x ORC_T outputs stream P and Q
+ * once per tile instead of once per output.  sym != 0: P == Q, only tiles on
+ * or below the diagonal are formed and mirrored. */
+#define ORC_T 32
+static void gemm_nt_cols(const double* P, const double* Q, size_t m, size_t n, double* out,
+                         int sym) {
+  const long nt = (long)((m + ORC_T - 1) / ORC_T);
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+  for (long bj = 0; bj < nt; bj++)
+    for (long bi = 0; bi < nt; bi++) {
+      if (sym && bi < bj) continue;
+      const size_t i0 = (size_t)bi * ORC_T, j0 = (size_t)bj * ORC_T;
+      const size_t ni = (m - i0 < ORC_T) ? m - i0 : ORC_T;
+      const size_t nj = (m - j0 < ORC_T) ? m - j0 : ORC_T;
+      double acc[ORC_T][ORC_T];
+      memset(acc, 0, sizeof(acc));
+      for (size_t c = 0; c < n; c++) {
+        const double* pc = P + c * m + i0;
+        const double* qc = Q + c * m + j0;
+        for (size_t jj = 0; jj < nj; jj++) {
+          const double qv = qc[jj];
+          for (size_t ii = 0; ii < ni; ii++) acc[jj][ii] += pc[ii] * qv;
+        }
+      }
+      for (size_t jj = 0; jj < nj; jj++)
+        for (size_t ii = 0; ii < ni; ii++) {
+          out[(i0 + ii) + (j0 + jj) * m] = acc[jj][ii];
+          if (sym) out[(j0 + jj) + (i0 + ii) * m] = acc[jj][ii];
+        }
+    }
 }
 
 /* Cholesky of a + j*I with gprx's retry loop (src/linalg/chol.rs:20-43):
@@ -474,7 +602,7 @@ int orc_exact_value_grad(const int32_t* ops, int nops, const double* theta, cons
     double* Ai = K;
     memset(Ai, 0, sizeof(double) * n * n);
     for (size_t i = 0; i < n; i++) Ai[i + i * n] = 1.0;
-    trsm_lower(L, n, Ai, n);
+    trsm_lower_from(L, n, Ai, n, 0, 1);
     trsm_lower_t(L, n, Ai, n);
     /* W lower = aa' - A^-1 */
     for (size_t j = 0; j < n; j++)
@@ -505,7 +633,8 @@ int orc_exact_predict(const int32_t* ops, int nops, const double* theta, const d
   double* L = (double*)malloc(sizeof(double) * n * n);
   orc_cross(ops, nops, theta, x, n, x, n, d, K);
   for (size_t i = 0; i < n; i++) K[i + i * n] += noise;
-  int rc = chol_with_jitter(K, n, L, 0.0, 1.0, 0, 0.0, NULL, 0);
+  /* The blocked factor (same factor up to summation order) above n = 2048. */
+  int rc = chol_with_jitter(K, n, L, 0.0, 1.0, 0, 0.0, NULL, n > 2048);
   if (rc != ORC_OK) {
     free(K);
     free(L);
@@ -583,14 +712,7 @@ static int vfe_assemble(const int32_t* ops, int nops, const double* theta, int P
   trsm_lower(s->L, m, s->A, n);
   /* B = A A' + s2 I (dense.rs:231-239) */
   double* B = (double*)calloc(m * m, sizeof(double));
-#pragma omp parallel for schedule(dynamic, 4)
-  for (long j = 0; j < (long)m; j++)
-    for (size_t i = (size_t)j; i < m; i++) {
-      double acc = 0.0;
-      for (size_t c = 0; c < n; c++) acc += s->A[i + c * m] * s->A[(size_t)j + c * m];
-      B[i + (size_t)j * m] = acc;
-      B[(size_t)j + i * m] = acc;
-    }
+  gemm_nt_cols(s->A, s->A, m, n, B, 1);
   for (size_t i = 0; i < m; i++) B[i + i * m] += s->noise;
   s->LB = (double*)malloc(sizeof(double) * m * m);
   /* B takes JitterPolicy::default() retries (vfe.rs:244-250): none. */
@@ -661,13 +783,7 @@ int orc_vfe_value_grad(const int32_t* ops, int nops, const double* theta, const 
     for (size_t c = 0; c < n; c++)
       for (size_t i = 0; i < m; i++) G[i + c * m] -= (s.w[i] * r[c] + s.A[i + c * m]) / noise;
     double* H = (double*)calloc(m * m, sizeof(double));
-#pragma omp parallel for schedule(static)
-    for (long j = 0; j < (long)m; j++)
-      for (size_t i = 0; i < m; i++) {
-        double acc = 0.0;
-        for (size_t c = 0; c < n; c++) acc += G[i + c * m] * s.A[(size_t)j + c * m];
-        H[i + (size_t)j * m] = acc;
-      }
+    gemm_nt_cols(G, s.A, m, n, H, 0);
     /* w_mn = L^-T G */
     trsm_lower_t(s.L, m, G, n);
     /* X = L^-T tril_half(H); Xt = L^-T X^T; w_mm = -1/2 (Xt^T + Xt) */

@@ -19,7 +19,12 @@
  *   VFE gradient -- central differences of the bound + Z = X equals exact;
  *   SVGP -- GPyTorch prior-q goldens svgp_*.json + central differences +
  *     Titsias-optimal q equals the VFE bound;
- *   RNG -- first outputs for seeds 0 and 42 (src/rng.rs:105-116).
+ *   RNG -- first outputs for seeds 0 and 42 (src/rng.rs:105-116); unit,
+ *     up_to, the Fisher-Yates shuffle and Box-Muller by closed forms worked
+ *     from those words (tests/test_rng.py).
+ *   Full sizes -- the dense routines used at n >= 2048 against LAPACK
+ *     (scipy) at n = 16384 and the VFE at n = 2^20 against the exact NLML
+ *     identities, recorded by tests/golden/make_fullsize.py.
  */
 #include <math.h>
 #include <stdint.h>

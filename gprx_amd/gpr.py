@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
-from .errors import raise_for
+from .errors import InvalidConfig, raise_for
 from .kernels import KernelSpec
 
 
@@ -45,10 +45,25 @@ class JitterPolicy:
 
     @staticmethod
     def fixed(j: float) -> "JitterPolicy":
+        """JitterPolicy::fixed (src/policy.rs:245-252): j finite and >= 0."""
+        if not (math.isfinite(j) and j >= 0.0):
+            raise InvalidConfig(f"jitter must be finite and non-negative, got {j}")
         return JitterPolicy(j, 1.0, j, 1 if j > 0 else 0)
 
     @staticmethod
     def adaptive(initial: float, multiplier: float, max_retries: int, max_jitter: float):
+        """JitterPolicy::adaptive (src/policy.rs:266-304): initial > 0,
+        multiplier > 1, max_retries >= 1 and max_jitter >= initial, all finite."""
+        if not (math.isfinite(initial) and initial > 0.0):
+            raise InvalidConfig(f"adaptive initial jitter must be finite and positive, got {initial}")
+        if not (math.isfinite(multiplier) and multiplier > 1.0):
+            raise InvalidConfig("adaptive jitter multiplier must be finite and greater than 1, "
+                                f"got {multiplier}")
+        if max_retries < 1:
+            raise InvalidConfig("adaptive jitter max_retries must be at least 1")
+        if not (math.isfinite(max_jitter) and max_jitter >= initial):
+            raise InvalidConfig(f"adaptive max_jitter must be finite and at least initial ({initial}), "
+                                f"got {max_jitter}")
         return JitterPolicy(initial, multiplier, max_jitter, max_retries)
 
     def c(self) -> L.JitterPolicyC:

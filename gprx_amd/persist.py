@@ -39,7 +39,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib as L
-from .errors import GprError, raise_for
+from .errors import GprError, InvalidConfig, raise_for
 from .kernels import ConstantKernel, PeriodicKernel, Product, RbfArdKernel, RbfKernel, Sum
 
 FORMAT_VERSION = 1
@@ -275,18 +275,15 @@ def decode_jitter(obj):
     from .gpr import JitterPolicy
     tag, body = _one_key(obj, "jitter")
     try:
+        # the reference builds the policy through its checked constructors
+        # (src/persist/config.rs:243-283, src/policy.rs:245-304)
         if tag == "fixed":
-            j = float(body["jitter"])
-            if not (math.isfinite(j) and j >= 0):
-                raise ValueError
-            return JitterPolicy.fixed(j)
+            return JitterPolicy.fixed(float(body["jitter"]))
         if tag == "adaptive":
             ini, mul = float(body["initial"]), float(body["multiplier"])
             ret, mx = int(body["max_retries"]), float(body["max_jitter"])
-            if not (ini >= 0 and mul >= 1 and ret >= 0 and mx >= ini and math.isfinite(mx)):
-                raise ValueError
             return JitterPolicy.adaptive(ini, mul, ret, mx)
-    except (KeyError, TypeError, ValueError):
+    except (KeyError, TypeError, ValueError, InvalidConfig):
         raise PersistFailed("config", f"jitter {tag} has invalid fields") from None
     raise PersistFailed("config", f"jitter {tag} is not fixed or adaptive")
 
@@ -306,6 +303,9 @@ def model_config(fitted, has_factor: bool) -> dict:
            "has_factor": has_factor, "factor_kind": "llt"}
     if gpr.precision == L.MIXED:
         cfg["precision"] = "mixed"
+        # The mixed form's refinement residuals use the kernel evaluated again
+        # in f64, the reference's ReevaluateKernel (src/persist/config.rs:171-178).
+        cfg["residual"] = "reevaluate_kernel"
     cfg["kernel"] = encode_kernel(kernel, it)
     cfg["likelihood"] = {"noise_variance": _bounded(math.exp(theta[-1]), *next(it))}
     cfg["jitter"] = encode_jitter(gpr.jitter)
@@ -358,8 +358,12 @@ def _check_config(cfg: dict):
         raise PersistFailed("config", f"factor_kind {kind!r} is not llt or ldlt")
     if cfg.get("precision", "double") not in ("double", "mixed"):
         raise PersistFailed("config", f"precision {cfg.get('precision')!r} is not double or mixed")
-    if cfg.get("residual", "promote_storage") != "promote_storage":
-        raise PersistFailed("config", f"residual {cfg.get('residual')!r} is not promote_storage")
+    # Either residual loads: the model is factored again in its own form.  The
+    # field is read only for mixed precision (src/persist/config.rs:130-140),
+    # but must still be one of its two values.
+    if cfg.get("residual", "promote_storage") not in ("promote_storage", "reevaluate_kernel"):
+        raise PersistFailed("config", f"residual {cfg.get('residual')!r} is not promote_storage "
+                                      "or reevaluate_kernel")
     if cfg.get("math", "accurate") != "accurate":
         raise PersistFailed("config", f"math {cfg.get('math')!r} is not accurate")
     if cfg.get("distance") is not None:

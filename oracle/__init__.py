@@ -110,6 +110,8 @@ def _setup(L):
     L.orc_adam_step.argtypes = [_dp, _dp, _dp, _dp, C.POINTER(C.c_uint64), C.c_size_t,
                                 C.c_double, C.c_double, C.c_double, C.c_double]
     L.orc_threads.restype = C.c_int
+    L.orc_set_verbose.argtypes = [C.c_int]
+    L.orc_set_verbose.restype = None
 
 
 class OracleError(RuntimeError):
@@ -230,3 +232,8 @@ def adam_step(z, g, m1, m2, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+def set_verbose(on: bool) -> None:
+    """Print one line per finished phase of an exact evaluation to stderr."""
+    lib().orc_set_verbose(int(on))

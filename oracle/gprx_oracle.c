@@ -31,9 +31,30 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <stdio.h>
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+/* When set, the exact evaluation prints one line per finished phase to
+ * stderr (a timed CPU baseline of minutes shows its progress). */
+static int orc_verbose = 0;
+void orc_set_verbose(int v) { orc_verbose = v; }
+static double orc_now(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+#define ORC_PHASE(name, t0)                                                              \
+  do {                                                                                   \
+    if (orc_verbose) {                                                                   \
+      fprintf(stderr, "[oracle] %s done at %.1f s\n", name, orc_now() - (t0));          \
+      fflush(stderr);                                                                    \
+    }                                                                                    \
+  } while (0)
 
 #define ORC_OK 0
 #define ORC_CHOLESKY_FAILED 1
@@ -580,13 +601,16 @@ int orc_exact_value_grad(const int32_t* ops, int nops, const double* theta, cons
                          double* alpha_out, double* jitter_taken, int blocked) {
   int P = orc_num_params(ops, nops, d);
   if (P < 0) return ORC_BAD_PROGRAM;
+  const double t0 = orc_now();
   double noise = exp(theta[P]);
   double* K = (double*)malloc(sizeof(double) * n * n);
   double* L = (double*)malloc(sizeof(double) * n * n);
   int rc = orc_cross(ops, nops, theta, x, n, x, n, d, K);
   for (size_t i = 0; i < n; i++) K[i + i * n] += noise;
+  ORC_PHASE("kernel fill", t0);
   rc = chol_with_jitter(K, n, L, jit_initial, jit_mult, jit_retries, jit_max, jitter_taken,
                         blocked);
+  ORC_PHASE("Cholesky", t0);
   if (rc != ORC_OK) {
     free(K);
     free(L);
@@ -608,11 +632,14 @@ int orc_exact_value_grad(const int32_t* ops, int nops, const double* theta, cons
     memset(Ai, 0, sizeof(double) * n * n);
     for (size_t i = 0; i < n; i++) Ai[i + i * n] = 1.0;
     trsm_lower_from(L, n, Ai, n, 0, 1);
+    ORC_PHASE("forward solve of L X = I", t0);
     trsm_lower_t(L, n, Ai, n);
+    ORC_PHASE("backward solve of L^T A^-1 = X", t0);
     /* W lower = aa' - A^-1 */
     for (size_t j = 0; j < n; j++)
       for (size_t i = j; i < n; i++) Ai[i + j * n] = alpha[i] * alpha[j] - Ai[i + j * n];
     orc_contract(ops, nops, theta, x, n, x, n, d, Ai, 1, grad);
+    ORC_PHASE("gradient contraction", t0);
     for (int p = 0; p < P; p++) grad[p] *= -0.5;
     double trw = 0.0;
     for (size_t i = 0; i < n; i++) trw += Ai[i + i * n];

@@ -236,3 +236,44 @@ def test_input_errors(gpu):
         m.value_and_gradient_into(np.array([0.0, np.inf]), np.empty(2))
     with pytest.raises(g.LengthMismatch):
         _fit(g.RbfArdKernel([1.0, 1.0]), 0.1, np.zeros(3), 3, 1, np.zeros(3))
+
+
+def test_mixed_refinement_stall_falls_back_to_fp64(gpu):
+    """An ill-conditioned noise variance makes the f32 corrections stall
+    (as the reference's ill-conditioned case, src/precision/tests.rs:40-41,
+    220-238).  RBF l = 0.5 on 1024 uniform points in [0, 1]^4: as the noise
+    falls from 1e-2, kappa(K + sigma^2 I) grows, the corrections through the
+    f32 inverse slow down and then stall (two residual ratios above 0.9, or 10
+    corrections without meeting 10 n eps_64, src/precision/refine.rs:56-98)
+    before the f32 factor itself fails.  At every point where refinement
+    stalled the evaluation is the FP64 model's, bit for bit."""
+    n, d = 1024, 4
+    rng = np.random.default_rng(31)
+    x = rng.random(n * d)
+    y = np.sin(3 * x[:n]) + np.cos(2 * x[n:2 * n]) + 0.1 * rng.standard_normal(n)
+    spec = g.RbfKernel(0.5)
+    X = x.reshape(d, n).T
+    K = np.exp(-((X[:, None, :] - X[None, :, :]) ** 2).sum(-1) / (2 * 0.5 ** 2))
+    ev = np.linalg.eigvalsh(K)
+    seen = []
+    stalled = 0
+    for noise in 10.0 ** np.arange(-2.0, -7.01, -0.25):
+        th = np.array([math.log(0.5), math.log(noise)])
+        m32 = _fit(spec, noise, x, n, d, y, precision=g.Precision.MIXED)
+        v32, g32 = _eval(m32, th)
+        fb, steps = m32.last_fallback(), m32.last_refine_steps()
+        m32.close()
+        kappa = (ev[-1] + noise) / (max(ev[0], 0.0) + noise)
+        print(f"  noise {noise:.2e} kappa*eps32 {kappa * 2.0 ** -23:.2e} fallback {fb} "
+              f"corrections {steps}", flush=True)
+        seen.append(fb)
+        if fb == g.Fallback.JITTER:
+            break  # below the window: the f32 factor fails
+        if fb == g.Fallback.REFINEMENT:
+            stalled += 1
+            m64 = _fit(spec, noise, x, n, d, y)
+            v64, g64 = _eval(m64, th)
+            m64.close()
+            assert v32 == v64 and np.array_equal(g32, g64)
+    assert seen[0] == g.Fallback.NONE
+    assert stalled >= 1, seen

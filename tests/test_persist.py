@@ -78,6 +78,33 @@ def test_jitter_json_round_trip():
     assert set(P.encode_jitter(g.JitterPolicy.adaptive(1e-8, 10.0, 5, 1e-3))) == {"adaptive"}
 
 
+@pytest.mark.parametrize("args", [(0.0, 10.0, 5, 1e-3), (-1e-8, 10.0, 5, 1e-3), (1e-8, 1.0, 5, 1e-3),
+                                  (1e-8, 0.5, 5, 1e-3), (1e-8, 10.0, 0, 1e-3), (1e-3, 10.0, 5, 1e-4),
+                                  (1e-8, 10.0, 5, float("inf")), (float("nan"), 10.0, 5, 1e-3)])
+def test_adaptive_jitter_domain_matches_the_reference(args):
+    """JitterPolicy::adaptive (src/policy.rs:266-304) refuses initial <= 0,
+    multiplier <= 1, max_retries < 1 and max_jitter < initial or non-finite,
+    and a config file holding such a policy does not load."""
+    with pytest.raises(g.InvalidConfig):
+        g.JitterPolicy.adaptive(*args)
+    ini, mul, ret, mx = args
+    body = {"adaptive": {"initial": ini, "multiplier": mul, "max_retries": ret, "max_jitter": mx}}
+    with pytest.raises(P.PersistFailed) as e:
+        P.decode_jitter(body)
+    assert e.value.kind == "config"
+    with pytest.raises(g.InvalidConfig):
+        g.JitterPolicy.fixed(-1e-6)
+
+
+@pytest.mark.parametrize("precision,residual", [("double", "reevaluate_kernel"), ("double", "promote_storage"),
+                                                ("mixed", "reevaluate_kernel"), ("mixed", "promote_storage")])
+def test_residual_field_loads_either_value(tmp_path, precision, residual):
+    """The residual is read only with mixed precision (src/persist/config.rs:130-140);
+    either value loads, the model being factored again in its own form."""
+    gpr, *_ = P.read_model(_write(tmp_path, _config(precision=precision, residual=residual)))
+    assert gpr.precision == (g.Precision.MIXED if precision == "mixed" else g.Precision.FP64)
+
+
 def _config(**over):
     cfg = {"format_version": 1, "n": 2, "d": 1, "has_factor": False, "factor_kind": "llt",
            "kernel": {"rbf": {"lengthscale": {"value": 1.0, "lo": 1e-5, "hi": 1e5}}},
@@ -117,6 +144,7 @@ def test_unknown_format_version_is_rejected(tmp_path):
     {"x_transform": {"standardize": {"mean": [0.0], "std": [1.0]}}},
     {"factor_kind": "qr"},
     {"jitter": {"fixed": {"jitter": -1.0}}},
+    {"residual": "promote"},
     {"likelihood": {"noise_variance": {"value": 0.0, "lo": 1e-5, "hi": 1e5}}},
 ])
 def test_unsupported_configs_fail_on_the_host(tmp_path, over):

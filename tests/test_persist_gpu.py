@@ -47,6 +47,8 @@ def test_save_load_round_trip(gpu, tmp_path, precision):
     assert cfg["format_version"] == 1 and cfg["has_factor"] is False
     assert cfg["factor_kind"] == "llt" and (cfg["n"], cfg["d"]) == (n, d)
     assert cfg.get("precision", "double") == ("mixed" if precision == g.Precision.MIXED else "double")
+    # mixed residuals use the kernel evaluated again in f64 (ReevaluateKernel)
+    assert cfg.get("residual") == ("reevaluate_kernel" if precision == g.Precision.MIXED else None)
     t = stnp.load_file(str(tmp_path / "model.safetensors"))
     assert set(t) == {"x", "y"}
     np.testing.assert_array_equal(t["x"].reshape(-1), x)

@@ -153,8 +153,9 @@ SIGNATURES = {
     "gprx_lbfgs_minimize": (C.c_int, [ObjectiveFn, _vp, _dp, C.c_size_t, C.POINTER(LbfgsOptionsC),
                                       _dp, _dp, C.POINTER(C.c_uint64), _err]),
     "gprx_hip_exact_fit": (C.c_int, [_dp, C.c_size_t, C.c_size_t, _dp, C.POINTER(KernelProgram), _dp,
-                                     C.c_size_t, _dp, C.c_int32, C.POINTER(LbfgsOptionsC), _dp, _dp,
-                                     C.POINTER(C.c_uint64), C.POINTER(C.c_int32), _err]),
+                                     C.c_size_t, _dp, C.c_int32, C.POINTER(JitterPolicyC),
+                                     C.POINTER(LbfgsOptionsC), _dp, _dp, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_int32), _err]),
     "gprx_tr_default_options": (None, [C.POINTER(TrOptionsC)]),
     "gprx_trust_region_minimize": (C.c_int, [Objective2Fn, _vp, _dp, C.c_size_t, C.POINTER(TrOptionsC),
                                              _dp, _dp, C.POINTER(C.c_uint64), _err]),

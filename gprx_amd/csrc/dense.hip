@@ -925,19 +925,4 @@ void zero(T* p, size_t count, hipStream_t stream) {
 template void zero<double>(double*, size_t, hipStream_t);
 template void zero<float>(float*, size_t, hipStream_t);
 
-__global__ void k_mfma_probe(const double* A, const double* B, double* C) {
-  const int l = threadIdx.x;
-  const double a = A[(l % 16) * 4 + l / 16];  // A row-major 16 x 4
-  const double b = B[(l / 16) * 16 + l % 16];  // B row-major 4 x 16
-  d4 acc = d4{};
-  acc = mfma16(a, b, acc);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) C[(4 * (l / 16) + q) * 16 + l % 16] = acc[q];  // C row-major
-}
-
-void mfma_probe(const double* A, const double* B, double* C, hipStream_t stream) {
-  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, stream, A, B, C);
-  GPRX_HIP_CHECK(hipGetLastError());
-}
-
 }  // namespace gprx

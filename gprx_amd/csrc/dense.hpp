@@ -96,8 +96,4 @@ void zero(T* p, size_t count, hipStream_t stream);
 // Sets kernel attributes (large dynamic LDS) once, outside any graph capture.
 void prepare_dense_kernels();
 
-// MFMA layout probe: C(16x16) = A(16x4) B(4x16) through one
-// v_mfma_f64_16x16x4_f64 with the lane layout the kernels assume.
-void mfma_probe(const double* A, const double* B, double* C, hipStream_t stream);
-
 }  // namespace gprx

@@ -873,6 +873,10 @@ int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, dou
     if (saved != GPRX_FP64) {  // the model predicts in its own form again
       const int rc2 = m->evaluate(m->theta.data(), false, &v, nullptr, err);
       if (rc2 != GPRX_OK) return rc2;
+      // and gives back the FP64 factor's buffers (3 n_pad^2 doubles at most);
+      // a later FP64 fallback allocates them again
+      GPRX_HIP_CHECK(hipStreamSynchronize(s));
+      for (DevBuf<double>* b : {&m->A64, &m->X64, &m->T64}) b->release();
     }
     *jitter = factor_jitter;
     return GPRX_OK;

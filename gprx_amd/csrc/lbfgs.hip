@@ -411,9 +411,9 @@ int gprx_lbfgs_minimize(gprx_objective_fn fn, void* ctx, const double* x0, size_
 
 int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
                        const gprx_kernel_program* kernel, const double* log_theta, size_t p,
-                       const double* bounds, int32_t precision, const gprx_lbfgs_options* opt,
-                       double* best_log_theta, double* best_value, uint64_t* iterations,
-                       int32_t* best_start, gprx_error* err) {
+                       const double* bounds, int32_t precision, const gprx_jitter_policy* jitter,
+                       const gprx_lbfgs_options* opt, double* best_log_theta, double* best_value,
+                       uint64_t* iterations, int32_t* best_start, gprx_error* err) {
   if (!log_theta || !best_log_theta || p == 0) {
     put_error(err, GPRX_LENGTH_MISMATCH, "log_theta and best_log_theta are required");
     return GPRX_LENGTH_MISMATCH;
@@ -477,6 +477,26 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
   std::vector<double*> grp(k);
   std::vector<double> nlml(k);
   std::vector<gprx_error> errs(k);
+  // The batch factors every point without jitter.  A point whose factor
+  // fails is evaluated again alone under the model's jitter policy, as the
+  // reference's fit evaluates every point (src/gpr/exact_fit.rs:280-283).
+  const bool retry = jitter && jitter->max_retries > 0;
+  std::unique_ptr<gprx_hip_model, void (*)(gprx_hip_model*)> alone(nullptr, gprx_hip_destroy);
+  auto eval_alone = [&](int s) -> int {
+    if (!alone) {
+      gprx_hip_model* m1 = nullptr;
+      const int rc1 = gprx_hip_exact_create(x, n, d, y, kernel, thp[s], p, precision, jitter, &m1,
+                                            &errs[s]);
+      if (rc1 != GPRX_OK) return rc1;
+      alone.reset(m1);
+    }
+    return gprx_hip_exact_value_and_grad(alone.get(), thp[s], p, &nlml[s], grp[s], &errs[s]);
+  };
+  // The status of each start's first evaluation: the fit's error when the
+  // first start finds no finite point (src/optimizer/mod.rs:210-214 with
+  // best_value's re-evaluation, src/optimizer/adapter.rs:266-278).
+  std::vector<gprx_error> start_err(k);
+  std::vector<char> started(k, 0);
   for (;;) {
     std::vector<int> live;
     for (int r = 0; r < k; ++r)
@@ -508,6 +528,17 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
     for (int s = 0; s < kb; ++s) {
       LbfgsRun& run = runs[members[s]];
       if (run.done()) continue;
+      if (retry && errs[s].status == GPRX_CHOLESKY_FAILED) {
+        errs[s].status = eval_alone(s);
+        if (errs[s].status == GPRX_DEVICE_ERROR) {
+          if (err) *err = errs[s];
+          return GPRX_DEVICE_ERROR;
+        }
+      }
+      if (!started[members[s]]) {
+        started[members[s]] = 1;
+        start_err[members[s]] = errs[s];
+      }
       const double* z = run.ask();
       for (int i = 0; i < np; ++i) gr[s][i] *= dparam_dz(z[i], iv[i]);
       double f = nlml[s];
@@ -522,6 +553,10 @@ int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
     const bool ok = runs[r].best_f < kBarrierCost && std::isfinite(runs[r].best_f);
     if (!ok) {
       if (r == 0) {
+        if (start_err[0].status != GPRX_OK) {
+          if (err) *err = start_err[0];
+          return start_err[0].status;
+        }
         put_error(err, GPRX_NOT_CONVERGED, "the first start found no point with a finite cost");
         return GPRX_NOT_CONVERGED;
       }

@@ -168,12 +168,22 @@ struct ExactLogitObjective : Objective2 {
   gprx_hip_model* model;
   std::vector<Interval> iv;
   int n;
+  // evaluations since the run started, and the status of its first (the
+  // start), which src/optimizer/trust_region.rs:166-168 returns as the run's
+  // error when the start cannot be evaluated
+  int calls = 0;
+  int start_rc = GPRX_OK;
+  gprx_error start_err{};
   bool eval(const double* z, double* f, double* g, double* H) override {
     std::vector<double> th(n), gt(n), Ht((size_t)n * n);
     for (int i = 0; i < n; ++i) th[i] = z_to_param(z[i], iv[i]);
     gprx_error e{};
-    if (gprx_hip_exact_hessian(model, th.data(), (size_t)n, f, gt.data(), Ht.data(), &e) != GPRX_OK)
-      return false;
+    const int rc = gprx_hip_exact_hessian(model, th.data(), (size_t)n, f, gt.data(), Ht.data(), &e);
+    if (++calls == 1) {
+      start_rc = rc;
+      start_err = e;
+    }
+    if (rc != GPRX_OK) return false;
     if (!std::isfinite(*f)) return false;
     for (int i = 0; i < n; ++i) {
       const double ji = dparam_dz(z[i], iv[i]);
@@ -281,10 +291,17 @@ int gprx_hip_exact_fit_trust_region(gprx_hip_model* model, const double* log_the
   for (uint32_t r = 0; r <= o.restarts; ++r) {
     if (r > 0)
       for (int i = 0; i < np; ++i) user_to_z(sample_user(rng, obj.iv[i]), obj.iv[i], &z0[i]);
+    obj.calls = 0;
+    obj.start_rc = GPRX_OK;
     TrResult t = trust_region(obj, z0.data(), np, o);
     const bool ok = !t.start_failed && t.f < kBarrier && std::isfinite(t.f);
     if (!ok) {
       if (r == 0) {
+        // the start's own error (CholeskyFailed, InvalidHyperparameter, ...)
+        if (t.start_failed && obj.start_rc != GPRX_OK) {
+          if (err) *err = obj.start_err;
+          return obj.start_rc;
+        }
         put_error(err, GPRX_NOT_CONVERGED, "the first start could not be evaluated");
         return GPRX_NOT_CONVERGED;
       }

@@ -161,8 +161,9 @@ class Gpr:
         err = L.GprxError()
         oc = opt.options_c()
         bptr = L.dptr(self.bounds) if self.bounds is not None else None
+        jit = self.jitter.c()
         rc = L.lib().gprx_hip_exact_fit(L.dptr(x), n, d, L.dptr(y), C.byref(prog), L.dptr(theta),
-                                        theta.size, bptr, self.precision, C.byref(oc),
+                                        theta.size, bptr, self.precision, C.byref(jit), C.byref(oc),
                                         L.dptr(best), C.byref(val), C.byref(it), C.byref(start),
                                         C.byref(err))
         raise_for(rc, err)

@@ -338,16 +338,20 @@ int gprx_lbfgs_minimize(gprx_objective_fn fn, void* ctx, const double* x0, size_
  * default (1e-5, 1e5) of every parameter; src/optimizer/logit.rs:75-230)
  * from log_theta and opt->restarts further starts.  All starts run together:
  * every round evaluates each live start's next point in one batched GPU
- * call (gprx_hip_batch_*, jitter policy fixed(0)); a failed point costs
- * 1e300 (src/optimizer/adapter.rs:23,116-150).  Writes the best start's
- * best point, its NLML, its iterations and its index (0 = log_theta).
- * The first start failing fails the fit; a failed restart is skipped
+ * call (gprx_hip_batch_*, factored without jitter); a point whose factor
+ * fails there is evaluated again alone under `jitter` (NULL or fixed(0): no
+ * retry), as every evaluation of the reference's fit takes the model's
+ * policy (src/gpr/exact_fit.rs:280-283); a point that still fails costs 1e300
+ * (src/optimizer/adapter.rs:23,116-150).  Writes the best start's best
+ * point, its NLML, its iterations and its index (0 = log_theta).  The first
+ * start failing fails the fit with the status of its evaluation (e.g.
+ * GPRX_CHOLESKY_FAILED); a failed restart is skipped
  * (src/optimizer/mod.rs:210-226). */
 int gprx_hip_exact_fit(const double* x, size_t n, size_t d, const double* y,
                        const gprx_kernel_program* kernel, const double* log_theta, size_t p,
-                       const double* bounds, int32_t precision, const gprx_lbfgs_options* opt,
-                       double* best_log_theta, double* best_value, uint64_t* iterations,
-                       int32_t* best_start, gprx_error* err);
+                       const double* bounds, int32_t precision, const gprx_jitter_policy* jitter,
+                       const gprx_lbfgs_options* opt, double* best_log_theta, double* best_value,
+                       uint64_t* iterations, int32_t* best_start, gprx_error* err);
 
 /* TrustRegion settings (src/optimizer/trust_region.rs:19-80): 100
  * iterations, gradient tolerance sqrt(eps), radius 1, largest radius 100;
@@ -380,7 +384,9 @@ int gprx_trust_region_minimize(gprx_objective2_fn fn, void* ctx, const double* x
  * trust-region steps on the NLML through the logit map of `bounds` (as
  * gprx_hip_exact_fit), each point's Hessian formed on the GPU
  * (gprx_hip_exact_hessian), from log_theta and then each restart draw in
- * turn.  The model is left at the best point. */
+ * turn.  The model is left at the best point.  A start that cannot be
+ * evaluated fails the fit with its evaluation's status
+ * (src/optimizer/trust_region.rs:166-168). */
 int gprx_hip_exact_fit_trust_region(gprx_hip_model* model, const double* log_theta, size_t p,
                                     const double* bounds, const gprx_tr_options* opt,
                                     double* best_log_theta, double* best_value,

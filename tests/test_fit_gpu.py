@@ -174,3 +174,43 @@ def test_trust_region_fit_goldens(gpu, name):
     # a local method from sklearn's start: at sklearn's optimum or a better one
     assert lml >= gd["log_marginal_likelihood"] - 1e-3 * abs(gd["log_marginal_likelihood"])
     assert np.all(np.isfinite(f.get_params()))
+
+
+# Two coincident points: with noise 1e-18, 1 + sigma^2 rounds to 1 and the
+# factor meets an exactly-zero pivot unless jitter is added.
+XD = np.array([0.0, 0.0, 0.25, 0.6, 1.0])
+YD = np.array([0.1, 0.3, -0.4, 0.2, 0.8])
+BOUNDS_D = [1e-5, 1e5, 1e-20, 1e5]
+
+
+def test_fit_takes_the_models_jitter_policy(gpu):
+    """Every evaluation of the reference's fit uses the model's jitter policy
+    (src/gpr/exact_fit.rs:280-283): from a start whose factor needs jitter,
+    the fit with an adaptive policy runs, and its result is the policy's
+    NLML at the fitted point on the single-model path."""
+    print("[fit] jitter policy", flush=True)
+    pol = g.JitterPolicy.adaptive(1e-8, 10.0, 5, 1e-3)
+    gp = rbf_gpr(1.0, 1e-18).with_jitter_policy(pol).with_bounds(BOUNDS_D)
+    start = gp.factor(XD, 5, 1, YD)
+    v0 = start.neg_log_marginal_likelihood()
+    assert start.last_jitter() > 0.0  # the start itself needs jitter
+    f = rbf_gpr(1.0, 1e-18).with_jitter_policy(pol).with_bounds(BOUNDS_D).with_optimizer(
+        g.Lbfgs().with_max_iterations(50)).fit(XD, 5, 1, YD)
+    got = f.fit_result
+    print(f"[fit] start nlml {v0} fitted {got.value} after {got.iterations} iterations", flush=True)
+    assert np.isfinite(got.value) and got.value <= v0
+    single = rbf_gpr(1.0, 1e-18).with_jitter_policy(pol).with_bounds(BOUNDS_D).factor(XD, 5, 1, YD)
+    gr = np.empty(2)
+    assert single.value_and_gradient_into(got.params, gr) == pytest.approx(got.value, rel=1e-12)
+
+
+@pytest.mark.parametrize("opt", ["lbfgs", "trust_region"])
+def test_unfactorable_start_fails_with_its_own_error(gpu, opt):
+    """With the default fixed(0) policy the start above cannot be factored:
+    the fit returns that evaluation's CholeskyFailed
+    (src/optimizer/mod.rs:210-214, src/optimizer/trust_region.rs:166-168),
+    not a convergence failure."""
+    print(f"[fit] unfactorable start, {opt}", flush=True)
+    o = g.Lbfgs() if opt == "lbfgs" else g.TrustRegion()
+    with pytest.raises(g.CholeskyFailed):
+        rbf_gpr(1.0, 1e-18).with_bounds(BOUNDS_D).with_optimizer(o).fit(XD, 5, 1, YD)

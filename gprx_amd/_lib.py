@@ -122,6 +122,8 @@ SIGNATURES = {
     "gprx_hip_exact_insert": (C.c_int, [_vp, _dp, C.c_size_t, C.c_double, _dp, _err]),
     "gprx_hip_exact_predict": (C.c_int, [_vp, _dp, C.c_size_t, C.c_size_t, C.c_int32, _dp, _dp,
                                          _err]),
+    "gprx_hip_exact_predict_noalloc": (C.c_int, [_vp, _dp, C.c_size_t, C.c_size_t, C.c_int32, _dp,
+                                                 _dp, _err]),
     "gprx_hip_get_params": (C.c_int, [_vp, _dp, C.c_size_t]),
     "gprx_hip_last_jitter": (C.c_double, [_vp]),
     "gprx_hip_last_fallback": (C.c_int, [_vp]),

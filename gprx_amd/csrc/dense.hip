@@ -217,11 +217,11 @@ static const unsigned* tile_schedule(int mt, int nt, int lower, int krange, int 
 }
 
 template <typename T, int TT>
-static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream) {
+static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream, bool scheduled) {
   const int mt = g.M / TT, nt = g.N / TT;
   const int total = g.lower ? mt * (mt + 1) / 2 : mt * nt;
   int slots = total;
-  g.sched = tile_schedule(mt, nt, g.lower, g.krange, g.K / TT, stream, &slots);
+  g.sched = scheduled ? tile_schedule(mt, nt, g.lower, g.krange, g.K / TT, stream, &slots) : nullptr;
   if (!g.sched) slots = total;
   dim3 grid(slots, g.batch > 0 ? g.batch : 1);
   if (!ta && !tb)
@@ -246,20 +246,32 @@ constexpr int kSmallTileBelow = 256;
 constexpr int kTinyTileBelow = 64;
 
 template <typename T>
-void gemm(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream) {
+static void gemm_tiles(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream, bool scheduled) {
   const int mt = g.M / kTile, nt = g.N / kTile;
   const long total = (g.lower ? (long)mt * (mt + 1) / 2 : (long)mt * nt) * (g.batch > 0 ? g.batch : 1);
   if (total == 0) return;
   if (total < kTinyTileBelow)
-    launch_gemm<T, 32>(ta, tb, g, stream);
+    launch_gemm<T, 32>(ta, tb, g, stream, scheduled);
   else if (total < kSmallTileBelow)
-    launch_gemm<T, 64>(ta, tb, g, stream);
+    launch_gemm<T, 64>(ta, tb, g, stream, scheduled);
   else
-    launch_gemm<T, 128>(ta, tb, g, stream);
+    launch_gemm<T, 128>(ta, tb, g, stream, scheduled);
+}
+
+template <typename T>
+void gemm(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream) {
+  gemm_tiles<T>(ta, tb, g, stream, true);
+}
+
+template <typename T>
+void gemm_unscheduled(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream) {
+  gemm_tiles<T>(ta, tb, g, stream, false);
 }
 
 template void gemm<double>(bool, bool, const GemmArgs<double>&, hipStream_t);
 template void gemm<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
+template void gemm_unscheduled<double>(bool, bool, const GemmArgs<double>&, hipStream_t);
+template void gemm_unscheduled<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // 128 x 128 diagonal block: Cholesky + inverse in LDS, in the arithmetic of

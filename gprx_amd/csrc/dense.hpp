@@ -44,6 +44,11 @@ constexpr unsigned kNoTile = 0xffffffffu;
 template <typename T>
 void gemm(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream);
 
+// The same product without the cached XCD schedule (whose first use of a
+// shape allocates its device copy): each workgroup maps its own tile.
+template <typename T>
+void gemm_unscheduled(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream);
+
 // Factors the 128 x 128 diagonal block at A (lower part read; storage T,
 // arithmetic f64) and writes X = L^-1 (lower, zero above) at X, the log of
 // each pivot at logdiag[0..128).  A non-positive or non-finite pivot sets

@@ -10,9 +10,11 @@
 //   inverse   A^-1 = X^T X                               (lower tiles, MFMA)
 //   gradient  -1/2 <alpha alpha' - A^-1, dK/dtheta>      (fused: K recomputed from x)
 //             -1/2 sigma^2 sum_i (alpha_i^2 - A^-1_ii)   (src/gpr/exact_fit.rs:676-682)
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "hessian.hpp"
 #include "model.hpp"
@@ -407,6 +409,80 @@ void exact_upload_points(DevBuf<double>& dst, const double* src, int n, int d, l
   upload_points(dst, src, n, d, rows_pad, dp, s);
 }
 }  // namespace gprx
+
+// ---------------------------------------------------------------------------
+// Prediction without allocation (gprx_hip_exact_predict_noalloc).  Its work
+// space is one file-scope device array, `scratch`, of kNaScratchDoubles
+// doubles (32 MiB; each GPU the library runs on holds one until the process
+// ends).  Queries run in chunks of kNaCols columns, so the array bounds the
+// model's padded point count (kNaRows), not the number of queries:
+//   [0, 4096)               the chunk's query points, kNaCols x 32 dimensions
+//   [4096, +rows*cols)       K(x, x*) of the chunk, npad x kNaCols (f64)
+//   [.., +rows*cols)         V = L^-1 K(x, x*) (f64), or its f32 K and V
+//   [.., +5*kNaCols)         k(x*, x*), column sums of squares, mean, and the
+//                            chunk's mean and variance staged for the host
+// Host code never uses the array's name as an address: hipGetSymbolAddress
+// gives each device's copy once (kept in g_na_ptr), and every kernel
+// argument and copy uses that pointer.
+constexpr long kNaRows = 16384;
+constexpr long kNaCols = kTile;
+constexpr long kNaDims = 32;
+constexpr long kNaOffX = 0;
+constexpr long kNaOffK = kNaCols * kNaDims;
+constexpr long kNaOffV = kNaOffK + kNaRows * kNaCols;
+constexpr long kNaOffVec = kNaOffV + kNaRows * kNaCols;
+constexpr long kNaScratchDoubles = kNaOffVec + 5 * kNaCols;
+static_assert((kNaOffV * sizeof(double)) % 256 == 0, "f32 views of V start 256-byte aligned");
+
+static __device__ __align__(256) double scratch[kNaScratchDoubles];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_init_mu;
+double* g_na_ptr[kNaMaxDevices] = {};
+// One call at a time per device: calls share that device's array.
+std::mutex g_na_call_mu[kNaMaxDevices];
+
+// The current device's copy of `scratch` (looked up once per device).
+int na_scratch(int* dev_out, double** out, gprx_error* err) {
+  int dev = 0;
+  GPRX_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) {
+    set_error(err, GPRX_UNSUPPORTED, "device index beyond the no-allocation table");
+    return GPRX_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_na_init_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(err, GPRX_DEVICE_ERROR, "hipGetSymbolAddress of the prediction scratch failed");
+      return GPRX_DEVICE_ERROR;
+    }
+    g_na_ptr[dev] = static_cast<double*>(p);
+  }
+  *dev_out = dev;
+  *out = g_na_ptr[dev];
+  return GPRX_OK;
+}
+
+// The chunk's outputs: mean, and the variance clamped at 0 (src/gpr/shared.rs:798-806)
+// with the noise added unless latent.
+__global__ void k_na_finish(const double* __restrict__ mu, const double* __restrict__ kss,
+                            const double* __restrict__ ss, int qc, int want_var, int latent,
+                            double noise, double* __restrict__ out_mean, double* __restrict__ out_var) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= qc) return;
+  out_mean[j] = mu[j];
+  if (want_var) {
+    double lat = kss[j] - ss[j];
+    if (lat < 0.0) lat = 0.0;
+    out_var[j] = latent ? lat : lat + noise;
+  }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -959,6 +1035,89 @@ int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, si
         var[j] = variance_kind == GPRX_VARIANCE_LATENT ? lat : lat + noise;
       }
     }
+    return GPRX_OK;
+  });
+}
+
+int gprx_hip_exact_predict_noalloc(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                                   int32_t variance_kind, double* mean, double* var,
+                                   gprx_error* err) {
+  return guarded(err, [&]() -> int {
+    if (!model || model->kind != gprx_hip_model::kExact || !mean) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "bad handle or null output");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    auto* m = static_cast<ExactModel*>(model);
+    if (d != (size_t)m->d) {
+      set_error(err, GPRX_LENGTH_MISMATCH, "query dimension differs from training");
+      return GPRX_LENGTH_MISMATCH;
+    }
+    if (m->npad > kNaRows || m->dp > kNaDims) {
+      set_error(err, GPRX_UNSUPPORTED,
+                "the no-allocation predict takes models of at most 16384 (padded) points");
+      return GPRX_UNSUPPORTED;
+    }
+    if (!m->factored) {
+      set_error(err, GPRX_INVALID_CONFIG,
+                "the no-allocation predict needs a factored model (evaluate it first)");
+      return GPRX_INVALID_CONFIG;
+    }
+    if (q == 0) return GPRX_OK;
+    if (!xs || !all_finite(xs, q * d)) {
+      set_error(err, GPRX_NON_FINITE_INPUT, "non-finite query");
+      return GPRX_NON_FINITE_INPUT;
+    }
+    int dev = 0;
+    double* sp = nullptr;
+    const int rs = na_scratch(&dev, &sp, err);
+    if (rs != GPRX_OK) return rs;
+    std::lock_guard<std::mutex> lock(g_na_call_mu[dev]);
+    hipStream_t s = m->stream;
+    m->upload_program(m->theta.data(), m->last_jitter);
+    const long np = m->npad;
+    double* xq = sp + kNaOffX;
+    double* Ks = sp + kNaOffK;
+    double* V = sp + kNaOffV;
+    float* K32 = reinterpret_cast<float*>(sp + kNaOffV);
+    float* V32 = K32 + np * kNaCols;
+    double* kss = sp + kNaOffVec;
+    double* ss = kss + kNaCols;
+    double* mu = ss + kNaCols;
+    double* omean = mu + kNaCols;
+    double* ovar = omean + kNaCols;
+    const bool want_var = var != nullptr;
+    const double noise = std::exp(m->theta[m->hp.n_params]);
+    for (size_t j0 = 0; j0 < q; j0 += (size_t)kNaCols) {
+      const int qc = (int)std::min<size_t>((size_t)kNaCols, q - j0);
+      // the chunk's points, column-major qc x d into kNaCols x dp with the
+      // padded dimensions zero
+      zero<double>(xq, (size_t)kNaCols * m->dp, s);
+      GPRX_HIP_CHECK(hipMemcpy2DAsync(xq, sizeof(double) * kNaCols, xs + j0, sizeof(double) * q,
+                                      sizeof(double) * qc, d, hipMemcpyHostToDevice, s));
+      DevPoints pq{xq, kNaCols, qc};
+      fill_cross<double>(m->dp, m->prog.p, m->pts(), pq, Ks, np, s);
+      gemv_t<double>(Ks, np, (int)np, (int)kNaCols, m->alpha.p, mu, 0, 0, 0, 0, s);
+      if (want_var) {
+        fill_diag(m->dp, m->prog.p, pq, kss, s);
+        if (m->cur_form == GPRX_FP64) {
+          gemm_call<double>(false, false, m->X64.p, np, Ks, np, V, np, (int)np, (int)kNaCols,
+                            (int)np, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s, false);
+          col_sumsq<double>(V, np, (int)np, (int)kNaCols, ss, s);
+        } else {
+          fill_cross<float>(m->dp, m->prog.p, m->pts(), pq, K32, np, s);
+          gemm_call<float>(false, false, m->X32.p, np, K32, np, V32, np, (int)np, (int)kNaCols,
+                           (int)np, 1.0, 0.0, kKhiRow, 0, kOrderColFastRev, s, false);
+          col_sumsq<float>(V32, np, (int)np, (int)kNaCols, ss, s);
+        }
+      }
+      hipLaunchKernelGGL(k_na_finish, dim3(1), dim3(kNaCols), 0, s, mu, kss, ss, qc, want_var ? 1 : 0,
+                         variance_kind == GPRX_VARIANCE_LATENT ? 1 : 0, noise, omean, ovar);
+      GPRX_HIP_CHECK(hipGetLastError());
+      GPRX_HIP_CHECK(hipMemcpyAsync(mean + j0, omean, sizeof(double) * qc, hipMemcpyDeviceToHost, s));
+      if (want_var)
+        GPRX_HIP_CHECK(hipMemcpyAsync(var + j0, ovar, sizeof(double) * qc, hipMemcpyDeviceToHost, s));
+    }
+    GPRX_HIP_CHECK(hipStreamSynchronize(s));
     return GPRX_OK;
   });
 }

@@ -42,7 +42,7 @@ bool all_finite(const double* p, size_t n);
 template <typename T>
 inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long ldb, T* C, long ldc,
                       int M, int N, int K, double alpha, double beta, int krange, int lower,
-                      int order, hipStream_t s) {
+                      int order, hipStream_t s, bool scheduled = true) {
   GemmArgs<T> g{};
   g.A = A;
   g.lda = lda;
@@ -59,7 +59,10 @@ inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long l
   g.lower = lower;
   g.order = order;
   g.batch = 1;
-  gemm<T>(ta, tb, g, s);
+  if (scheduled)
+    gemm<T>(ta, tb, g, s);
+  else
+    gemm_unscheduled<T>(ta, tb, g, s);
 }
 
 // Streams of the recursive factor.  `main` carries the recursion and its

@@ -183,6 +183,21 @@ int gprx_hip_exact_delete(gprx_hip_model* model, size_t index, double* nlml, gpr
 int gprx_hip_exact_predict(gprx_hip_model* model, const double* xs, size_t q, size_t d,
                            int32_t variance_kind, double* mean, double* var, gprx_error* err);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, code that must not call hipMalloc).  The same
+ * predictions as gprx_hip_exact_predict, computed without any allocation:
+ * the work space is a file-scope device array of 32 MiB that each GPU the
+ * library runs on holds until the process ends, and queries run in chunks
+ * of 128, so any q is taken.  Limits: the model must already be factored
+ * (GPRX_INVALID_CONFIG otherwise: evaluate it first) and hold at most 16384
+ * points after padding to 128 (GPRX_UNSUPPORTED otherwise).  Every call on
+ * a GPU shares that GPU's array and leaves the last call's contents in it,
+ * so calls that overlapped on one GPU would race: the library runs them one
+ * at a time per device (each returns after its stream work has finished). */
+int gprx_hip_exact_predict_noalloc(gprx_hip_model* model, const double* xs, size_t q, size_t d,
+                                   int32_t variance_kind, double* mean, double* var,
+                                   gprx_error* err);
+
 /* Current parameters (FittedGpr::get_params) and the jitter the last factor
  * took. */
 int gprx_hip_get_params(const gprx_hip_model* model, double* log_theta, size_t p);

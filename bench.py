@@ -4,9 +4,12 @@
 A step is one NLML + gradient evaluation (all 10 log-parameters) at fixed
 parameters, with the training data resident in HBM: the reference's own perf
 clock (FittedGpr::value_and_gradient_into after Gpr<Fixed>::factor).
-`value` is the mixed form's evaluations/s; the FP64 form (the correctness
-oracle form on the GPU) is timed in the same run and reported beside it with
-the mixed form's errors against it.
+`value` is the FP64 form's evaluations/s: the form that meets its accuracy
+bar (1e-10 against the CPU f64 path).  The mixed form (f32 MFMA factor and
+inverse, f64 refinement of alpha) is timed in the same run and reported in
+`mixed` with its errors against FP64; it does not meet its 1e-8 / 1e-6 bar.
+The 256-problem batch of the batched workload is timed beside it, batched
+against the looped single-problem path (`batch256`).
 
 Every library call returns after a stream synchronisation, so the host clock
 around K calls brackets exactly K evaluations.
@@ -43,6 +46,9 @@ PEAK_HBM = 8000.0  # GB/s
 # WRITE_SIZE on that launch, each scaled by what the counter reads for a 1 GiB
 # device copy (x2.0 and x1.0); the run is summarised under profiles/.
 XTX_TRAFFIC_N16384 = {"mixed": 2.957e+10, "fp64": 6.610e+10}
+# HBM-side bytes of one whole evaluation at n = 16384: FETCH_SIZE + WRITE_SIZE
+# summed over the evaluation's dispatches, scaled as above.
+EVAL_TRAFFIC_N16384 = {"fp64": 1.748e11}
 
 
 def exact_flops(n: int, d: int) -> float:
@@ -95,28 +101,37 @@ def time_form(gx, wl, precision, steps, warmup):
     return model, val, grad.copy(), wall, phases / steps, float(np.mean(evs))
 
 
-def cpu_baseline(wl, cpu_n):
-    """The oracle (a restatement of gprx's CPU algorithm: Cholesky, solve(L, I),
-    the W contraction) on the first cpu_n points, all host threads, one
-    evaluation; scaled to n by the n^3 cost."""
+def cpu_baseline(wl, warmup, reps):
+    """The oracle (a restatement of gprx's CPU algorithm: Cholesky,
+    solve(L, I), the W contraction) on the whole workload, all host threads,
+    on the reference's clock (compare/perf/gprx/src/timing.rs: discard
+    `warmup` evaluations, then the median of `reps`).  One evaluation at
+    n = 16384 takes about a minute on 16 host threads, so the default is
+    PERF_WARMUP = 1 and PERF_REPS = 1, settings of that clock; each finished
+    phase prints a line."""
     import oracle
 
-    n0 = min(cpu_n, wl.n)
-    X = wl.x.reshape(wl.d, wl.n)[:, :n0].reshape(-1)
-    y = wl.y[:n0]
     ops = wl.kernel().program()
     th = wl.theta()
-    t0 = time.perf_counter()
-    oracle.exact_value_grad(ops, th, X, n0, wl.d, y, blocked=True)
-    t = time.perf_counter() - t0
-    scale = (n0 / wl.n) ** 3
+    oracle.set_verbose(True)
+    samples = []
+    for i in range(warmup + reps):
+        print(f"[bench] CPU baseline: oracle evaluation {i + 1} of {warmup + reps} at n={wl.n}",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        oracle.exact_value_grad(ops, th, wl.x, wl.n, wl.d, wl.y, blocked=True)
+        if i >= warmup:
+            samples.append(time.perf_counter() - t0)
+    oracle.set_verbose(False)
+    t = float(np.median(samples))
     return {
-        "value": scale / t,
+        "value": 1.0 / t,
         "unit": "evals/s",
         "cores": oracle.threads(),
         "kind": "port",
-        "sample": f"one NLML+grad eval of the oracle at n={n0} d={wl.d} ({t:.2f} s), "
-                  f"scaled to n={wl.n} by (n/{n0})^3",
+        "sample": f"the whole workload (n={wl.n} d={wl.d}): median of {reps} oracle NLML+grad "
+                  f"evaluation(s) after {warmup} discarded, {t:.1f} s each; the oracle restates "
+                  f"gprx's CPU algorithm (Cholesky, solve(L, I), W contraction) in C with OpenMP",
     }
 
 
@@ -128,6 +143,12 @@ def bench_exact(args, gx):
     m64, v64, g64, wall64, ph64, ev64 = time_form(gx, wl, gx.Precision.FP64, args.steps, args.warmup)
     print(f"[bench] FP64 {wall64 / args.steps * 1e3:.1f} ms/eval; timing MIXED", file=sys.stderr,
           flush=True)
+    if "mixed" not in args.forms.split(","):
+        # profiling runs of the FP64 evaluation alone
+        m64.close()
+        return {"metric": "NLML+grad evals/s, exact GPR FP64 only", "value": args.steps / wall64,
+                "unit": "evals/s", "ms_per_step": wall64 / args.steps * 1e3,
+                "phase_ms": [round(v, 3) for v in ph64], "eval_ms_hip_events": ev64}
     m32, v32, g32, wall32, ph32, ev32 = time_form(gx, wl, gx.Precision.MIXED, args.steps, args.warmup)
     print(f"[bench] MIXED {wall32 / args.steps * 1e3:.1f} ms/eval", file=sys.stderr, flush=True)
 
@@ -135,10 +156,13 @@ def bench_exact(args, gx):
     evals32 = args.steps / wall32
     F = exact_flops(wl.n, wl.d)
     lauum = inverse_product_flops(wl.n)
-    # Dominant kernel: the X^T X tile-GEMM launch (one launch per evaluation),
-    # timed with HIP events on the model's stream.
+    # The X^T X tile-GEMM launch (one per evaluation), timed with HIP events
+    # on the model's stream.
     ach32 = lauum / (ph32[3] * 1e-3) / 1e12
     ach64 = lauum / (ph64[3] * 1e-3) / 1e12
+    # The whole evaluation, from HIP events on the model's stream around it.
+    eval64 = F / (ev64 * 1e-3) / 1e12
+    eval32 = F / (ev32 * 1e-3) / 1e12
 
     extra = {}
     if not args.no_predict:
@@ -150,44 +174,66 @@ def bench_exact(args, gx):
         t32 = time.perf_counter() - t0
         extra["predict_ms"] = {"fp64": t64 * 1e3, "mixed": t32 * 1e3, "q": wl.q}
         extra["mixed_predict_rel_err"] = {"mean": rel(p32.mean, p64.mean), "variance": rel(p32.variance, p64.variance)}
-
+    e_nlml, e_grad = rel(v32, v64), rel(g32, g64)
+    mixed = {
+        "evals_per_s": evals32, "ms_per_step": wall32 / args.steps * 1e3,
+        "vs_fp64": evals32 / evals64,
+        "rel_err_vs_fp64": {"nlml": e_nlml, "grad_max": e_grad,
+                            "grad": [float(v) for v in np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)]},
+        "meets_target": {"nlml_le_1e-8": e_nlml <= 1e-8, "grad_le_1e-6": e_grad <= 1e-6,
+                         "speedup_ge_3": evals32 / evals64 >= 3.0},
+        "refine_steps": m32.last_refine_steps(), "fallback": m32.last_fallback(),
+        "roofline": {"bound": "mfma", "kernel": "whole mixed evaluation", "achieved": eval32,
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": eval32 / PEAK_FP32},
+    }
+    m64.close()
+    m32.close()
     out = {
         "metric": "NLML+grad evals/s, exact GPR n=16384 d=8 ARD-RBF: FP32-mixed vs FP64 on GPU",
-        "value": evals32,
+        "value": evals64,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall32 / args.steps * 1e3,
+        "ms_per_step": wall64 / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mixed (f64 fill, log-det, refinement)",
+        "dtype": "f64",
         "data": "synthetic (seeded SeededRng, seed 2)",
         "config": {"workload": "exact GPR NLML+grad, Constant x RbfArd", "n": wl.n, "d": wl.d,
                    "params": int(wl.theta().size), "parallelism": "single GPU"},
-        "fp64_evals_per_s": evals64,
-        "fp64_ms_per_step": wall64 / args.steps * 1e3,
-        "mixed_vs_fp64": evals32 / evals64,
-        "mixed_rel_err_vs_fp64": {"nlml": rel(v32, v64), "grad_max": rel(g32, g64),
-                                  "grad": [float(v) for v in np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)]},
-        "mixed_refine_steps": m32.last_refine_steps(),
-        "mixed_fallback": m32.last_fallback(),
+        "value_form": "FP64 (meets 1e-10 against the CPU f64 oracle); the mixed form is in `mixed`",
+        "mixed": mixed,
         "phase_ms": {"names": ["fill", "factor+inverse", "alpha", "XtX", "contract"],
-                     "mixed": [round(v, 3) for v in ph32], "fp64": [round(v, 3) for v in ph64]},
-        "eval_tflops": {"mixed": F / (wall32 / args.steps) / 1e12, "fp64": F / (wall64 / args.steps) / 1e12,
-                        "algorithmic_flop_per_eval": F},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm<float,T,N,128> (X^T X, one launch per eval)",
-                     "achieved": ach32, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
-                     "traffic": XTX_TRAFFIC_N16384["mixed"] if wl.n == 16384 else None},
-        "roofline_fp64": {"bound": "mfma", "kernel": "k_gemm<double,T,N,128> (X^T X)", "achieved": ach64,
-                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64,
-                          "traffic": XTX_TRAFFIC_N16384["fp64"] if wl.n == 16384 else None},
+                     "fp64": [round(v, 3) for v in ph64], "mixed": [round(v, 3) for v in ph32]},
+        "algorithmic_flop_per_eval": F,
+        "roofline": {"bound": "mfma",
+                     "kernel": "whole FP64 evaluation, HIP events on its stream (by rocprof total time "
+                               "its dominant kernel is k_gemm<double,N,N,128>, the factor's "
+                               "trailing and inverse products)",
+                     "achieved": eval64, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": eval64 / PEAK_FP64,
+                     "traffic": EVAL_TRAFFIC_N16384.get("fp64") if wl.n == 16384 else None},
+        "roofline_xtx": {"bound": "mfma", "kernel": "k_gemm<double,T,N,128> (X^T X, one launch per eval)",
+                         "achieved": ach64, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64,
+                         "traffic": XTX_TRAFFIC_N16384["fp64"] if wl.n == 16384 else None},
+        "roofline_xtx_mixed": {"bound": "mfma", "kernel": "k_gemm<float,T,N,128> (X^T X)",
+                               "achieved": ach32, "peak": PEAK_FP32, "unit": "TFLOP/s",
+                               "frac": ach32 / PEAK_FP32},
         **extra,
     }
+    if not args.no_batch:
+        print("[bench] 256-problem batch (FP64): batched and looped", file=sys.stderr, flush=True)
+        forms, Fb = time_batch(gx, ("fp64",), steps=2, warmup=1)
+        fb = forms["fp64"]
+        out["batch256"] = {"workload": "256 exact ARD-RBF problems, n in [256, 4096], d in {2,4,8,16} "
+                                       "(seed 5), FP64",
+                           "batched_ms": fb["batched_ms"], "looped_ms": fb["looped_ms"],
+                           "batched_vs_looped": fb["speedup"],
+                           "batched_vs_looped_err": fb["batched_vs_looped_err"],
+                           "eval_tflops": Fb / (fb["batched_ms"] * 1e-3) / 1e12}
     if not args.no_cpu_baseline:
-        print(f"[bench] CPU baseline (oracle, n={args.cpu_n})", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_warmup, args.cpu_reps)
     return out
 
 
@@ -282,7 +328,11 @@ def bench_svgp(args, gx):
     }
 
 
-def bench_batch(args, gx):
+def time_batch(gx, forms, steps, warmup):
+    """The 256-problem batched workload in each precision of `forms`: the
+    batched evaluation and the looped single-problem path, each timed over
+    `steps` calls after `warmup`.  Returns ({form: timings and errors},
+    algorithmic FLOP of one batch)."""
     from gprx_amd.configs import batched_problems
 
     raw = batched_problems(k=256)
@@ -297,10 +347,12 @@ def bench_batch(args, gx):
         a, e = np.asarray(a, float), np.asarray(e, float)
         return float(np.max(np.abs(a - e) / np.maximum(np.abs(e), 1.0)))
 
-    for name, prec in (("fp64", gx.Precision.FP64), ("mixed", gx.Precision.MIXED)):
+    precs = {"fp64": gx.Precision.FP64, "mixed": gx.Precision.MIXED}
+    for name in forms:
+        prec = precs[name]
         print(f"[bench] batch of {len(probs)}: {name} batched", file=sys.stderr, flush=True)
         bt = gx.GprBatch(probs, precision=prec)
-        (nl, gr, er), wall_b = _timed(lambda: bt.value_and_gradient(thetas), args.steps, args.warmup)
+        (nl, gr, er), wall_b = _timed(lambda: bt.value_and_gradient(thetas), steps, warmup)
         batched_fallbacks = int((bt.last_fallback() != 0).sum())
         bt.close()
         print(f"[bench] batch of {len(probs)}: {name} looped single-problem path", file=sys.stderr,
@@ -311,11 +363,11 @@ def bench_batch(args, gx):
         def loop():
             return [mdl.value_and_gradient_into(t, g_) for mdl, t, g_ in zip(models, thetas, grads)]
 
-        vals, wall_l = _timed(loop, args.steps, args.warmup)
+        vals, wall_l = _timed(loop, steps, warmup)
         fallbacks = sum(mdl.last_fallback() != 0 for mdl in models)
         for mdl in models:
             mdl.close()
-        form = {"batched_ms": wall_b / args.steps * 1e3, "looped_ms": wall_l / args.steps * 1e3,
+        form = {"batched_ms": wall_b / steps * 1e3, "looped_ms": wall_l / steps * 1e3,
                 "speedup": wall_l / wall_b, "failed": sum(e is not None for e in er),
                 "batched_fp64_fallbacks": batched_fallbacks,
                 "looped_fp64_fallbacks": int(fallbacks),
@@ -329,22 +381,27 @@ def bench_batch(args, gx):
             form["looped_vs_fp64_err"] = {"nlml": err1(vals, ref[0]),
                                           "grad": max(err1(a, b) for a, b in zip(grads, ref[1]))}
         out_forms[name] = form
+    return out_forms, F
+
+
+def bench_batch(args, gx):
+    out_forms, F = time_batch(gx, ("fp64", "mixed"), args.steps, args.warmup)
     fb, fm = out_forms["fp64"], out_forms["mixed"]
     ach = F / (fm["batched_ms"] * 1e-3) / 1e12
     ach64 = F / (fb["batched_ms"] * 1e-3) / 1e12
     return {
         "metric": "batch evals/s, 256 exact ARD-RBF problems n in [256, 4096], batched vs looped",
-        "value": 1e3 / fm["batched_ms"], "unit": "batch evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": fm["batched_ms"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed",
+        "value": 1e3 / fb["batched_ms"], "unit": "batch evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fb["batched_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded SeededRng, seed 5)",
-        "config": {"workload": "batched exact GPR NLML+grad", "problems": len(probs),
-                   "sum_n": int(sum(p[3] for p in probs)), "parallelism": "single GPU"},
+        "config": {"workload": "batched exact GPR NLML+grad", "problems": 256,
+                   "parallelism": "single GPU"},
         "fp64": fb, "mixed": fm, "algorithmic_flop_per_batch": F,
-        "roofline": {"bound": "mfma", "kernel": "whole batched evaluation", "achieved": ach,
-                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32, "traffic": None},
-        "roofline_fp64": {"bound": "mfma", "kernel": "whole batched evaluation", "achieved": ach64,
-                          "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64},
+        "roofline": {"bound": "mfma", "kernel": "whole batched FP64 evaluation", "achieved": ach64,
+                     "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "traffic": None},
+        "roofline_mixed": {"bound": "mfma", "kernel": "whole batched mixed evaluation", "achieved": ach,
+                           "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": ach / PEAK_FP32},
     }
 
 
@@ -615,8 +672,15 @@ def main():
                     default="exact")
     ap.add_argument("--restarts", type=int, default=16)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-warmup", type=int, default=1,
+                    help="oracle evaluations discarded before timing (PERF_WARMUP)")
+    ap.add_argument("--cpu-reps", type=int, default=1,
+                    help="timed oracle evaluations, median reported (PERF_REPS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="fp64,mixed",
+                    help="exact workload: 'fp64' times the FP64 evaluation alone (profiling runs)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-problem batch in the exact workload's line")
     ap.add_argument("--no-predict", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

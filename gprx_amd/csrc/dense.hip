@@ -274,9 +274,10 @@ template void gemm_unscheduled<double>(bool, bool, const GemmArgs<double>&, hipS
 template void gemm_unscheduled<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// 128 x 128 diagonal block: Cholesky + inverse in LDS, in the arithmetic of
-// the storage type C (f64 for the FP64 factor; f32 for the f32 factor of
-// the mixed form, like every other step of that factor).
+// 128 x 128 diagonal block: Cholesky + inverse in LDS, in f64 arithmetic
+// for both forms: the mixed form's f32 blocks are read, factored and
+// inverted in f64, and their inverse is written back rounded to f32 (its
+// log-pivots stay f64).
 //
 // Layout in LDS (column stride DS): L lower (r >= c) at s[c * DS + r]; the
 // inverse X = L^-1 stored transposed above the diagonal, X[r][c] (r > c) at
@@ -571,11 +572,14 @@ __device__ __forceinline__ void update_blocks(C* s, int c0, int bi, int bj, bool
   }
 }
 
+// T is the storage type of A and X; the block is factored and inverted in
+// f64 in LDS whatever T is (the mixed form's diagonal blocks are FP64).
 template <typename T>
 __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
-                                double* __restrict__ logdiag, int* info, T* s, T* xd,
+                                double* __restrict__ logdiag, int* info, double* s, double* xd,
                                 int* sbad) {
-  constexpr int DS = DiagCfg<T>::DS;
+  using C = double;
+  constexpr int DS = DiagCfg<C>::DS;
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   if (tid == 0) {
     *sbad = 0;
@@ -596,7 +600,7 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
       const int r = e & 127, c = e >> 7;
-      s[c * DS + r] = (r >= c) ? v[i] : (T)0;
+      s[c * DS + r] = (r >= c) ? (C)v[i] : (C)0;
     }
   }
   __syncthreads();
@@ -639,16 +643,16 @@ __device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long ld
   // HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
   invert_diag16(s, xd);
   __syncthreads();
-  inverse_level<T, 1>(s, xd);
-  inverse_level<T, 2>(s, xd);
-  inverse_level<T, 4>(s, xd);
+  inverse_level<C, 1>(s, xd);
+  inverse_level<C, 2>(s, xd);
+  inverse_level<C, 4>(s, xd);
 #pragma unroll
   for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
     T v[kBatch];
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       const int e = tid + 256 * (b0 + i);
-      v[i] = x_at<T>(s, xd, e & 127, e >> 7);
+      v[i] = (T)x_at<C>(s, xd, e & 127, e >> 7);
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
@@ -664,8 +668,8 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const T* A, long lda, T* X, 
                                                     double* logdiag, int* info, long sa, long sx,
                                                     long slog) {
   extern __shared__ double smem[];
-  T* s = reinterpret_cast<T*>(smem);
-  T* xd = s + kTile * DiagCfg<T>::DS;
+  double* s = smem;
+  double* xd = s + kTile * DiagCfg<double>::DS;
   int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const long z = blockIdx.x;
   potrf_block_lds<T>(A + z * sa, lda, X + z * sx, ldx, logdiag + z * slog, info, s, xd, sbad);
@@ -677,17 +681,19 @@ __global__ __launch_bounds__(256) void k_potrf_diag_list(T* A, T* X, const long*
                                                          const long* log_off, double* logdiag,
                                                          int* info, const int* info_idx) {
   extern __shared__ double smem[];
-  T* s = reinterpret_cast<T*>(smem);
-  T* xd = s + kTile * DiagCfg<T>::DS;
+  double* s = smem;
+  double* xd = s + kTile * DiagCfg<double>::DS;
   int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const int b = blockIdx.x;
   potrf_block_lds<T>(A + a_off[b], ld[b], X + x_off[b], ld[b], logdiag + log_off[b],
                      info + info_idx[b], s, xd, sbad);
 }
 
+// LDS of the diagonal kernel: the block and its inverse's diagonal in f64
+// for either storage type.
 template <typename T>
 constexpr size_t kDiagLds =
-    (size_t)kTile * DiagCfg<T>::DS * sizeof(T) + (kTile + 1) * sizeof(T) + 16;
+    (size_t)kTile * DiagCfg<double>::DS * sizeof(double) + (kTile + 1) * sizeof(double) + 16;
 
 template <typename T>
 static void set_diag_lds_once() {

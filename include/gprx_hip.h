@@ -81,12 +81,15 @@ typedef struct gprx_kernel_program {
   size_t n_ops;
 } gprx_kernel_program;
 
-/* Arithmetic form.  FP64: everything in f64 (the oracle form).  MIXED: kernel
- * fills and all dense trailing products in f32 on MFMA with f32 accumulation;
- * f64 for the diagonal-block factorizations, the log-determinant, small
- * m x m factors, and the iterative refinement of alpha with f64 residuals;
- * falls back to FP64 when refinement stalls or when the f32 factor needs
- * more jitter than a set bound (gprx_hip_set_mixed_jitter_bound). */
+/* Arithmetic form.  FP64: everything in f64 (the oracle form).  MIXED: all
+ * dense trailing and inverse products in f32 on MFMA with f32 accumulation,
+ * and the sparse models' kernel fills in f32; f64 for the exact model's
+ * kernel fill (rounded to f32 for the factor, kept in f64 for the
+ * residuals), the 128 x 128 diagonal-block factorizations and their
+ * inverses, the log-determinant, the small m x m factors, and the iterative
+ * refinement of alpha with f64 residuals; falls back to FP64 when
+ * refinement stalls or when the f32 factor needs more jitter than a set
+ * bound (gprx_hip_set_mixed_jitter_bound). */
 typedef enum gprx_precision { GPRX_FP64 = 0, GPRX_MIXED = 1 } gprx_precision;
 
 /* Why the last mixed evaluation ran in FP64 (gprx_hip_last_fallback). */

@@ -527,8 +527,9 @@ def bench_hessian(args, gx):
 
 
 def bench_online(args, gx):
-    """OnlineGpr::insert at config 2's kernel: from n = 16384 - K points,
-    K insertions inside the 128-row padding (FP64), each O(n^2)."""
+    """OnlineGpr::insert and delete at config 2's kernel: from n = 16384 - K
+    points, K insertions inside the 128-row padding (FP64), each O(n^2),
+    then K deletions at seeded positions, each O(n^2)."""
     from gprx_amd.configs import exact_ard
 
     K = args.steps
@@ -551,7 +552,25 @@ def bench_online(args, gx):
     ref = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).factor(wl.x, wl.n, wl.d, wl.y)
     err = abs(nlml - ref.neg_log_marginal_likelihood()) / abs(ref.neg_log_marginal_likelihood())
     ref.close()
+    # OnlineGpr::delete: K deletions at seeded positions, each O(n^2) on X = L^-1
+    from gprx_amd.rng import SeededRng
+
+    r = SeededRng(7)
+    keep = list(range(wl.n))
+    print(f"[bench] online: {K} deletions from {wl.n} points", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        idx = r.up_to(len(keep) - 1)
+        nlml_d = model.delete(idx)
+        keep.pop(idx)
+    wall_d = time.perf_counter() - t0
+    kk = np.array(keep)
+    xk = np.concatenate([wl.x[q * wl.n + kk] for q in range(wl.d)])
+    ref = gx.Gpr(wl.kernel(), gx.GaussianLikelihood(wl.noise)).factor(xk, kk.size, wl.d, wl.y[kk])
+    err_d = abs(nlml_d - ref.neg_log_marginal_likelihood()) / abs(ref.neg_log_marginal_likelihood())
+    ref.close()
     model.close()
+    ms_d = wall_d / K * 1e3
     ms = wall / K * 1e3
     n_avg = wl.n - K / 2
     # two passes over the lower triangle of X (X k, X^T l)
@@ -568,6 +587,15 @@ def bench_online(args, gx):
         "roofline": {"bound": "hbm", "kernel": "one insertion (two triangular matrix-vector passes)",
                      "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s", "frac": ach / PEAK_HBM,
                      "traffic": None},
+        "delete_ms": ms_d,
+        "deletions": K,
+        "delete_nlml_rel_err_vs_fresh_factor": err_d,
+        # a deletion reads X's lower triangle and writes the shifted one, then
+        # reads the new one twice for alpha
+        "delete_roofline": {"bound": "hbm", "kernel": "one deletion (X downdate, alpha passes)",
+                            "achieved": 4 * (wl.n * wl.n / 2) * 8 / (ms_d * 1e-3) / 1e9,
+                            "peak": PEAK_HBM, "unit": "GB/s",
+                            "frac": 4 * (wl.n * wl.n / 2) * 8 / (ms_d * 1e-3) / 1e9 / PEAK_HBM},
     }
 
 

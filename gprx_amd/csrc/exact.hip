@@ -583,6 +583,97 @@ __global__ void k_alpha_extend(double* alpha, const double* __restrict__ w, int 
   if (j == n) alpha[n] = c;
 }
 
+// Deletion of point p from the factor, on X = L^-1 alone (OnlineGpr::delete,
+// src/gpr/online.rs:564-620, which downdates its LDL^T, src/linalg/ldlt.rs:113).
+// With L = [L11 0 0; r^T d 0; L31 l L33], removing row and column p leaves
+// L31 and L11 and turns L33 into L33' = L33 G, G G^T = I + v v^T with
+// v = L33^-1 l = -X[p+1:, p] / X[p][p]: a rank-one update, always defined.
+// In terms of X, rows above p keep X11 and the trailing rows become
+//   [X31', X33'] = G^-1 ([X31, X33] + v [X[p, :p], 0]),
+// and G^-1 applies as a prefix sum per column: with beta_k = 1 + sum_{i<=k} v_i^2,
+//   y_k = (m_k - v_k S_k / beta_{k-1}) / G_kk,  S_k = sum_{i<k} v_i m_i,
+//   G_kk = sqrt(beta_k / beta_{k-1}).
+// One workgroup per new column jn (old column jn, or jn + 1 from p on) reads
+// X and writes the shifted matrix into Y; the last new row and column are
+// padding (identity).  v, 1 / beta_{k-1} and 1 / G_kk arrive from the host.
+__global__ __launch_bounds__(256) void k_delete_rows(const double* __restrict__ X,
+                                                     double* __restrict__ Y, long ld, int p,
+                                                     const double* __restrict__ v,
+                                                     const double* __restrict__ ibp,
+                                                     const double* __restrict__ ig) {
+  const int jn = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double* yc = Y + (long)jn * ld;
+  if (jn == ld - 1) {  // new padding column: e_{ld-1}
+    for (long i = tid; i < ld; i += 256) yc[i] = (i == ld - 1) ? 1.0 : 0.0;
+    return;
+  }
+  const long j = jn < p ? jn : jn + 1;
+  const double* xc = X + j * ld;
+  for (int i = tid; i < p; i += 256) yc[i] = (j < p) ? xc[i] : 0.0;
+  if (tid == 0) yc[ld - 1] = 0.0;  // new padding row
+  const int mt = (int)(ld - 1 - p);  // trailing rows: new p .. ld-2 from old p+1 .. ld-1
+  if (mt <= 0) return;
+  const double xp = (j < p) ? X[p + j * ld] : 0.0;
+  // above old row j of an old column j > p the column is zero, and so is y
+  const int k0 = (j > p) ? (int)(j - (p + 1)) : 0;
+  for (int k = tid; k < k0; k += 256) yc[p + k] = 0.0;
+  __shared__ double wsum[4];
+  constexpr int E = 4;  // consecutive rows per thread per chunk
+  double carry = 0.0;
+  for (int c0 = k0; c0 < mt; c0 += 256 * E) {
+    double mv[E], pr[E];
+    double tsum = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = c0 + tid * E + e;
+      double val = 0.0, prod = 0.0;
+      if (k < mt) {
+        val = xc[p + 1 + k] + v[k] * xp;
+        prod = v[k] * val;
+      }
+      mv[e] = val;
+      pr[e] = prod;
+      tsum += prod;
+    }
+    // exclusive prefix of the threads' sums over the workgroup (fixed order)
+    double incl = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    double before = 0.0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    const double chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    double run = carry + before + (incl - tsum);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = c0 + tid * E + e;
+      if (k < mt) yc[p + k] = (mv[e] - v[k] * run * ibp[k]) * ig[k];
+      run += pr[e];
+    }
+    carry += chunk;
+  }
+}
+
+// log L'_ii: rows above p keep theirs; trailing rows gain log G_kk; the last
+// row is padding (log 1).
+__global__ void k_delete_logdiag(const double* __restrict__ ld_old, const double* __restrict__ lg,
+                                 double* __restrict__ out, int p, long np) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  if (i < p)
+    out[i] = ld_old[i];
+  else if (i < np - 1)
+    out[i] = ld_old[i + 1] + lg[i - p];
+  else
+    out[i] = 0.0;
+}
+
 // Re-pads the model for n_new points: every npad-sized buffer again, the
 // points and targets from the host copies, no factor.
 void regrow(ExactModel* m, long npad_new) {
@@ -838,7 +929,81 @@ int gprx_hip_exact_hessian(gprx_hip_model* model, const double* log_theta, size_
 // OnlineGpr::delete (src/gpr/online.rs:564-620): drops the point at row
 // `index` and factors the model again (O(n^3); the reference downdates an
 // LDL^T factor, which this model, holding L^-1 rather than L, does not have).
+// The FP64 model's deletion in O(n^2) (the downdate above), then alpha =
+// X^T (X y) and the NLML.  The factor keeps the jitter it was formed with,
+// as the reference's downdated LDL^T does.
+int exact_delete_downdate(ExactModel* m, int p, double* nlml, gprx_error* err) {
+  hipStream_t s = m->stream;
+  if (!m->factored || m->cur_form != GPRX_FP64) {
+    double v0 = 0.0;
+    const int rc = m->evaluate(m->theta.data(), false, &v0, nullptr, err);
+    if (rc != GPRX_OK) return rc;
+  }
+  const int n = m->n, d = m->d;
+  const long np = m->npad;
+  const int mt = (int)(np - 1 - p);
+  // column p of X from the diagonal down
+  std::vector<double> col((size_t)(np - p));
+  GPRX_HIP_CHECK(hipMemcpyAsync(col.data(), m->X64.p + p + (long)p * np, sizeof(double) * (np - p),
+                                hipMemcpyDeviceToHost, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  const double xpp = col[0];
+  std::vector<double> hv(std::max(mt, 1)), hibp(std::max(mt, 1)), hig(std::max(mt, 1)),
+      hlg(std::max(mt, 1));
+  double beta_prev = 1.0;
+  for (int k = 0; k < mt; ++k) {
+    const double vk = -col[(size_t)k + 1] / xpp;
+    const double t = vk * vk / beta_prev;
+    const double beta = beta_prev + vk * vk;
+    hv[k] = vk;
+    hibp[k] = 1.0 / beta_prev;
+    hig[k] = 1.0 / std::sqrt(beta / beta_prev);
+    hlg[k] = 0.5 * std::log1p(t);
+    beta_prev = beta;
+  }
+  if (mt > 0) {
+    GPRX_HIP_CHECK(hipMemcpyAsync(m->r.p, hv.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(m->dlt.p, hibp.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(m->p1.p, hig.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
+    GPRX_HIP_CHECK(hipMemcpyAsync(m->q1.p, hlg.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
+  }
+  // the downdated X into A64 (free between evaluations), then the buffers swap
+  hipLaunchKernelGGL(k_delete_rows, dim3((unsigned)np), dim3(256), 0, s, m->X64.p, m->A64.p, np, p,
+                     m->r.p, m->dlt.p, m->p1.p);
+  GPRX_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_delete_logdiag, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                     m->logdiag.p, m->q1.p, m->tv.p, p, np);
+  GPRX_HIP_CHECK(hipGetLastError());
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->logdiag.p, m->tv.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+  std::swap(m->X64.p, m->A64.p);
+  std::swap(m->X64.n, m->A64.n);
+  // the points and targets without row p (rows keep their order)
+  std::vector<double> hx_new((size_t)(n - 1) * d);
+  for (int q = 0; q < d; ++q)
+    for (int i = 0, k = 0; i < n; ++i)
+      if (i != p) hx_new[(size_t)q * (n - 1) + k++] = m->hx[(size_t)q * n + i];
+  m->hx = std::move(hx_new);
+  m->hy.erase(m->hy.begin() + p);
+  m->n = n - 1;
+  upload_points(m->x, m->hx.data(), m->n, d, np, m->dp, s);
+  std::vector<double> yh((size_t)np, 0.0);
+  std::memcpy(yh.data(), m->hy.data(), sizeof(double) * m->n);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+  // alpha = X^T (X y) and the NLML's sums
+  tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvRow, m->y.p, m->tv.p, nullptr, m->tpart.p, s);
+  tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvCol, m->tv.p, m->alpha.p, nullptr, m->tpart.p, s);
+  dot_and_sum(m->y.p, m->alpha.p, m->logdiag.p, m->n, m->scal.p, s);
+  GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+  m->z_valid = false;
+  const double v = 0.5 * m->hscal.p[0] + m->hscal.p[1] + 0.5 * (double)m->n * kLog2Pi;
+  if (nlml) *nlml = v;
+  return GPRX_OK;
+}
+
 int exact_delete(ExactModel* m, size_t index, double* nlml, gprx_error* err) {
+  if (m->precision == GPRX_FP64) return exact_delete_downdate(m, (int)index, nlml, err);
+  // the mixed form factors the remaining points again
   const int n = m->n, d = m->d;
   std::vector<double> hx_new((size_t)(n - 1) * d);
   for (int q = 0; q < d; ++q)

@@ -4,6 +4,8 @@ OnlineGpr::insert in src/gpr/online.rs).  The reference's libgp goldens
 check the NLML, predictive mean and observation variance after each step,
 1e-8), and insertions across the 128-row padding against a model factored
 from scratch at every step."""
+import math
+
 import numpy as np
 import pytest
 
@@ -125,3 +127,50 @@ def test_delete_refuses_the_last_point_and_bad_indices(gpu):
     assert m.n == 1
     with pytest.raises(g.InsufficientData):
         m.delete(0)
+
+
+def test_mixed_deletion_factors_the_remaining_points_again(gpu):
+    """A mixed model's deletion forms its factor again from the remaining
+    points: the same results as a fresh mixed model of them, bit for bit."""
+    rng = np.random.default_rng(12)
+    n, d = 300, 3
+    x = rng.uniform(0, 1, n * d)
+    y = np.sin(4 * x[:n]) + 0.1 * rng.standard_normal(n)
+    spec = g.ConstantKernel(1.2) * g.RbfArdKernel([0.4, 0.5, 0.7])
+    lik = g.GaussianLikelihood(0.05)
+    m = g.Gpr(spec, lik).with_precision(g.Precision.MIXED).factor(x, n, d, y)
+    keep = list(range(n))
+    print("[online] mixed deletions", flush=True)
+    for idx in (0, 150, 297):
+        nlml = m.delete(idx)
+        keep.pop(idx)
+        xk = np.concatenate([x[q * n + np.array(keep)] for q in range(d)])
+        ref = g.Gpr(spec, lik).with_precision(g.Precision.MIXED).factor(xk, len(keep), d, y[keep])
+        assert nlml == ref.neg_log_marginal_likelihood(), idx
+        ref.close()
+
+
+def test_deletion_keeps_the_factors_jitter(gpu):
+    """The downdate keeps the jitter the factor was formed with (the
+    reference's downdated LDL^T keeps its own): after deleting one of two
+    coincident points the NLML is the oracle's at noise sigma^2 + j."""
+    import oracle
+
+    rng = np.random.default_rng(13)
+    n, d = 60, 2
+    pts = rng.uniform(0, 1, (n, d))
+    pts[1] = pts[0]
+    x = pts.T.copy().ravel()
+    y = np.cos(3 * pts[:, 0]) + 0.1 * rng.standard_normal(n)
+    spec = g.RbfKernel(0.3)
+    noise = 1e-18
+    pol = g.JitterPolicy.adaptive(1e-3, 10.0, 3, 1e-1)
+    m = g.Gpr(spec, g.GaussianLikelihood(noise)).with_jitter_policy(pol).factor(x, n, d, y)
+    j = m.last_jitter()
+    assert j > 0.0
+    nlml = m.delete(1)
+    keep = [i for i in range(n) if i != 1]
+    xk = pts[keep].T.copy().ravel()
+    th = np.array([math.log(0.3), math.log(noise + j)])
+    ov, _, _, _ = oracle.exact_value_grad(spec.program(), th, xk, n - 1, d, y[keep], want_grad=False)
+    assert rel_err(nlml, ov) < 1e-10

@@ -6,16 +6,20 @@
 // j tiles runs in the same three launches,
 //   1. column update A_ij -= L_i,0:j L_j,0:j^T, i >= j (one tile task each,
 //      reduction length 128 j)
-//   2. diagonal block j: Cholesky + inverse in LDS (storage arithmetic) -> X_jj
+//   2. diagonal block j: Cholesky + inverse in LDS (in f64 for both forms) -> X_jj
 //   3. panel L_ij = A_ij X_jj^T (i > j) and Y_jk = X_jj L_jk (k < j)
 // then the inverse by distance, X_ij = -sum_k Y_ik X_kj (reduction 128 (i-j)),
 // A^-1 = X^T X, alpha and the fused gradient contraction, each as one
 // grouped launch.  Task lists are built once; each launch runs a slot
 // schedule over the 8 XCDs picked by timing candidates on the batch itself.
-// The FP64 evaluation is captured in one hipGraph and replayed per optimizer
-// iteration with only the per-problem parameters re-uploaded; the mixed form
-// refines alpha with f64 residuals, tile matrix-vector products restricted
-// to the problems still refining.
+// Each evaluation is captured in hipGraphs and replayed per optimizer
+// iteration with only the per-problem parameters re-uploaded.  The mixed
+// form is two graphs: the first factors, forms alpha and refines it twice
+// with f64 residuals on every problem, recording each residual pass; one
+// readback applies the refinement's stop rules to them, the few problems
+// still refining go on uncaptured over their own tiles, and the second
+// graph forms A^-1 and the gradient.  A problem whose f32 factor failed or
+// whose refinement did not converge is evaluated alone in FP64.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -49,7 +53,7 @@ struct TileTask {
   int K;
   int tb;     // B stored N x K (NT) instead of K x N (NN)
   int ta;     // A stored K x M (TN)
-  int lower;  // diagonal tile: keep r >= c
+  int lower;  // bit 0: diagonal tile, keep r >= c; bit 1: run the K slices from the end
   int arenas;
   float alpha, beta;
 };
@@ -70,10 +74,10 @@ __device__ __forceinline__ void run_task(const TileTask& t, int quad, T* X0, T* 
   for (int i = 0; i < FB; ++i)
 #pragma unroll
     for (int j = 0; j < FB; ++j) acc[i][j] = AV{};
-  tile_mainloop<T, TA, TB, TT>(Ab, t.ld, Bb, t.ld, 0, t.K / Cfg<T, TT>::BK, sm, acc);
+  tile_mainloop<T, TA, TB, TT>(Ab, t.ld, Bb, t.ld, 0, t.K / Cfg<T, TT>::BK, sm, acc, (t.lower & 2) != 0);
   const int lane = threadIdx.x & 63, wave = wave_index();
   const int wm = wave & 1, wn = wave >> 1;
-  const bool diag = t.lower && qr == qc;
+  const bool diag = (t.lower & 1) && qr == qc;
   // old values of a row of FB blocks before their stores (as k_gemm's epilogue)
 #pragma unroll
   for (int i = 0; i < FB; ++i) {
@@ -371,6 +375,19 @@ __global__ __launch_bounds__(256) void k_problem_residual(const ProbDesc* probs,
   }
 }
 
+// Residual pass `it` of the captured mixed evaluation: hist[p][it] holds
+// ||r||_inf and ||A||_inf ||alpha||_inf + ||y||_inf (the denominator of the
+// stop test of src/precision/refine.rs:56-98) from the scalars
+// k_problem_residual wrote.
+__global__ void k_store_residual(const double* __restrict__ sc, double* __restrict__ hist, int K,
+                                 int it, int passes) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= K) return;
+  const double* q = sc + 8 * p;
+  hist[(long)p * 2 * passes + 2 * it] = q[4];
+  hist[(long)p * 2 * passes + 2 * it + 1] = q[7] * q[5] + q[6];
+}
+
 // alpha += d on the problems whose flag is set.
 __global__ void k_masked_axpy(const ProbDesc* probs, const int* active, double* alpha,
                               const double* d) {
@@ -416,6 +433,8 @@ struct BatchModel : gprx_hip_model {
     for (gprx_hip_model* m : fb) gprx_hip_destroy(m);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (graph_b_exec) (void)hipGraphExecDestroy(graph_b_exec);
+    if (graph_b) (void)hipGraphDestroy(graph_b);
     if (stream) {
       (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
@@ -445,6 +464,17 @@ struct BatchModel : gprx_hip_model {
   DevBuf<int> info, active;
   HostBuf<int> hinfo, hactive;
   HostBuf<double> hscal, hacc;
+  // The captured mixed evaluation refines alpha kMixedCorrections times on
+  // every problem; each of its residual passes is recorded in rhist.
+  static constexpr int kMixedCorrections = 2;
+  DevBuf<double> rhist;
+  HostBuf<double> hrhist;
+  hipGraph_t graph_b = nullptr;  // mixed: X^T X, scalars, contraction
+  hipGraphExec_t graph_b_exec = nullptr;
+  DevBuf<FillTask> act_tasks;    // lower tiles of the problems still refining
+  HostBuf<FillTask> hact;
+  std::vector<FillTask> hfill;   // host copy; problem p owns [hbeg[p], hend[p])
+  std::vector<int> hbeg, hend;
   // Task lists (host copy kept for the schedules).
   struct Range {
     int off = 0, count = 0;
@@ -471,15 +501,11 @@ struct BatchModel : gprx_hip_model {
   std::vector<Range> step_diag;
   DevBuf<FillTask> fill_tasks;
   int n_fill = 0;
-  std::vector<FillTask> hfill;  // host copy; problem p owns [hbeg[p], hend[p])
-  std::vector<int> hbeg, hend;
   DevBuf<int> d_fbeg, d_fend;
   struct DimGroup {
     int dp, off, count;
   };
   std::vector<DimGroup> dp_groups;
-  DevBuf<FillTask> act_tasks;   // lower tiles of the problems still refining
-  HostBuf<FillTask> hact;
   DevBuf<double> mvpart;        // tile matrix-vector partials, 4 * 128 per lower tile
   DevBuf<int> d_blkprob;        // problem of each 128-block of the vector arena
 
@@ -558,6 +584,9 @@ struct BatchModel : gprx_hip_model {
       close_range(inv[d]);
     }
     // A^-1 (lower) = X^T X: C_ij = sum_{k >= i} X_ki^T X_kj, into arena 0.
+    // A problem's tiles share the end of their K-range (its last row), so
+    // they run from it down: tiles started together read the same rows of X
+    // at the same time (as the single-problem X^T X does).
     open_range(lauum);
     for (int p = 0; p < K; ++p) {
       const HostProb& h = hp[p];
@@ -567,7 +596,7 @@ struct BatchModel : gprx_hip_model {
           const long a = h.moff + (long)i * kTile + (long)i * kTile * L;
           const long b = h.moff + (long)i * kTile + (long)j * kTile * L;
           const long c = h.moff + (long)i * kTile + (long)j * kTile * L;
-          add(p, TileTask{a, b, c, (int)L, (h.nt - i) * kTile, 0, 1, i == j, 3, 1.f, 0.f});
+          add(p, TileTask{a, b, c, (int)L, (h.nt - i) * kTile, 0, 1, (i == j ? 1 : 0) | 2, 3, 1.f, 0.f});
         }
     }
     close_range(lauum);
@@ -597,12 +626,12 @@ struct BatchModel : gprx_hip_model {
       dp_groups.back().count = (int)ft.size() - dp_groups.back().off;
     }
     n_fill = (int)ft.size();
-    hfill = ft;
-    hbeg = fbeg;
-    hend = fend;
     upload(fill_tasks, ft, stream);
     upload(d_fbeg, fbeg, stream);
     upload(d_fend, fend, stream);
+    hfill = ft;
+    hbeg = fbeg;
+    hend = fend;
   }
 
   // Slots of one launch under a schedule policy.  Items are tasks (tile
@@ -626,7 +655,7 @@ struct BatchModel : gprx_hip_model {
         items.push_back(Item{t << 2, task_prob[t], w});
       } else {
         for (int q = 0; q < 4; ++q) {
-          if (tk.lower && q == 2) continue;  // rows 0-63, columns 64-127: above the diagonal
+          if ((tk.lower & 1) && q == 2) continue;  // rows 0-63, columns 64-127: above the diagonal
           items.push_back(Item{t << 2 | q, task_prob[t], w});
         }
       }
@@ -784,6 +813,8 @@ struct BatchModel : gprx_hip_model {
     return rc;
   }
 
+  // Uploads the lower tiles of the problems with act[p] set; returns their
+  // count (the restricted matrix-vector products of the refinement).
   int set_active(const std::vector<int>& act) {
     int c = 0;
     hact.alloc(std::max(n_fill, 1));
@@ -928,6 +959,47 @@ struct BatchModel : gprx_hip_model {
                        alpha.p, logd.p, A64.p, scal.p);
     contract<double>(A64.p);
   }
+
+  // The device parts of one mixed evaluation (capturable).  Part A: f32
+  // factor and inverse, alpha through the f32 inverse, then
+  // kMixedCorrections corrections with f64 residuals on every problem, the
+  // residual of each pass recorded (kMixedCorrections + 1 passes; r holds
+  // the last).  Part B, after the refinement has finished: X^T X, the
+  // scalars and the contraction.  `active` holds 1 for every problem.
+  void record_mixed_a() {
+    fill<float>(A32.p);
+    factor<float>(A32.p, P32.p);
+    tile_mv<float>(P32.p, yar.p, tv.p, kTmvRow);
+    tile_mv<float>(P32.p, tv.p, alpha.p, kTmvCol);
+    const int passes = kMixedCorrections + 1;
+    for (int it = 0; it < passes; ++it) {
+      // p1 = A alpha over the lower triangle; on the first pass also the
+      // absolute row sums (max: ||A||_inf)
+      tile_mv<double>(K64.p, alpha.p, p1.p, it == 0 ? kTmvSymAbs : kTmvSym, q1.p);
+      hipLaunchKernelGGL(k_problem_residual, dim3(K), dim3(256), 0, stream, probs.p, yar.p, p1.p,
+                         nullptr, alpha.p, it == 0 ? q1.p : nullptr, nullptr, r.p, scal.p);
+      hipLaunchKernelGGL(k_store_residual, dim3((K + 255) / 256), dim3(256), 0, stream, scal.p,
+                         rhist.p, K, it, passes);
+      if (it == kMixedCorrections) break;
+      correct_alpha(-1);
+    }
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
+  void record_mixed_b() {
+    launch_tasks<float>(lauum_s, A32.p, P32.p);
+    hipLaunchKernelGGL((k_problem_scalars<float>), dim3(K), dim3(256), 0, stream, probs.p, yar.p,
+                       alpha.p, logd.p, A32.p, scal.p);
+    contract<float>(A32.p);
+  }
+  // alpha += X^T (X r) on the problems flagged in `active` (n_act tiles in
+  // act_tasks, or every problem when n_act < 0)
+  void correct_alpha(int n_act) {
+    tile_mv<float>(P32.p, r.p, tv.p, kTmvRow, nullptr, n_act);
+    tile_mv<float>(P32.p, tv.p, dlt.p, kTmvCol, nullptr, n_act);
+    hipLaunchKernelGGL(k_masked_axpy, dim3((65536 + 255) / 256, K), dim3(256), 0, stream, probs.p,
+                       active.p, alpha.p, dlt.p);
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
 };
 
 }  // namespace
@@ -1045,6 +1117,8 @@ int gprx_hip_batch_create(const gprx_problem* problems, size_t k, int32_t precis
     B.active.alloc(k);
     B.hinfo.alloc(k);
     B.hactive.alloc(k);
+    B.rhist.alloc((size_t)k * 2 * (BatchModel::kMixedCorrections + 1));
+    B.hrhist.alloc((size_t)k * 2 * (BatchModel::kMixedCorrections + 1));
     B.scal.alloc(8 * k);
     B.hscal.alloc(8 * k);
     B.accs.alloc(k * kMaxAcc);
@@ -1093,78 +1167,97 @@ int gprx_hip_batch_value_and_grad(gprx_hip_model* model, const double* const* lo
     B->tune();
     GPRX_HIP_CHECK(hipMemsetAsync(B->info.p, 0, sizeof(int) * B->K, s));
     GPRX_HIP_CHECK(hipEventRecord(B->ev[0], s));
-    if (B->precision == GPRX_FP64) {
-      if (!B->graph_exec) {
-        // Capture one evaluation; replays reuse it with new parameters.
-        GPRX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const bool fp64 = B->precision == GPRX_FP64;
+    if (!fp64) {  // graph A's corrections apply to every problem
+      for (int p = 0; p < B->K; ++p) B->hactive.p[p] = 1;
+      GPRX_HIP_CHECK(hipMemcpyAsync(B->active.p, B->hactive.p, sizeof(int) * B->K, hipMemcpyHostToDevice, s));
+    }
+    if (!B->graph_exec) {
+      // Capture one evaluation (the mixed form: its two parts); replays
+      // reuse it with new parameters.
+      GPRX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      if (fp64)
         B->record_fp64();
-        GPRX_HIP_CHECK(hipStreamEndCapture(s, &B->graph));
-        GPRX_HIP_CHECK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+      else
+        B->record_mixed_a();
+      GPRX_HIP_CHECK(hipStreamEndCapture(s, &B->graph));
+      GPRX_HIP_CHECK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+      if (!fp64) {
+        GPRX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        B->record_mixed_b();
+        GPRX_HIP_CHECK(hipStreamEndCapture(s, &B->graph_b));
+        GPRX_HIP_CHECK(hipGraphInstantiate(&B->graph_b_exec, B->graph_b, nullptr, nullptr, 0));
       }
-      GPRX_HIP_CHECK(hipGraphLaunch(B->graph_exec, s));
-    } else {
-      // Mixed: f32 arenas, f64 diagonal blocks, alpha refined with f64 residuals.
-      B->fill<float>(B->A32.p);
-      B->factor<float>(B->A32.p, B->P32.p);
-      B->tile_mv<float>(B->P32.p, B->yar.p, B->tv.p, kTmvRow);
-      B->tile_mv<float>(B->P32.p, B->tv.p, B->alpha.p, kTmvCol);
-      std::vector<int> act(B->K, 1), streak(B->K, 0), conv(B->K, 0);
+    }
+    GPRX_HIP_CHECK(hipGraphLaunch(B->graph_exec, s));
+    if (!fp64) {
+      // One readback of the recorded residual passes; the stop rules of
+      // src/precision/refine.rs:56-98 (converged: ||r|| < 10 n eps_64 (||A||
+      // ||alpha|| + ||y||); stalled: two ratios above 0.9; at most 10
+      // residual passes) applied to them, and the problems still refining
+      // go on uncaptured, restricted to their own tiles.
+      const int passes = BatchModel::kMixedCorrections + 1;
+      GPRX_HIP_CHECK(hipMemcpyAsync(B->hrhist.p, B->rhist.p, sizeof(double) * 2 * passes * B->K,
+                                    hipMemcpyDeviceToHost, s));
+      GPRX_HIP_CHECK(hipMemcpyAsync(B->hinfo.p, B->info.p, sizeof(int) * B->K, hipMemcpyDeviceToHost, s));
+      GPRX_HIP_CHECK(hipStreamSynchronize(s));
+      std::vector<int> act(B->K, 0), conv(B->K, 0), streak(B->K, 0);
       std::vector<double> prev(B->K, -1.0);
-      int n_act = -1;  // every problem
-      for (int it = 0; it < 10; ++it) {
-        // p1 = A alpha over the lower triangle; on the first pass also the
-        // absolute row sums (max: ||A||_inf)
-        B->tile_mv<double>(B->K64.p, B->alpha.p, B->p1.p, it == 0 ? kTmvSymAbs : kTmvSym, B->q1.p, n_act);
-        hipLaunchKernelGGL(k_problem_residual, dim3(B->K), dim3(256), 0, s, B->probs.p, B->yar.p,
-                           B->p1.p, nullptr, B->alpha.p, it == 0 ? B->q1.p : nullptr, nullptr,
-                           B->r.p, B->scal.p);
-        GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
-        if (it == 0)
-          GPRX_HIP_CHECK(hipMemcpyAsync(B->hinfo.p, B->info.p, sizeof(int) * B->K, hipMemcpyDeviceToHost, s));
-        GPRX_HIP_CHECK(hipStreamSynchronize(s));
-        bool any = false;
-        for (int p = 0; p < B->K; ++p) {
-          if (it == 0 && B->hinfo.p[p]) act[p] = 0;  // f32 factor failed: FP64 below
-          if (!act[p]) continue;
-          const double* sc = B->hscal.p + 8 * p;
-          const double tol = 10.0 * B->hp[p].n * 2.220446049250313e-16;
-          const double denom = sc[7] * sc[5] + sc[6];
-          if (denom > 0.0 && sc[4] / denom < tol) {
-            act[p] = 0;
-            conv[p] = 1;
-            continue;
-          }
-          if (prev[p] >= 0.0) {
-            const double ratio = prev[p] > 0.0 ? sc[4] / prev[p] : 0.0;
-            if (ratio > 0.9) {
-              if (++streak[p] >= 2) {
-                act[p] = 0;
-                continue;
-              }
-            } else {
-              streak[p] = 0;
-            }
-          }
-          prev[p] = sc[4];
-          any = true;
+      // one residual pass of problem p: returns false once p stops refining
+      auto step = [&](int p, double rinf, double den) {
+        const double tol = 10.0 * B->hp[p].n * 2.220446049250313e-16;
+        if (den > 0.0 && rinf / den < tol) {
+          conv[p] = 1;
+          return false;
         }
-        if (!any) break;
+        if (prev[p] >= 0.0) {
+          const double ratio = prev[p] > 0.0 ? rinf / prev[p] : 0.0;
+          if (ratio > 0.9) {
+            if (++streak[p] >= 2) return false;
+          } else {
+            streak[p] = 0;
+          }
+        }
+        prev[p] = rinf;
+        return true;
+      };
+      int n_live = 0;
+      for (int p = 0; p < B->K; ++p) {
+        if (!ok[p] || B->hinfo.p[p]) continue;
+        bool live = true;
+        for (int it = 0; it < passes && live; ++it) {
+          const double* hh = B->hrhist.p + (long)p * 2 * passes + 2 * it;
+          live = step(p, hh[0], hh[1]);
+        }
+        act[p] = live ? 1 : 0;
+        n_live += act[p];
+      }
+      for (int it = passes; it < 10 && n_live > 0; ++it) {
         for (int p = 0; p < B->K; ++p) B->hactive.p[p] = act[p];
         GPRX_HIP_CHECK(hipMemcpyAsync(B->active.p, B->hactive.p, sizeof(int) * B->K, hipMemcpyHostToDevice, s));
-        n_act = B->set_active(act);
-        // correction d = X^T (X r) on the problems still refining
-        B->tile_mv<float>(B->P32.p, B->r.p, B->tv.p, kTmvRow, nullptr, n_act);
-        B->tile_mv<float>(B->P32.p, B->tv.p, B->dlt.p, kTmvCol, nullptr, n_act);
-        hipLaunchKernelGGL(k_masked_axpy, dim3((65536 + 255) / 256, B->K), dim3(256), 0, s,
-                           B->probs.p, B->active.p, B->alpha.p, B->dlt.p);
+        const int n_act = B->set_active(act);
+        B->correct_alpha(n_act);
+        B->tile_mv<double>(B->K64.p, B->alpha.p, B->p1.p, kTmvSym, B->q1.p, n_act);
+        hipLaunchKernelGGL(k_problem_residual, dim3(B->K), dim3(256), 0, s, B->probs.p, B->yar.p,
+                           B->p1.p, nullptr, B->alpha.p, nullptr, nullptr, B->r.p, B->scal.p);
+        GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
+        GPRX_HIP_CHECK(hipStreamSynchronize(s));
+        n_live = 0;
+        for (int p = 0; p < B->K; ++p) {
+          if (!act[p]) continue;
+          const double* sc = B->hscal.p + 8 * p;
+          act[p] = step(p, sc[4], sc[7] * sc[5] + sc[6]) ? 1 : 0;
+          n_live += act[p];
+        }
       }
-      for (int p = 0; p < B->K; ++p)
-        if (!conv[p] && ok[p])
-          B->fb_reason[p] = B->hinfo.p[p] ? GPRX_FALLBACK_JITTER : GPRX_FALLBACK_REFINEMENT;
-      B->launch_tasks<float>(B->lauum_s, B->A32.p, B->P32.p);
-      hipLaunchKernelGGL((k_problem_scalars<float>), dim3(B->K), dim3(256), 0, s, B->probs.p,
-                         B->yar.p, B->alpha.p, B->logd.p, B->A32.p, B->scal.p);
-      B->contract<float>(B->A32.p);
+      for (int p = 0; p < B->K; ++p) {
+        if (!ok[p]) continue;
+        if (B->hinfo.p[p])
+          B->fb_reason[p] = GPRX_FALLBACK_JITTER;
+        else if (!conv[p])
+          B->fb_reason[p] = GPRX_FALLBACK_REFINEMENT;
+      }
+      GPRX_HIP_CHECK(hipGraphLaunch(B->graph_b_exec, s));
     }
     GPRX_HIP_CHECK(hipEventRecord(B->ev[1], s));
     GPRX_HIP_CHECK(hipMemcpyAsync(B->hscal.p, B->scal.p, sizeof(double) * 8 * B->K, hipMemcpyDeviceToHost, s));
@@ -1224,6 +1317,14 @@ int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t s
     if (B->graph) {
       GPRX_HIP_CHECK(hipGraphDestroy(B->graph));
       B->graph = nullptr;
+    }
+    if (B->graph_b_exec) {
+      GPRX_HIP_CHECK(hipGraphExecDestroy(B->graph_b_exec));
+      B->graph_b_exec = nullptr;
+    }
+    if (B->graph_b) {
+      GPRX_HIP_CHECK(hipGraphDestroy(B->graph_b));
+      B->graph_b = nullptr;
     }
     if (policy < 0) {
       B->policy = -1;

@@ -174,9 +174,12 @@ int gprx_hip_exact_insert(gprx_hip_model* model, const double* x_new, size_t d, 
 int gprx_hip_exact_export_factor(gprx_hip_model* model, double* l, size_t n, double* alpha,
                                  double* jitter, gprx_error* err);
 
-/* Replaces OnlineGpr::delete (src/gpr/online.rs:564-620): drops the point
- * at row `index` (rows keep their order) and factors the model again at its
- * parameters, O(n^3), writing the new NLML (nullable).  A model of one
+/* Replaces OnlineGpr::delete (src/gpr/online.rs:564-620, the LDL^T row
+ * delete of src/linalg/ldlt.rs:113): drops the point at row `index` (rows
+ * keep their order) and writes the new NLML (nullable).  An FP64 model
+ * downdates its factor's inverse in O(n^2) on the device (a rank-one update
+ * of the trailing factor), keeping the jitter the factor was formed with; a
+ * mixed model is factored again at its parameters, O(n^3).  A model of one
  * point refuses (GPRX_INSUFFICIENT_DATA). */
 int gprx_hip_exact_delete(gprx_hip_model* model, size_t index, double* nlml, gprx_error* err);
 
@@ -311,9 +314,16 @@ int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t s
 int gprx_hip_batch_schedule(const gprx_hip_model* model, int32_t* policy,
                             int32_t* small_tile_below, double* tuned_ms);
 
-/* A mixed batch evaluates a problem whose f32 factor fails or whose
- * refinement stalls again alone in FP64 (as gprx_hip_exact_value_and_grad
- * with the default jitter policy), so its results and status are FP64's.
+/* A mixed batch evaluation replays two captured hipGraphs: the first factors
+ * and refines alpha twice with f64 residuals on every problem, recording each
+ * residual; after one readback the stop rules of
+ * src/precision/refine.rs:56-98 decide per problem, the problems still
+ * refining continue (at most 10 residual passes), and the second graph forms
+ * A^-1 and the gradient.  A problem whose f32 factor fails
+ * (GPRX_FALLBACK_JITTER) or whose refinement stalls or does not converge
+ * (GPRX_FALLBACK_REFINEMENT) is evaluated again alone in FP64 (as
+ * gprx_hip_exact_value_and_grad with the default jitter policy), so its
+ * results and status are FP64's.
  * reasons[k] receives each problem's gprx_fallback for the last call.  No
  * counterpart in the reference (it evaluates problems one by one). */
 int gprx_hip_batch_last_fallback(const gprx_hip_model* model, int32_t* reasons, size_t k);

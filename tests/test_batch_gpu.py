@@ -107,3 +107,39 @@ def test_mixed_batch_falls_back_per_problem(gpu):
     # a second call reuses the kept FP64 model
     nlml2, grads2, _ = b.value_and_gradient(thetas)
     assert nlml2[1] == v and np.array_equal(grads2[1], gr)
+
+
+def test_mixed_batch_refinement_fallback_per_problem(gpu):
+    """The mixed batch refines alpha in its captured graph and continues the
+    problems still refining under the reference's stop rules: over a sweep
+    of ever smaller noise variances on one smooth RBF Gram (as in the
+    single-path stall test), the ill-conditioned end stalls or fails its f32
+    factor and is evaluated alone in FP64, equal to the single FP64 model bit
+    for bit, while the well-conditioned start stays mixed; graph replays
+    give the same results."""
+    n, d = 1024, 4
+    rng = np.random.default_rng(31)
+    x = rng.random(n * d)
+    y = np.sin(3 * x[:n]) + np.cos(2 * x[n:2 * n]) + 0.1 * rng.standard_normal(n)
+    noises = [10.0 ** e for e in np.arange(-2.0, -5.51, -0.25)]
+    probs = [(g.RbfKernel(0.5), g.GaussianLikelihood(s2), x, n, d, y) for s2 in noises]
+    print(f"[batch mixed refinement fallback] {len(probs)} noises", flush=True)
+    b = g.GprBatch(probs, precision=g.Precision.MIXED)
+    thetas = [np.concatenate([p[0].get_params(), [p[1].log_param()]]) for p in probs]
+    ref = []
+    for (sp, lik, xi, ni, di, yi), th in zip(probs, thetas):
+        m = g.Gpr(sp, lik).prepare(xi, ni, di, yi)
+        gr = np.empty(th.size)
+        ref.append((m.value_and_gradient_into(th, gr), gr))
+        m.close()
+    for rep in range(2):
+        nlml, grads, errs = b.value_and_gradient(thetas)
+        fb = list(b.last_fallback())
+        print(f"  rep {rep} fallbacks {fb}", flush=True)
+        assert all(e is None for e in errs), errs
+        assert fb[0] == g.Fallback.NONE
+        assert g.Fallback.REFINEMENT in fb
+        for i, f in enumerate(fb):
+            if f != g.Fallback.NONE:
+                assert nlml[i] == ref[i][0] and np.array_equal(grads[i], ref[i][1]), i
+        assert_close(nlml[0], ref[0][0], 1e-4, f"nlml 0 rep {rep}")

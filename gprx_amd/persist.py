@@ -341,12 +341,30 @@ def _json_depth(obj) -> int:
     return deepest
 
 
+# The loader of each model kind (src/persist/config.rs:311-336).
+_LOADERS = {"exact": "LoadedGpr::load", "sgpr": "LoadedSgpr::load",
+            "online_sgpr": "LoadedSgpr::load", "svgp": "LoadedSvgp::load"}
+
+
+def _expect_model(cfg: dict, expected) -> str:
+    """The config's model kind (absent means exact, src/persist/config.rs:311-317),
+    or PersistFailed kind "wrong model" naming the loader it needs."""
+    kind = cfg.get("model", "exact")
+    if kind not in _LOADERS:
+        raise PersistFailed("config", f"model {kind!r} is not exact, sgpr, online_sgpr or svgp")
+    if kind not in expected:
+        raise PersistFailed("wrong model",
+                            f"config.json holds a {kind} model; load it with {_LOADERS[kind]}")
+    return kind
+
+
 def _check_config(cfg: dict):
     """Validates what this backend can hold; returns (n, d, has_factor)."""
     if not isinstance(cfg, dict):
         raise PersistFailed("config", "config.json is not a JSON object")
     if cfg.get("format_version") != FORMAT_VERSION:
         raise UnsupportedPersistVersion(cfg.get("format_version"))
+    _expect_model(cfg, ("exact",))
     try:
         n, d, has_factor = int(cfg["n"]), int(cfg["d"]), bool(cfg["has_factor"])
         kind = cfg["factor_kind"]
@@ -444,3 +462,141 @@ def load(directory):
     saved parameters with the saved jitter policy and precision."""
     gpr, x, y, n, d = read_model(directory)
     return gpr.factor(x, n, d, y)
+
+
+# ---------------------------------------------------------------- sparse models
+#
+# Sgpr and Svgp directories (src/persist/sparse.rs): config.json is the
+# SparseConfig of src/persist/config.rs:534-565 (the exact model's fields with
+# `model` and `m`, without the factor); model.safetensors holds x [n, d],
+# y [n], z [m, d] and z_train [m, d] (the inducing points in training
+# coordinates: z itself under identity transforms), and for Svgp the
+# whitened q(u) as q_mean [m] and q_l [m, m] (column-major, zeros above the
+# diagonal).  No factor is stored: a load factors again at the saved
+# parameters and inducing points.  K_mm is factored with the sparse default
+# jitter policy adaptive(1e-8, 10, 5, 1e-3) (src/policy.rs:220-229), the one
+# this backend's sparse models use, so that policy is written and the only
+# one a load accepts.
+
+def _kmm_jitter():
+    from .gpr import JitterPolicy
+    return JitterPolicy.adaptive(1e-8, 10.0, 5, 1e-3)
+
+
+def sparse_config(kind: str, kernel, theta, n: int, m: int, d: int, mixed: bool) -> dict:
+    """SparseConfig of a sparse model at log-parameters theta ([kernel | log
+    noise]), fields in the struct's order, serde defaults left out."""
+    kernel = copy.deepcopy(kernel)
+    kernel.set_params(theta[:-1])
+    it = iter([DEFAULT_BOUNDS] * len(theta))
+    cfg = {"format_version": FORMAT_VERSION, "model": kind, "n": n, "m": m, "d": d}
+    if mixed:
+        cfg["precision"] = "mixed"
+    cfg["kernel"] = encode_kernel(kernel, it)
+    cfg["likelihood"] = {"noise_variance": _bounded(math.exp(theta[-1]), *next(it))}
+    cfg["jitter"] = encode_jitter(_kmm_jitter())
+    cfg["x_unfitted"] = "identity"
+    cfg["y_unfitted"] = "identity"
+    cfg["x_transform"] = "identity"
+    cfg["y_transform"] = "identity"
+    return cfg
+
+
+def save_sparse(kind: str, kernel, theta, x, y, z, n: int, m: int, d: int, mixed: bool, directory,
+                q=None) -> None:
+    """write_sparse (src/persist/sparse.rs:95-163): the tensors, then
+    config.json, each written atomically.  q = (q_mean [m], q_l packed by
+    columns, rows i >= j) for Svgp."""
+    dirp = Path(directory)
+    try:
+        dirp.mkdir(parents=True, exist_ok=True)
+    except OSError as e:
+        raise PersistFailed("io", f"create {dirp}: {e}") from None
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+    tensors = {"x": (x, (n, d)), "y": (np.ascontiguousarray(y, dtype=np.float64), (n,)),
+               "z": (z, (m, d)), "z_train": (z, (m, d))}
+    if q is not None:
+        q_mean, packed = q
+        q_l = np.zeros((m, m))  # [col, row] here, so the bytes are column-major
+        k = 0
+        for j in range(m):
+            q_l[j, j:] = packed[k:k + m - j]
+            k += m - j
+        tensors["q_mean"] = (np.ascontiguousarray(q_mean, dtype=np.float64), (m,))
+        tensors["q_l"] = (q_l.reshape(-1), (m, m))
+    cfg = sparse_config(kind, kernel, np.asarray(theta, dtype=np.float64), n, m, d, mixed)
+    _write_atomic(dirp / TENSOR_FILE, safetensors_parts(tensors))
+    _write_atomic(dirp / CONFIG_FILE, [json.dumps(cfg, indent=2).encode()])
+
+
+def read_sparse(directory, kind: str):
+    """(kernel, noise, mixed, x, y, z, n, m, d, q) of a sparse directory of
+    model `kind`, every check done on the host; q = (q_mean, packed q_l) for
+    Svgp, None for Sgpr."""
+    dirp = Path(directory)
+    try:
+        raw = (dirp / CONFIG_FILE).read_bytes()
+    except OSError as e:
+        raise PersistFailed("io", f"read {dirp / CONFIG_FILE}: {e}") from None
+    try:
+        cfg = json.loads(raw)
+    except (UnicodeDecodeError, json.JSONDecodeError, RecursionError) as e:
+        raise PersistFailed("config", f"config.json is not JSON: {e}") from None
+    if not isinstance(cfg, dict):
+        raise PersistFailed("config", "config.json is not a JSON object")
+    if _json_depth(cfg) > MAX_JSON_DEPTH:
+        raise PersistFailed("config", f"config.json nests past the recursion limit of "
+                                      f"{MAX_JSON_DEPTH}")
+    if cfg.get("format_version") != FORMAT_VERSION:
+        raise UnsupportedPersistVersion(cfg.get("format_version"))
+    found = _expect_model(cfg, ("sgpr", "online_sgpr") if kind == "sgpr" else ("svgp",))
+    if found == "online_sgpr":
+        raise PersistFailed("config", "online SGPR models are outside this backend")
+    try:
+        n, m, d = int(cfg["n"]), int(cfg["m"]), int(cfg["d"])
+    except (KeyError, TypeError, ValueError):
+        raise PersistFailed("config", "config.json lacks n, m or d") from None
+    if n < 1 or m < 1 or d < 1:
+        raise PersistFailed("config", f"n = {n}, m = {m} and d = {d} must be positive")
+    if cfg.get("precision", "double") not in ("double", "mixed"):
+        raise PersistFailed("config", f"precision {cfg.get('precision')!r} is not double or mixed")
+    if cfg.get("residual", "promote_storage") not in ("promote_storage", "reevaluate_kernel"):
+        raise PersistFailed("config", f"residual {cfg.get('residual')!r} is not promote_storage "
+                                      "or reevaluate_kernel")
+    if cfg.get("math", "accurate") != "accurate":
+        raise PersistFailed("config", f"math {cfg.get('math')!r} is not accurate")
+    if cfg.get("distance") is not None:
+        raise PersistFailed("config", "supplied-distance kernels are outside this backend")
+    for key in ("x_unfitted", "y_unfitted", "x_transform", "y_transform"):
+        if cfg.get(key) != "identity":
+            raise PersistFailed("config", f"{key} {cfg.get(key)!r} is not identity")
+    if decode_jitter(cfg.get("jitter")) != _kmm_jitter():
+        raise PersistFailed("config", "this backend factors K_mm with the sparse default policy "
+                                      "adaptive(1e-8, 10, 5, 1e-3) only")
+    bounds: list = []
+    kernel = decode_kernel(cfg.get("kernel"), bounds)
+    lik = cfg.get("likelihood")
+    noise, nb = _decode_bounded(lik.get("noise_variance") if isinstance(lik, dict) else None,
+                                "noise_variance")
+    bounds.append(nb)
+    if any(tuple(b) != DEFAULT_BOUNDS for b in bounds):
+        raise PersistFailed("config", "sparse models here take the default (1e-5, 1e5) bounds")
+    tensors = read_safetensors(dirp / TENSOR_FILE)
+    x = _tensor(tensors, "x", (n, d)).copy()
+    y = _tensor(tensors, "y", (n,)).copy()
+    z = _tensor(tensors, "z", (m, d)).copy()
+    z_train = _tensor(tensors, "z_train", (m, d))
+    if not np.array_equal(z_train, z):
+        raise PersistFailed("config", "z_train differs from z, which identity transforms make equal")
+    q = None
+    if kind == "svgp":
+        q_mean = _tensor(tensors, "q_mean", (m,)).copy()
+        q_l = _tensor(tensors, "q_l", (m, m)).reshape(m, m)  # [col, row]
+        if np.any(np.tril(q_l, -1)) or not np.all(np.diagonal(q_l) > 0.0):  # row < col
+            raise PersistFailed("tensor", "q_l must be lower triangular with a positive diagonal, "
+                                          "and q finite")
+        packed = np.concatenate([q_l[j, j:] for j in range(m)])
+        q = (q_mean, packed)
+    mixed = cfg.get("precision", "double") == "mixed"
+    return kernel, noise, mixed, x, y, z, n, m, d, q

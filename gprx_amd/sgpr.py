@@ -41,7 +41,11 @@ class Sgpr:
                                           L.dptr(theta), theta.size, self.precision, C.byref(h),
                                           C.byref(err))
         raise_for(rc, err)
-        return FittedSgpr(h, theta, n, d, m)
+        f = FittedSgpr(h, theta, n, d, m)
+        # what a save writes back (x and z column-major, d rows of n and m)
+        f._kernel, f._precision = self.kernel, self.precision
+        f._x, f._y, f._z = x.copy(), y.copy(), z.copy()
+        return f
 
     def factor(self, x, n, d, y, z, m) -> "FittedSgpr":
         f = self.prepare(x, n, d, y, z, m)
@@ -105,6 +109,25 @@ class FittedSgpr:
                                            L.dptr(var), C.byref(err))
         raise_for(rc, err)
         return Prediction(mean, var)
+
+    def save(self, directory) -> None:
+        """save_sgpr (src/persist/sparse.rs:165-175): config.json and the x,
+        y, z tensors at the current parameters; no factor is written."""
+        from .persist import save_sparse
+        save_sparse("sgpr", self._kernel, self._theta, self._x, self._y, self._z, self.n,
+                    self.m, self.d, self._precision == L.MIXED, directory)
+
+    @staticmethod
+    def load(directory) -> "FittedSgpr":
+        """LoadedSgpr::load (src/persist/sparse.rs:547-556) for a coordinate
+        Sgpr: reads and checks the directory on the host, then factors on the
+        device at the saved parameters and inducing points."""
+        from .persist import read_sparse
+        kernel, noise, mixed, x, y, z, n, m, d, _ = read_sparse(directory, "sgpr")
+        model = Sgpr(kernel, GaussianLikelihood(noise))
+        if mixed:
+            model = model.with_precision(L.MIXED)
+        return model.factor(x, n, d, y, z, m)
 
     def last_eval_ms(self) -> float:
         return L.lib().gprx_hip_last_eval_ms(self._h)

@@ -53,7 +53,11 @@ class Svgp:
                                           L.dptr(theta), theta.size, self.precision, C.byref(h),
                                           C.byref(err))
         raise_for(rc, err)
-        return FittedSvgp(h, theta, n, d, m)
+        f = FittedSvgp(h, theta, n, d, m)
+        # what a save writes back (x and z column-major, d rows of n and m)
+        f._kernel, f._precision = self.kernel, self.precision
+        f._x, f._y, f._z = x.copy(), y.copy(), z.copy()
+        return f
 
 
 class FittedSvgp:
@@ -112,6 +116,33 @@ class FittedSvgp:
 
     def predict(self, xs, q, d, params=None) -> Prediction:
         return self.predict_with(xs, q, d, params)
+
+    def save(self, directory, params=None) -> None:
+        """save_svgp (src/persist/sparse.rs:190-195): config.json and the x,
+        y, z, q_mean and q_l tensors of `params` (the model's own by default);
+        no factor is written."""
+        from .persist import save_sparse
+        p = L.f64(self.params if params is None else params)
+        t, m = self.theta.size, self.m
+        if p.size != t + m + m * (m + 1) // 2:
+            raise_for(L.LENGTH_MISMATCH, f"params has {p.size} entries, the model takes "
+                                         f"{t + m + m * (m + 1) // 2}")
+        save_sparse("svgp", self._kernel, p[:t], self._x, self._y, self._z, self.n, m, self.d,
+                    self._precision == L.MIXED, directory, q=(p[t:t + m], p[t + m:]))
+
+    @staticmethod
+    def load(directory) -> "FittedSvgp":
+        """LoadedSvgp::load (src/persist/sparse.rs:728-737) for a coordinate
+        Svgp: reads and checks the directory on the host, then prepares the
+        device model with the saved parameters and q(u) as its own."""
+        from .persist import read_sparse
+        kernel, noise, mixed, x, y, z, n, m, d, q = read_sparse(directory, "svgp")
+        model = Svgp(kernel, GaussianLikelihood(noise))
+        if mixed:
+            model = model.with_precision(L.MIXED)
+        f = model.prepare(x, n, d, y, z, m)
+        f.params = np.concatenate([f.theta, q[0], q[1]])
+        return f
 
     def last_eval_ms(self) -> float:
         return L.lib().gprx_hip_last_eval_ms(self._h)

@@ -221,3 +221,179 @@ def test_missing_directory_is_an_io_failure(tmp_path):
     with pytest.raises(P.PersistFailed) as e:
         P.read_model(tmp_path / "does-not-exist")
     assert e.value.kind == "io"
+
+
+# ------------------------------------------------------------------ sparse models
+
+def _sparse_dir(tmp_path, kind="sgpr", mixed=False, seed=1):
+    """A sparse directory written by save_sparse on the host, with what it holds."""
+    r = np.random.default_rng(seed)
+    n, m, d = 9, 4, 2
+    x, y, z = r.normal(size=n * d), r.normal(size=n), r.normal(size=m * d)
+    spec = g.ConstantKernel(1.3) * g.RbfArdKernel([0.7, 1.9])
+    theta = np.concatenate([spec.get_params(), [np.log(0.05)]])
+    q = None
+    if kind == "svgp":
+        q_mean = r.normal(size=m)
+        packed = r.normal(size=m * (m + 1) // 2)
+        k = 0
+        for j in range(m):  # positive diagonal
+            packed[k] = abs(packed[k]) + 0.1
+            k += m - j
+        q = (q_mean, packed)
+    P.save_sparse(kind, spec, theta, x, y, z, n, m, d, mixed, tmp_path, q=q)
+    return dict(n=n, m=m, d=d, x=x, y=y, z=z, theta=theta, q=q, spec=spec)
+
+
+def _edit_config(path, **changes):
+    cfg = json.loads((path / "config.json").read_text())
+    for k, v in changes.items():
+        if v is None:
+            cfg.pop(k, None)
+        else:
+            cfg[k] = v
+    (path / "config.json").write_text(json.dumps(cfg))
+
+
+@pytest.mark.parametrize("kind", ["sgpr", "svgp"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sparse_config_fields_in_the_structs_order(tmp_path, kind, mixed):
+    """SparseConfig (src/persist/config.rs:534-565): fields in declaration
+    order, precision left out at double, the sparse default K_mm policy."""
+    _sparse_dir(tmp_path, kind, mixed)
+    cfg = json.loads((tmp_path / "config.json").read_text())
+    keys = ["format_version", "model", "n", "m", "d"] + (["precision"] if mixed else []) + \
+        ["kernel", "likelihood", "jitter", "x_unfitted", "y_unfitted", "x_transform",
+         "y_transform"]
+    assert list(cfg) == keys
+    assert cfg["model"] == kind and (cfg["n"], cfg["m"], cfg["d"]) == (9, 4, 2)
+    assert cfg["jitter"] == {"adaptive": {"initial": 1e-8, "multiplier": 10.0,
+                                          "max_retries": 5, "max_jitter": 1e-3}}
+    nv = cfg["likelihood"]["noise_variance"]
+    assert (nv["lo"], nv["hi"]) == (1e-5, 1e5) and nv["value"] == pytest.approx(0.05, rel=1e-15)
+    assert cfg["kernel"]["product"]["left"]["constant"]["constant"]["value"] == \
+        pytest.approx(1.3, rel=1e-15)
+    assert "has_factor" not in cfg and "distance_cache" not in cfg
+
+
+@pytest.mark.parametrize("kind", ["sgpr", "svgp"])
+def test_sparse_tensors_and_read_back(tmp_path, kind):
+    """x [n, d], y [n], z [m, d], z_train = z (identity transforms), and for
+    Svgp q_mean [m] and q_l [m, m] column-major with zeros above the
+    diagonal (src/persist/sparse.rs:139-154)."""
+    w = _sparse_dir(tmp_path, kind)
+    n, m, d = w["n"], w["m"], w["d"]
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    want = {"x", "y", "z", "z_train"} | ({"q_mean", "q_l"} if kind == "svgp" else set())
+    assert set(t) == want
+    assert t["x"].shape == (n, d) and t["z"].shape == (m, d)
+    np.testing.assert_array_equal(t["x"].reshape(-1), w["x"])
+    np.testing.assert_array_equal(t["z"].reshape(-1), w["z"])
+    np.testing.assert_array_equal(t["z_train"], t["z"])
+    kernel, noise, mixed, x, y, z, n2, m2, d2, q = P.read_sparse(tmp_path, kind)
+    assert (n2, m2, d2, mixed) == (n, m, d, False)
+    assert noise == pytest.approx(0.05, rel=1e-15)
+    np.testing.assert_allclose(kernel.get_params(), w["spec"].get_params(), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(x, w["x"])
+    np.testing.assert_array_equal(y, w["y"])
+    np.testing.assert_array_equal(z, w["z"])
+    if kind == "svgp":
+        # the bytes are column-major: element (row, col) at col * m + row
+        ql = t["q_l"].reshape(-1)
+        lower = np.zeros((m, m))
+        k = 0
+        for j in range(m):
+            lower[j:, j] = w["q"][1][k:k + m - j]
+            k += m - j
+        np.testing.assert_array_equal(ql, lower.T.reshape(-1))
+        np.testing.assert_array_equal(q[0], w["q"][0])
+        np.testing.assert_array_equal(q[1], w["q"][1])
+    else:
+        assert q is None
+
+
+def test_each_loader_rejects_the_other_models(tmp_path):
+    """each_loader_rejects_the_other_models (tests/sparse_persist.rs:286-343):
+    PersistFailed of kind "wrong model" whose message names the loader."""
+    _sparse_dir(tmp_path / "sgpr", "sgpr")
+    _sparse_dir(tmp_path / "svgp", "svgp")
+    cases = [(lambda: P.read_model(tmp_path / "sgpr"), "LoadedSgpr::load"),
+             (lambda: P.read_sparse(tmp_path / "sgpr", "svgp"), "LoadedSgpr::load"),
+             (lambda: P.read_sparse(tmp_path / "svgp", "sgpr"), "LoadedSvgp::load"),
+             (lambda: P.read_model(tmp_path / "svgp"), "LoadedSvgp::load")]
+    exact = tmp_path / "exact"
+    exact.mkdir()
+    cfg = json.loads((tmp_path / "sgpr" / "config.json").read_text())
+    for k in ("model", "m"):
+        cfg.pop(k)
+    cfg.update(has_factor=False, factor_kind="llt")
+    (exact / "config.json").write_text(json.dumps(cfg))
+    cases.append((lambda: P.read_sparse(exact, "sgpr"), "LoadedGpr::load"))
+    for call, loader in cases:
+        with pytest.raises(P.PersistFailed) as e:
+            call()
+        assert e.value.kind == "wrong model" and loader in str(e.value)
+
+
+@pytest.mark.parametrize("change, what", [
+    ({"model": "online_sgpr"}, "online"),
+    ({"model": "gp"}, "model"),
+    ({"format_version": 2}, "version"),
+    ({"m": 0}, "positive"),
+    ({"precision": "single"}, "precision"),
+    ({"residual": "promote"}, "residual"),
+    ({"math": "fast_approx"}, "math"),
+    ({"x_transform": {"standardize": {}}}, "x_transform"),
+    ({"distance": {"table": {}}}, "distance"),
+    ({"jitter": {"fixed": {"jitter": 1e-7}}}, "K_mm"),
+    ({"jitter": {"adaptive": {"initial": 1e-8, "multiplier": 10.0, "max_retries": 4,
+                              "max_jitter": 1e-3}}}, "K_mm"),
+])
+def test_sparse_load_refuses_what_this_backend_cannot_hold(tmp_path, change, what):
+    _sparse_dir(tmp_path, "sgpr")
+    _edit_config(tmp_path, **change)
+    with pytest.raises((P.PersistFailed, P.UnsupportedPersistVersion)) as e:
+        P.read_sparse(tmp_path, "sgpr")
+    assert what.lower() in str(e.value).lower()
+
+
+def _rewrite_tensors(path, **arrays):
+    t = stnp.load_file(str(path / "model.safetensors"))
+    t.update(arrays)
+    (path / "model.safetensors").write_bytes(stnp.save(t))
+
+
+def test_sparse_load_checks_z_train_and_shapes(tmp_path):
+    w = _sparse_dir(tmp_path, "sgpr")
+    z = stnp.load_file(str(tmp_path / "model.safetensors"))["z"]
+    bumped = z.copy()
+    bumped[0, 0] = np.nextafter(bumped[0, 0], np.inf)
+    _rewrite_tensors(tmp_path, z_train=bumped)
+    with pytest.raises(P.PersistFailed, match="z_train"):
+        P.read_sparse(tmp_path, "sgpr")
+    _rewrite_tensors(tmp_path, z_train=z, z=z[:-1])
+    with pytest.raises(P.PersistFailed, match="shape"):
+        P.read_sparse(tmp_path, "sgpr")
+    assert w["m"] == z.shape[0]
+
+
+@pytest.mark.parametrize("fault", ["upper", "zero_diag", "negative_diag", "nan_mean"])
+def test_svgp_load_checks_q(tmp_path, fault):
+    """read_q (src/persist/sparse.rs:393-407): q_l lower triangular with a
+    positive diagonal, q finite, else a tensor error."""
+    w = _sparse_dir(tmp_path, "svgp")
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    ql, qm = t["q_l"].copy(), t["q_mean"].copy()  # ql[c, r]: column-major bytes
+    if fault == "upper":
+        ql[2, 1] = 0.5          # row 1 of column 2, above the diagonal
+    elif fault == "zero_diag":
+        ql[1, 1] = 0.0
+    elif fault == "negative_diag":
+        ql[3, 3] = -ql[3, 3]
+    else:
+        qm[0] = np.nan
+    _rewrite_tensors(tmp_path, q_l=ql, q_mean=qm)
+    with pytest.raises(P.PersistFailed) as e:
+        P.read_sparse(tmp_path, "svgp")
+    assert e.value.kind == "tensor"
+    assert w["m"] == 4

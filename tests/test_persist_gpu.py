@@ -176,3 +176,68 @@ def test_failed_tensor_write_leaves_the_previous_config(gpu, tmp_path):
         fixed_rbf([0.0, 1.0, 2.0], [0.0, 1.0, 0.5]).save(tmp_path)
     assert (tmp_path / "config.json").read_bytes() == before
     assert sorted(p.name for p in tmp_path.iterdir()) == ["config.json", "model.safetensors"]
+
+
+# ------------------------------------------------------------------ sparse models
+#
+# save_sgpr / save_svgp and LoadedSgpr::load / LoadedSvgp::load
+# (src/persist/sparse.rs:165-195, 547-556, 728-737) as
+# tests/sparse_persist.rs checks them: a loaded model predicts as the saved
+# one, from the same x, y and z to the bit.
+
+@pytest.mark.parametrize("precision", [g.Precision.FP64, g.Precision.MIXED])
+def test_sgpr_save_load_round_trip(gpu, tmp_path, precision):
+    n, m, d = 700, 64, 3
+    x, y, xs = _data(n, d, 3)
+    z = x.reshape(d, n)[:, ::n // m][:, :m].reshape(-1).copy()  # column-major m x d
+    print(f"[persist] sgpr round trip n={n} m={m} d={d} precision={precision}", flush=True)
+    f = g.Sgpr(_spec(d), g.GaussianLikelihood(0.05)).with_precision(precision).factor(
+        x, n, d, y, z, m)
+    theta = f.get_params() + 0.05  # the parameters last evaluated are the ones saved
+    want_nlml = f.value(theta)
+    f.save(tmp_path)
+    cfg = json.loads((tmp_path / "config.json").read_text())
+    assert cfg["model"] == "sgpr" and (cfg["n"], cfg["m"], cfg["d"]) == (n, m, d)
+    assert cfg.get("precision", "double") == ("mixed" if precision == g.Precision.MIXED else "double")
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    np.testing.assert_array_equal(t["x"].reshape(-1), x)
+    np.testing.assert_array_equal(t["z"].reshape(-1), z)
+    back = g.FittedSgpr.load(tmp_path)
+    assert (back.n, back.m, back.d) == (n, m, d)
+    np.testing.assert_allclose(back.get_params(), theta, rtol=0, atol=1e-14)
+    # exp then log may move a parameter by an ulp, which can flip an f32
+    # rounding in the mixed form
+    tol = 1e-6 if precision == g.Precision.MIXED else 1e-12
+    np.testing.assert_allclose(back.neg_log_marginal_likelihood(), want_nlml, rtol=tol)
+    _assert_same_predictions(f, back, xs, d, tol)
+    with pytest.raises(P.PersistFailed, match="LoadedSgpr::load"):
+        g.FittedGpr.load(tmp_path)
+    with pytest.raises(P.PersistFailed, match="LoadedSgpr::load"):
+        g.FittedSvgp.load(tmp_path)
+
+
+def test_svgp_save_load_round_trip(gpu, tmp_path):
+    n, m, d = 900, 48, 2
+    x, y, xs = _data(n, d, 4)
+    z = x.reshape(d, n)[:, ::n // m][:, :m].reshape(-1).copy()
+    print(f"[persist] svgp round trip n={n} m={m} d={d}", flush=True)
+    f = g.Svgp(_spec(d), g.GaussianLikelihood(0.05)).prepare(x, n, d, y, z, m)
+    params, _ = f.adam(steps=30, batch_size=64, lr=0.01, seed=3)
+    f.save(tmp_path)
+    t = stnp.load_file(str(tmp_path / "model.safetensors"))
+    assert set(t) == {"x", "y", "z", "z_train", "q_mean", "q_l"}
+    np.testing.assert_array_equal(t["q_mean"], params[f.theta.size:f.theta.size + m])
+    assert not np.any(np.tril(t["q_l"], -1))  # [col, row] as read: row < col is zero
+    back = g.FittedSvgp.load(tmp_path)
+    np.testing.assert_allclose(back.params, params, rtol=0, atol=1e-14)
+    qn = xs.size // d
+    pa, pb = f.predict(xs, qn, d), back.predict(xs, qn, d)
+    np.testing.assert_allclose(pb.mean, pa.mean, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(pb.variance, pa.variance, rtol=1e-12, atol=1e-12)
+    ga, gb = np.empty(params.size), np.empty(params.size)
+    va = f.value_and_gradient_into(params, ga)
+    vb = back.value_and_gradient_into(back.params, gb)
+    np.testing.assert_allclose(vb, va, rtol=1e-12)
+    np.testing.assert_allclose(gb, ga, rtol=1e-9, atol=1e-9 * np.abs(ga).max())
+    with pytest.raises(P.PersistFailed, match="LoadedSvgp::load"):
+        g.FittedSgpr.load(tmp_path)

@@ -430,13 +430,14 @@ void upload(DevBuf<V>& dst, const std::vector<V>& src, hipStream_t s) {
 struct BatchModel : gprx_hip_model {
   BatchModel() : gprx_hip_model(kBatch) {}
   ~BatchModel() override {
+    // nothing of this batch is in flight once its stream has drained
+    if (stream) (void)hipStreamSynchronize(stream);
     for (gprx_hip_model* m : fb) gprx_hip_destroy(m);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (graph_b_exec) (void)hipGraphExecDestroy(graph_b_exec);
     if (graph_b) (void)hipGraphDestroy(graph_b);
     if (stream) {
-      (void)hipStreamSynchronize(stream);
       for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);
@@ -1309,7 +1310,9 @@ int gprx_hip_batch_set_schedule(gprx_hip_model* model, int32_t policy, int32_t s
       return GPRX_UNSUPPORTED;
     }
     auto* B = static_cast<BatchModel*>(model);
-    // a captured evaluation holds the old slots: capture again
+    // a captured evaluation holds the old slots: capture again (once no
+    // launch of the old graphs is in flight)
+    GPRX_HIP_CHECK(hipStreamSynchronize(B->stream));
     if (B->graph_exec) {
       GPRX_HIP_CHECK(hipGraphExecDestroy(B->graph_exec));
       B->graph_exec = nullptr;

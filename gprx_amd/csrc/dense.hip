@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "dense.hpp"
+#include "diag_block.hpp"
 #include "tile_gemm.hpp"
 #include "tile_mv.hpp"
 
@@ -274,405 +275,15 @@ template void gemm_unscheduled<double>(bool, bool, const GemmArgs<double>&, hipS
 template void gemm_unscheduled<float>(bool, bool, const GemmArgs<float>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// 128 x 128 diagonal block: Cholesky + inverse in LDS, in f64 arithmetic
-// for both forms: the mixed form's f32 blocks are read, factored and
-// inverted in f64, and their inverse is written back rounded to f32 (its
-// log-pivots stay f64).
-//
-// Layout in LDS (column stride DS): L lower (r >= c) at s[c * DS + r]; the
-// inverse X = L^-1 stored transposed above the diagonal, X[r][c] (r > c) at
-// s[r * DS + c]; X[r][r] = 1 / L[r][r] in xd[r].  Steps, per 16-column panel:
-//   (a) wave 0 factors the 16 x 16 diagonal sub-block in registers (lane r
-//       owns row r; pivots and columns are broadcast with v_readlane);
-//   (b) panel rows below, one row per lane: L_i = A_i L_D^-T by forward
-//       substitution;
-//   (c) trailing update A_ij -= L_i L_j^T (lower) on MFMA.
-// Then the inverses of the eight 16 x 16 diagonal blocks, one column per
-// lane, and the off-diagonal blocks of X by doubling, X21 = -X22 L21 X11
-// over runs of 16, 32 and 64 rows, on MFMA.  Only the factor's 128 pivots
-// form a serial chain; everything on it runs on one wave with the pivot
-// and column broadcasts in registers.
+// 128 x 128 diagonal blocks (diag_block.hpp): one workgroup per block.
 // ---------------------------------------------------------------------------
-// LDS column stride: odd, so a row read across lanes (stride DS) meets
-// distinct banks; for f32, DS = 17 mod 64 also spreads the four lane groups
-// of an MFMA operand read (rows k .. k + 3) over the banks.
-template <typename C>
-struct DiagCfg;
-template <>
-struct DiagCfg<double> {
-  static constexpr int DS = 129;
-};
-template <>
-struct DiagCfg<float> {
-  static constexpr int DS = 145;
-};
-
-// X[r][c] from the block's LDS layout (xd[kTile] holds 0.0).  The address
-// is selected and one load issued: a selected value would be turned back
-// into exec-mask branches around the loads, which keeps the compiler from
-// issuing the operands of several MFMAs ahead of their use.
-template <typename C>
-__device__ __forceinline__ C x_at(const C* s, const C* xd, int r, int c) {
-  constexpr int DS = DiagCfg<C>::DS;
-  const C* p = r > c ? s + r * DS + c : (r == c ? xd + r : xd + kTile);
-  return *p;
-}
-
-// Value of v in lane `src` of the wave, src uniform (a constant once the
-// loops around it are unrolled): two v_readlane_b32, no LDS round trip.
-__device__ __forceinline__ double lane_bcast(double v, int src) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float lane_bcast(float v, int src) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-
-__device__ __forceinline__ void tri_index_small(int t, int& u, int& v) {
-  u = 0;
-  while ((u + 1) * (u + 2) / 2 <= t) ++u;
-  v = t - u * (u + 1) / 2;
-}
-
-// p^-1/2 from the hardware estimate and Newton steps y <- y + y/2 (1 - p y^2),
-// each squaring the relative error: two for f64 (from an estimate good to
-// about f32 precision), one for f32.
-__device__ __forceinline__ float rsqrt_nr(float p) {
-  const float y = __builtin_amdgcn_rsqf(p);
-  const float e = fmaf(-p * y, y, 1.0f);
-  return fmaf(0.5f * y, e, y);
-}
-__device__ __forceinline__ double rsqrt_nr(double p) {
-  double y = __builtin_amdgcn_rsq(p);
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const double e = fma(-p * y, y, 1.0);
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
-}
-
-// One doubling level of the diagonal-block inverse: for each pair of
-// adjacent diagonal runs of HB 16-blocks at block offset o,
-//   T   = L21 X11   (X11 lower: k >= column)  -> stored over L21
-//   X21 = -X22 T    (X22 lower: k <= row)     -> X's transposed storage
-// L21 is read by other waves while T is formed, so T is held in registers
-// until a barrier; X21's slots are read by no one in the second product.
-// Wave w takes HB blocks (ui, uj) = (i, (w + i) mod HB), i < HB, of pair
-// w * pairs / 4: every wave then has the same reduction lengths in both
-// products (sum over uj of HB - uj, and over ui of ui + 1), and its HB
-// accumulator chains advance together in one K loop.
-template <typename C, int HB, int W>
-__device__ __forceinline__ void inverse_level_wave(C* s, const C* xd) {
-  constexpr int DS = DiagCfg<C>::DS;
-  using AV = typename Acc<C>::V;
-  constexpr int kPairs = 8 / (2 * HB);
-  constexpr int o = (W * kPairs / 4) * 2 * HB;
-  const int lane = threadIdx.x & 63;
-  AV acc[HB];
-#pragma unroll
-  for (int i = 0; i < HB; ++i) acc[i] = AV{};
-  // operands of every chain for one k step are loaded before its MFMAs, so
-  // one LDS round trip covers up to HB products; every bound is static
-#pragma unroll
-  for (int kb = 0; kb < HB; ++kb)
-#pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
-      const int k = (o + kb) * 16 + kk + (lane >> 4);
-      C a[HB], b[HB];
-#pragma unroll
-      for (int i = 0; i < HB; ++i) {
-        const int bi = o + HB + i, bj = o + (W + i) % HB;
-        if (kb >= (W + i) % HB) {  // X11 is lower: k >= column block
-          a[i] = s[k * DS + bi * 16 + (lane & 15)];
-          b[i] = x_at(s, xd, k, bj * 16 + (lane & 15));
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < HB; ++i)
-        if (kb >= (W + i) % HB) acc[i] = mfma16(a[i], b[i], acc[i]);
-    }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < HB; ++i) {
-    const int bi = o + HB + i, bj = o + (W + i) % HB;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      s[(bj * 16 + (lane & 15)) * DS + bi * 16 + drow<C>(lane, q)] = acc[i][q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < HB; ++i) acc[i] = AV{};
-#pragma unroll
-  for (int kb = 0; kb < HB; ++kb)
-#pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
-      const int k = (o + HB + kb) * 16 + kk + (lane >> 4);
-      C a[HB], b[HB];
-#pragma unroll
-      for (int i = kb; i < HB; ++i) {  // X22 is lower: k <= row block
-        const int bi = o + HB + i, bj = o + (W + i) % HB;
-        a[i] = x_at(s, xd, bi * 16 + (lane & 15), k);
-        b[i] = s[(bj * 16 + (lane & 15)) * DS + k];
-      }
-#pragma unroll
-      for (int i = kb; i < HB; ++i) acc[i] = mfma16(a[i], b[i], acc[i]);
-    }
-#pragma unroll
-  for (int i = 0; i < HB; ++i) {
-    const int bi = o + HB + i, bj = o + (W + i) % HB;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      s[(bi * 16 + drow<C>(lane, q)) * DS + bj * 16 + (lane & 15)] = -acc[i][q];
-  }
-  __syncthreads();
-}
-
-// Dispatch on the (scalar) wave index, so each wave's reduction bounds are
-// compile-time constants: data-dependent guards on MFMA accumulators make
-// the compiler copy every accumulator at each merge.
-template <typename C, int HB>
-__device__ __forceinline__ void inverse_level(C* s, const C* xd) {
-  switch (wave_index()) {
-    case 0: inverse_level_wave<C, HB, 0>(s, xd); break;
-    case 1: inverse_level_wave<C, HB, 1>(s, xd); break;
-    case 2: inverse_level_wave<C, HB, 2>(s, xd); break;
-    default: inverse_level_wave<C, HB, 3>(s, xd); break;
-  }
-}
-
-// Stage (a) of the diagonal block, run by one wave: factors the 16 x 16
-// block at (c0, c0) of s in registers and writes L (lower), X = L^-1
-// (transposed storage above the diagonal, diagonal in xd) and log L_ii.
-template <typename C>
-__device__ __forceinline__ void factor16(C* s, C* xd, double* logdiag, int* sbad, int c0) {
-  constexpr int DS = DiagCfg<C>::DS;
-  const int lane = threadIdx.x & 63;
-  // Lane r owns row r of the 16 x 16 block: d[k] = A/L[r][k].  Entries
-  // right of the diagonal (k > r) start at zero and collect values no lower
-  // entry ever reads, so every update runs unconditionally.  Step j: pivot
-  // p = A[j][j], rs = p^-1/2; column j of L is d[j] * rs (lane j: p rs =
-  // sqrt(p)); rank-1 update of the columns right of j.
-  const int r = lane & 15;
-  C d[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) d[k] = s[(c0 + k) * DS + c0 + r];
-  bool bad = false;
-  C lrr = 1, xrr = 1;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    C p = lane_bcast(d[j], j);
-    // f32: a subnormal pivot fails too (its estimate may be flushed)
-    if (!(p > 0) || !isfinite(p) || (sizeof(C) == 4 && p < (C)1.17549435e-38)) {
-      bad = true;
-      p = 1;
-    }
-    const C rs = rsqrt_nr(p);
-    d[j] *= rs;
-    // column j + 1 first: it holds the next pivot
-#pragma unroll
-    for (int c = j + 1; c < 16; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
-    if (r == j) {
-      lrr = d[j];
-      xrr = rs;
-    }
-  }
-  if (bad && lane == 0) *sbad = 1;
-  if (lane < 16) {
-    // Every register index is a compile-time constant: a lane-dependent
-    // index (d[r]) would move the arrays to scratch memory.
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-      if (k <= r) s[(c0 + k) * DS + c0 + r] = d[k];
-    xd[c0 + r] = xrr;
-    logdiag[c0 + r] = log((double)lrr);
-  }
-}
-
-// Panel rows below the diagonal block of columns c0..c0+15: row R of
-// L_i = A_i L_D^-T, one row per thread, by forward substitution with the
-// columns of L_D read as broadcasts (every lane the same address):
-//   x_j = a_j / L[j][j],  a_c -= x_j L[c][j]  (c > j).
-template <typename C>
-__device__ __forceinline__ void panel_rows(C* s, const C* xd, int c0, int rows) {
-  constexpr int DS = DiagCfg<C>::DS;
-  const int t = threadIdx.x;
-  if (t >= rows) return;
-  const int R = c0 + 16 + t;
-  C a[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) a[k] = s[(c0 + k) * DS + R];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    a[j] *= xd[c0 + j];
-#pragma unroll
-    for (int c = j + 1; c < 16; ++c) a[c] = fma(-a[j], s[(c0 + j) * DS + c0 + c], a[c]);
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) s[(c0 + k) * DS + R] = a[k];
-}
-
-// X_D = L_D^-1 for the eight 16 x 16 diagonal blocks: thread (b, c) of the
-// first 128 solves L_b x = e_c by forward substitution (x_k = 0 for k < c
-// without a branch) and stores X[k][c] (k > c) transposed above the
-// diagonal; X[c][c] is in xd already.
-template <typename C>
-__device__ __forceinline__ void invert_diag16(C* s, const C* xd) {
-  constexpr int DS = DiagCfg<C>::DS;
-  const int t = threadIdx.x;
-  if (t >= kTile) return;
-  const int o = (t >> 4) * 16, c = t & 15;
-  C b[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) b[k] = k == c ? 1 : 0;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    b[k] *= xd[o + k];
-#pragma unroll
-    for (int m = k + 1; m < 16; ++m) b[m] = fma(-b[k], s[(o + k) * DS + o + m], b[m]);
-  }
-#pragma unroll
-  for (int k = 1; k < 16; ++k)
-    if (k > c) s[(o + k) * DS + o + c] = b[k];
-}
-
-// A_(bi,bj) -= L_bi L_bj^T (panel columns c0 .. c0+15) for one block, or two
-// when `two` (independent chains interleaved); diagonal blocks keep r >= c.
-template <typename C>
-__device__ __forceinline__ void update_blocks(C* s, int c0, int bi, int bj, bool two, int bi2,
-                                              int bj2) {
-  constexpr int DS = DiagCfg<C>::DS;
-  using AV = typename Acc<C>::V;
-  const int lane = threadIdx.x & 63;
-  AV acc = AV{}, acc2 = AV{};
-#pragma unroll
-  for (int kk = 0; kk < 16; kk += 4) {
-    const int kr = kk + (lane >> 4);
-    const C a = s[(c0 + kr) * DS + bi * 16 + (lane & 15)];
-    const C b = s[(c0 + kr) * DS + bj * 16 + (lane & 15)];
-    acc = mfma16(a, b, acc);
-    if (two) {
-      const C a2 = s[(c0 + kr) * DS + bi2 * 16 + (lane & 15)];
-      const C b2 = s[(c0 + kr) * DS + bj2 * 16 + (lane & 15)];
-      acc2 = mfma16(a2, b2, acc2);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int row = bi * 16 + drow<C>(lane, q);
-    const int col = bj * 16 + (lane & 15);
-    if (bi != bj || row >= col) s[col * DS + row] -= acc[q];
-    if (two) {
-      const int row2 = bi2 * 16 + drow<C>(lane, q);
-      const int col2 = bj2 * 16 + (lane & 15);
-      if (bi2 != bj2 || row2 >= col2) s[col2 * DS + row2] -= acc2[q];
-    }
-  }
-}
-
-// T is the storage type of A and X; the block is factored and inverted in
-// f64 in LDS whatever T is (the mixed form's diagonal blocks are FP64).
-template <typename T>
-__device__ __forceinline__ void potrf_block_lds(const T* __restrict__ A, long lda, T* __restrict__ X, long ldx,
-                                double* __restrict__ logdiag, int* info, double* s, double* xd,
-                                int* sbad) {
-  using C = double;
-  constexpr int DS = DiagCfg<C>::DS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
-  if (tid == 0) {
-    *sbad = 0;
-    xd[kTile] = 0;  // the zero x_at reads above the diagonal
-  }
-  // 16 loads in flight per thread: the whole block lies inside the matrix,
-  // so the upper part is read and dropped rather than branched around.
-  constexpr int kBatch = 16;
-#pragma unroll
-  for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
-    T v[kBatch];
-#pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-      const int e = tid + 256 * (b0 + i);
-      v[i] = A[(e & 127) + (long)(e >> 7) * lda];
-    }
-#pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-      const int e = tid + 256 * (b0 + i);
-      const int r = e & 127, c = e >> 7;
-      s[c * DS + r] = (r >= c) ? (C)v[i] : (C)0;
-    }
-  }
-  __syncthreads();
-
-  // Right-looking over 16-column panels with lookahead: while waves 1-3
-  // apply panel jb to the trailing matrix, wave 0 applies it to diagonal
-  // block jb + 1 and factors that block (stage (a) is a serial chain on one
-  // wave, so it runs beside the bulk of the update instead of after it).
-  if (wave == 0) factor16(s, xd, logdiag, sbad, 0);
-  __syncthreads();
-  for (int jb = 0; jb < 8; ++jb) {
-    const int c0 = jb * 16;
-    // (b) panel: L_i = A_i L_D^-T for the rows below
-    panel_rows(s, xd, c0, kTile - c0 - 16);
-    __syncthreads();
-    if (jb == 7) break;
-    // (c) trailing update A_ij -= L_i L_j^T over the lower blocks of
-    // (jb+1 .. 7)^2: block (jb+1, jb+1) and stage (a) on wave 0, the others
-    // two at a time on waves 1-3.
-    if (wave == 0) {
-      update_blocks(s, c0, jb + 1, jb + 1, false, 0, 0);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      factor16(s, xd, logdiag, sbad, c0 + 16);
-    } else {
-      const int nb = 7 - jb;
-      const int count = nb * (nb + 1) / 2;
-      for (int t = wave; t < count; t += 6) {
-        int u, v;
-        tri_index_small(t, u, v);
-        int u2 = 0, v2 = 0;
-        const bool two = t + 3 < count;
-        if (two) tri_index_small(t + 3, u2, v2);
-        update_blocks(s, c0, jb + 1 + u, jb + 1 + v, two, jb + 1 + u2, jb + 1 + v2);
-      }
-    }
-    __syncthreads();
-  }
-
-  // X = L^-1: the 16 x 16 diagonal blocks, then by doubling over halves of
-  // HB blocks (HB = 1, 2, 4) X21 = -X22 (L21 X11).
-  invert_diag16(s, xd);
-  __syncthreads();
-  inverse_level<C, 1>(s, xd);
-  inverse_level<C, 2>(s, xd);
-  inverse_level<C, 4>(s, xd);
-#pragma unroll
-  for (int b0 = 0; b0 < kTile * kTile / 256; b0 += kBatch) {
-    T v[kBatch];
-#pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-      const int e = tid + 256 * (b0 + i);
-      v[i] = (T)x_at<C>(s, xd, e & 127, e >> 7);
-    }
-#pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-      const int e = tid + 256 * (b0 + i);
-      X[(e & 127) + (long)(e >> 7) * ldx] = v[i];
-    }
-  }
-  if (tid == 0 && *sbad) atomicExch(info, 1);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(const T* A, long lda, T* X, long ldx,
                                                     double* logdiag, int* info, long sa, long sx,
                                                     long slog) {
   extern __shared__ double smem[];
-  double* s = smem;
-  double* xd = s + kTile * DiagCfg<double>::DS;
-  int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const long z = blockIdx.x;
-  potrf_block_lds<T>(A + z * sa, lda, X + z * sx, ldx, logdiag + z * slog, info, s, xd, sbad);
+  diag::factor_block<T>(A + z * sa, lda, X + z * sx, ldx, logdiag + z * slog, info, smem);
 }
 
 template <typename T>
@@ -681,19 +292,13 @@ __global__ __launch_bounds__(256) void k_potrf_diag_list(T* A, T* X, const long*
                                                          const long* log_off, double* logdiag,
                                                          int* info, const int* info_idx) {
   extern __shared__ double smem[];
-  double* s = smem;
-  double* xd = s + kTile * DiagCfg<double>::DS;
-  int* sbad = reinterpret_cast<int*>(xd + kTile + 1);
   const int b = blockIdx.x;
-  potrf_block_lds<T>(A + a_off[b], ld[b], X + x_off[b], ld[b], logdiag + log_off[b],
-                     info + info_idx[b], s, xd, sbad);
+  diag::factor_block<T>(A + a_off[b], ld[b], X + x_off[b], ld[b], logdiag + log_off[b],
+                        info + info_idx[b], smem);
 }
 
-// LDS of the diagonal kernel: the block and its inverse's diagonal in f64
-// for either storage type.
 template <typename T>
-constexpr size_t kDiagLds =
-    (size_t)kTile * DiagCfg<double>::DS * sizeof(double) + (kTile + 1) * sizeof(double) + 16;
+constexpr size_t kDiagLds = diag::kLdsBytes;
 
 template <typename T>
 static void set_diag_lds_once() {

@@ -232,8 +232,10 @@ def test_svgp_save_load_round_trip(gpu, tmp_path):
     np.testing.assert_allclose(back.params, params, rtol=0, atol=1e-14)
     qn = xs.size // d
     pa, pb = f.predict(xs, qn, d), back.predict(xs, qn, d)
-    np.testing.assert_allclose(pb.mean, pa.mean, rtol=1e-12, atol=1e-12)
-    np.testing.assert_allclose(pb.variance, pa.variance, rtol=1e-12, atol=1e-12)
+    # the parameters pass through exp and log (an ulp each), which K_mm's
+    # conditioning carries into the predictions at about 1e-11
+    np.testing.assert_allclose(pb.mean, pa.mean, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(pb.variance, pa.variance, rtol=1e-10, atol=1e-10)
     ga, gb = np.empty(params.size), np.empty(params.size)
     va = f.value_and_gradient_into(params, ga)
     vb = back.value_and_gradient_into(back.params, gb)

@@ -45,10 +45,10 @@ PEAK_HBM = 8000.0  # GB/s
 # HBM-side bytes per X^T X launch at n = 16384: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on that launch, each scaled by what the counter reads for a 1 GiB
 # device copy (x2.0 and x1.0); the run is summarised under profiles/.
-XTX_TRAFFIC_N16384 = {"mixed": 2.957e+10, "fp64": 6.610e+10}
+XTX_TRAFFIC_N16384 = {"mixed": 2.957e+10, "fp64": 6.705e+10}
 # HBM-side bytes of one whole evaluation at n = 16384: FETCH_SIZE + WRITE_SIZE
 # summed over the evaluation's dispatches, scaled as above.
-EVAL_TRAFFIC_N16384 = {"fp64": 1.748e11}
+EVAL_TRAFFIC_N16384 = {"fp64": 1.745e11}
 
 
 def exact_flops(n: int, d: int) -> float:

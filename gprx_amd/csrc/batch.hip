@@ -243,17 +243,12 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
 // Sums each problem's partial rows [beg[p], end[p]) into out[p].
 __global__ __launch_bounds__(256) void k_reduce_by_problem(const double* partial, const int* beg,
                                                            const int* end, double* out) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int p = blockIdx.x / kMaxAcc, k = blockIdx.x % kMaxAcc;
   double acc = 0.0;
   for (int t = beg[p] + threadIdx.x; t < end[p]; t += 256) acc += partial[(long)t * kMaxAcc + k];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[(long)p * kMaxAcc + k] = red[0];
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) out[(long)p * kMaxAcc + k] = acc;
 }
 
 // Matrix-vector products over the lower 128-tiles (prob, ti, tj) of every
@@ -302,7 +297,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_problem_scalars(const ProbDesc* probs, const double* y,
                                                          const double* alpha, const double* logd,
                                                          const T* Ainv, double* out) {
-  __shared__ double r0[256], r1[256], r2[256];
+  __shared__ double r0[4], r1[4], r2[4];
   const ProbDesc pd = probs[blockIdx.x];
   double a = 0.0, b = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < pd.n; i += 256) {
@@ -311,22 +306,13 @@ __global__ __launch_bounds__(256) void k_problem_scalars(const ProbDesc* probs, 
     b += logd[pd.voff + i];
     if (Ainv) c += al * al - (double)Ainv[pd.moff + (long)i * (pd.npad + 1)];
   }
-  r0[threadIdx.x] = a;
-  r1[threadIdx.x] = b;
-  r2[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      r0[threadIdx.x] += r0[threadIdx.x + s];
-      r1[threadIdx.x] += r1[threadIdx.x + s];
-      r2[threadIdx.x] += r2[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  a = block_sum<256>(a, r0);
+  b = block_sum<256>(b, r1);
+  c = block_sum<256>(c, r2);
   if (threadIdx.x == 0) {
-    out[blockIdx.x * 8 + 0] = r0[0];
-    out[blockIdx.x * 8 + 1] = r1[0];
-    if (Ainv) out[blockIdx.x * 8 + 2] = r2[0];
+    out[blockIdx.x * 8 + 0] = a;
+    out[blockIdx.x * 8 + 1] = b;
+    if (Ainv) out[blockIdx.x * 8 + 2] = c;
   }
 }
 
@@ -337,7 +323,7 @@ __global__ __launch_bounds__(256) void k_problem_residual(const ProbDesc* probs,
                                                           const double* alpha, const double* q1,
                                                           const double* q2, double* r,
                                                           double* out) {
-  __shared__ double m0[256], m1[256], m2[256], m3[256];
+  __shared__ double m0[4], m1[4], m2[4], m3[4];
   const ProbDesc pd = probs[blockIdx.x];
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int i = threadIdx.x; i < pd.npad; i += 256) {
@@ -353,25 +339,15 @@ __global__ __launch_bounds__(256) void k_problem_residual(const ProbDesc* probs,
     a2 = fmax(a2, fabs(y[k]));
     if (q1) a3 = fmax(a3, q2 ? q1[k] + q2[k] : q1[k]);
   }
-  m0[threadIdx.x] = a0;
-  m1[threadIdx.x] = a1;
-  m2[threadIdx.x] = a2;
-  m3[threadIdx.x] = a3;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
-      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
-      m2[threadIdx.x] = fmax(m2[threadIdx.x], m2[threadIdx.x + s]);
-      m3[threadIdx.x] = fmax(m3[threadIdx.x], m3[threadIdx.x + s]);
-    }
-    __syncthreads();
-  }
+  a0 = block_max<256>(a0, m0);
+  a1 = block_max<256>(a1, m1);
+  a2 = block_max<256>(a2, m2);
+  a3 = block_max<256>(a3, m3);
   if (threadIdx.x == 0) {
-    out[blockIdx.x * 8 + 4] = m0[0];
-    out[blockIdx.x * 8 + 5] = m1[0];
-    out[blockIdx.x * 8 + 6] = m2[0];
-    if (q1) out[blockIdx.x * 8 + 7] = m3[0];
+    out[blockIdx.x * 8 + 4] = a0;
+    out[blockIdx.x * 8 + 5] = a1;
+    out[blockIdx.x * 8 + 6] = a2;
+    if (q1) out[blockIdx.x * 8 + 7] = a3;
   }
 }
 

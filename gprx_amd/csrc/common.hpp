@@ -51,6 +51,48 @@ __device__ __forceinline__ int wave_index() {
   return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
+// Sum over the 64 lanes of a wavefront by butterfly shuffles; every lane
+// ends with the same value, added in the same order.
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  return v;
+}
+struct SumOp {
+  __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct MaxOp {
+  __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+__device__ __forceinline__ double wave_sum(double v) { return wave_reduce(v, SumOp{}); }
+
+// Reduction over a workgroup of NT threads (all must call it): each
+// wavefront reduces by shuffles, the NT / 64 results meet in red (NT / 64
+// doubles, one array per concurrent reduction) and wave 0 reduces them by
+// shuffles again.  The result is in every lane of wave 0; the order is
+// fixed, so a rerun gives the same bits.
+template <int NT, typename Op>
+__device__ __forceinline__ double block_reduce(double v, double* red, Op op, double init) {
+  constexpr int NW = NT / 64;
+  static_assert(NT % 64 == 0 && NW <= 64, "whole wavefronts, at most 64");
+  v = wave_reduce(v, op);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = init;
+  if (wave == 0) t = wave_reduce(lane < NW ? red[lane] : init, op);
+  return t;
+}
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  return block_reduce<NT>(v, red, SumOp{}, 0.0);
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+  return block_reduce<NT>(v, red, MaxOp{}, 0.0);
+}
+
 struct DevProgram {
   int32_t ns;       // slots
   int32_t d;        // true input dimension (<= padded dimension)

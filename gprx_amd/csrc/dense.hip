@@ -408,12 +408,6 @@ void gemv_n(const T* M, long ld, int rows, int cols, const double* v, double* ou
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_t(const T* __restrict__ M, long ld, int rows,
                                                 int cols, const double* __restrict__ v,

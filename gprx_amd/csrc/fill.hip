@@ -424,17 +424,12 @@ __global__ __launch_bounds__(256) void k_contract_diag(const DevProgram* __restr
 
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial,
                                                          int tiles, double* __restrict__ out) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int k = blockIdx.x;
   double acc = 0.0;
   for (int t = threadIdx.x; t < tiles; t += 256) acc += partial[(long)t * kMaxAcc + k];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[k] = red[0];
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) out[k] = acc;
 }
 
 // ---------------------------------------------------------------------------

@@ -116,12 +116,6 @@ __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 // Block reduction of the accumulators into partial[tile * kMaxAcc + k].
 template <int DP>
 __device__ __forceinline__ void store_partials(double (&acc)[kMaxSlots][kAccPerLeaf],

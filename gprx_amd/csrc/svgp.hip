@@ -84,7 +84,7 @@ __global__ __launch_bounds__(1024) void k_point_terms(const double* yb, const do
                                                       const double* a2, const double* u2,
                                                       const double* kxx_p, int b, double* resid,
                                                       double* out) {
-  __shared__ double red[1024];
+  __shared__ double red[16];
   const double kxx = *kxx_p;
   double acc = 0.0;
   for (int c = threadIdx.x; c < b; c += 1024) {
@@ -94,13 +94,8 @@ __global__ __launch_bounds__(1024) void k_point_terms(const double* yb, const do
     acc += r * r + v;
   }
   for (int c = b + threadIdx.x; c < ((b + 127) / 128) * 128; c += 1024) resid[c] = 0.0;
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0];
+  acc = block_sum<1024>(acc, red);
+  if (threadIdx.x == 0) out[0] = acc;
 }
 
 // G = (mu resid^T + A - LU) / s2 over m x b, zero in the padding.
@@ -145,7 +140,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_q_grad(const double* mu, const double* packed, int m,
                                                 const double* Ares, const T* Mq, long ld, double c,
                                                 double* gmu, double* gL, double* part) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int j = blockIdx.x;
   const long col = (long)j * m - (long)j * (j - 1) / 2;  // packed index of (j, j)
   double s0 = 0.0;
@@ -156,14 +151,9 @@ __global__ __launch_bounds__(256) void k_q_grad(const double* mu, const double* 
     const double g = i == j ? v - 1.0 / v : v;
     gL[k] = g + c * (double)Mq[i + (long)j * ld];
   }
-  red[threadIdx.x] = s0;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
+  s0 = block_sum<256>(s0, red);
   if (threadIdx.x == 0) {
-    part[3 * j] = red[0];
+    part[3 * j] = s0;
     part[3 * j + 1] = log(packed[col]);
     part[3 * j + 2] = mu[j] * mu[j];
     gmu[j] = mu[j] - c * Ares[j];
@@ -172,19 +162,13 @@ __global__ __launch_bounds__(256) void k_q_grad(const double* mu, const double* 
 
 // out[q] = sum_j part[3 j + q], q = 0, 1, 2 (one workgroup, fixed order).
 __global__ __launch_bounds__(256) void k_q_sums(const double* part, int m, double* out) {
-  __shared__ double red[3][256];
+  __shared__ double red[3][4];
   double s[3] = {0.0, 0.0, 0.0};
   for (int j = threadIdx.x; j < m; j += 256)
     for (int q = 0; q < 3; ++q) s[q] += part[3 * j + q];
-  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = s[q];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    __syncthreads();
-  }
+  for (int q = 0; q < 3; ++q) s[q] = block_sum<256>(s[q], red[q]);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) out[q] = red[q][0];
+    for (int q = 0; q < 3; ++q) out[q] = s[q];
 }
 
 // Adam over the variational block: z holds mu (m) then the packed Lq in the

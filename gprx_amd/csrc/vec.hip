@@ -12,28 +12,21 @@ bool all_finite(const double* p, size_t n) {
   return true;
 }
 
-// One 1024-thread block; fixed-order tree reductions.
+// One 1024-thread block; wavefront-shuffle reductions in a fixed order.
+// out[0] = a.b (y'alpha), out[1] = sum c (the log pivots: log |A| / 2).
 __global__ __launch_bounds__(1024) void k_dot_and_sum(const double* a, const double* b,
                                                       const double* c, int n, double* out) {
-  __shared__ double r0[1024], r1[1024];
+  __shared__ double r0[16], r1[16];
   double s0 = 0.0, s1 = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) {
     s0 += a[i] * b[i];
     s1 += c[i];
   }
-  r0[threadIdx.x] = s0;
-  r1[threadIdx.x] = s1;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      r0[threadIdx.x] += r0[threadIdx.x + s];
-      r1[threadIdx.x] += r1[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  s0 = block_sum<1024>(s0, r0);
+  s1 = block_sum<1024>(s1, r1);
   if (threadIdx.x == 0) {
-    out[0] = r0[0];
-    out[1] = r1[0];
+    out[0] = s0;
+    out[1] = s1;
   }
 }
 
@@ -46,16 +39,11 @@ void dot_and_sum(const double* a, const double* b, const double* c, int n, doubl
 template <typename T>
 __global__ __launch_bounds__(1024) void k_noise_trace(const double* alpha, const T* M, long ld,
                                                       int n, double* out) {
-  __shared__ double r0[1024];
+  __shared__ double r0[16];
   double s0 = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) s0 += alpha[i] * alpha[i] - (double)M[(long)i * (ld + 1)];
-  r0[threadIdx.x] = s0;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) r0[threadIdx.x] += r0[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = r0[0];
+  s0 = block_sum<1024>(s0, r0);
+  if (threadIdx.x == 0) out[0] = s0;
 }
 
 template <typename T>
@@ -70,7 +58,7 @@ __global__ __launch_bounds__(1024) void k_residual_norms(const double* y, const 
                                                          const double* p2, const double* w,
                                                          const double* q1, const double* q2,
                                                          double* r, int n, double* out) {
-  __shared__ double m0[1024], m1[1024], m2[1024], m3[1024];
+  __shared__ double m0[16], m1[16], m2[16], m3[16];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) {
     const double ri = y[i] - (p2 ? p1[i] + p2[i] : p1[i]);
@@ -80,25 +68,15 @@ __global__ __launch_bounds__(1024) void k_residual_norms(const double* y, const 
     a2 = fmax(a2, fabs(y[i]));
     if (q1) a3 = fmax(a3, q2 ? q1[i] + q2[i] : q1[i]);
   }
-  m0[threadIdx.x] = a0;
-  m1[threadIdx.x] = a1;
-  m2[threadIdx.x] = a2;
-  m3[threadIdx.x] = a3;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
-      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
-      m2[threadIdx.x] = fmax(m2[threadIdx.x], m2[threadIdx.x + s]);
-      m3[threadIdx.x] = fmax(m3[threadIdx.x], m3[threadIdx.x + s]);
-    }
-    __syncthreads();
-  }
+  a0 = block_max<1024>(a0, m0);
+  a1 = block_max<1024>(a1, m1);
+  a2 = block_max<1024>(a2, m2);
+  a3 = block_max<1024>(a3, m3);
   if (threadIdx.x == 0) {
-    out[0] = m0[0];
-    out[1] = m1[0];
-    out[2] = m2[0];
-    if (q1) out[3] = m3[0];
+    out[0] = a0;
+    out[1] = a1;
+    out[2] = a2;
+    if (q1) out[3] = a3;
   }
 }
 
@@ -283,20 +261,15 @@ void symmetrize_lower(double* M, long ld, int n, hipStream_t s) {
 
 __global__ __launch_bounds__(1024) void k_frob_dot(const double* A, long lda, const double* B,
                                                    long ldb, int rows, int cols, double* out) {
-  __shared__ double red[1024];
+  __shared__ double red[16];
   double acc = 0.0;
   const long total = (long)rows * cols;
   for (long e = threadIdx.x; e < total; e += 1024) {
     const int i = (int)(e % rows), j = (int)(e / rows);
     acc += A[i + j * lda] * B[i + j * ldb];
   }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0];
+  acc = block_sum<1024>(acc, red);
+  if (threadIdx.x == 0) out[0] = acc;
 }
 
 void frob_dot(const double* A, long lda, const double* B, long ldb, int rows, int cols, double* out,
@@ -375,16 +348,11 @@ void add_diag(double* M, long ld, int n, double v, hipStream_t s) {
 }
 
 __global__ __launch_bounds__(1024) void k_diag_sum(const double* M, long ld, int n, double* out) {
-  __shared__ double red[1024];
+  __shared__ double red[16];
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) acc += M[(long)i * (ld + 1)];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0];
+  acc = block_sum<1024>(acc, red);
+  if (threadIdx.x == 0) out[0] = acc;
 }
 
 void diag_sum(const double* M, long ld, int n, double* out, hipStream_t s) {

@@ -71,19 +71,28 @@ inline void gemm_call(bool ta, bool tb, const T* A, long lda, const T* B, long l
 // one stream per device shared by every model, restricted to all CUs but
 // kReservedCus (2 on each of the 8 XCDs), so the chain's kernels always
 // find free CUs while the bulk product fills the rest.  Per recursion
-// depth k, ev[2k] hands L21 to bulk and ev[2k+1] returns Tmp.  Both streams
-// keep the default priority.
+// depth k, ev[2k] hands L21 to bulk and ev[2k+1] returns Tmp.  `ahead`,
+// restricted like bulk, takes the two top depths' lookahead (the first
+// columns of L21 and their share of the update, launched once a block's
+// first quarter is factored); aev[2k] hands X11's first quarter to it and
+// aev[2k+1] returns the update.  All three streams keep the default
+// priority.
 constexpr int kMaxDepth = 24;
 constexpr int kReservedCus = 16;
 // Shared bulk stream of the current device (created on first use).
 hipStream_t bulk_stream();
+// Shared lookahead stream of the current device (created on first use).
+hipStream_t ahead_stream();
 struct FactorStreams {
-  hipStream_t main = nullptr, bulk = nullptr;  // bulk is shared, not owned
+  hipStream_t main = nullptr, bulk = nullptr, ahead = nullptr;  // bulk, ahead shared
   hipEvent_t ev[2 * kMaxDepth] = {};
+  hipEvent_t aev[4] = {};  // lookahead hand-over per depth 0, 1
   void init(hipStream_t s) {
     main = s;
     bulk = bulk_stream();
+    ahead = ahead_stream();
     for (auto& e : ev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : aev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   // Every bulk product is waited for on main before the factor returns, so
   // once main is synchronized no bulk work of this model is pending.
@@ -93,7 +102,13 @@ struct FactorStreams {
         (void)hipEventDestroy(e);
         e = nullptr;
       }
+    for (auto& e : aev)
+      if (e) {
+        (void)hipEventDestroy(e);
+        e = nullptr;
+      }
     bulk = nullptr;
+    ahead = nullptr;
   }
 };
 

@@ -184,6 +184,28 @@ struct Prefetch<float> {
   static constexpr int P = 4;
 };
 
+// Instruction order of one slice: its LDS stores, global loads and the
+// next slice's fragment reads each issued between two of its NM MFMAs, so
+// the LDS work has finished by the time the last MFMA issues and the
+// barrier that closes the slice does not wait on it.
+template <int NM, int NG, int NW, int NR>
+__device__ __forceinline__ void interleave_slice() {
+  constexpr int kMfma = 0x008, kVmemRead = 0x020, kDsRead = 0x100, kDsWrite = 0x200;
+  static_for<NW>([&](auto) {
+    __builtin_amdgcn_sched_group_barrier(kMfma, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(kDsWrite, 1, 0);
+  });
+  static_for<NG>([&](auto) {
+    __builtin_amdgcn_sched_group_barrier(kMfma, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(kVmemRead, 1, 0);
+  });
+  static_for<NR>([&](auto) {
+    __builtin_amdgcn_sched_group_barrier(kMfma, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(kDsRead, 1, 0);
+  });
+  if constexpr (NM > NG + NW + NR) __builtin_amdgcn_sched_group_barrier(kMfma, NM - NG - NW - NR, 0);
+}
+
 // acc += op(A)[ti tile, kb..kb+nk*BK) op(B)[.., tj tile], with Ablk / Bblk
 // the tile origins: A "row-contiguous" operand has element (mn, k) at
 // P[mn + k * ld], a "k-contiguous" one at P[k + mn * ld].  Four waves in a
@@ -292,6 +314,8 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
       }
     }
   };
+  // LDS fragment reads of one slice per lane
+  constexpr int NR = (BK / (4 * VEC)) * FB * ((TA ? 1 : VEC) + (TB ? VEC : 1));
   auto mma = [&](const Frag& f) {
 #pragma unroll
     for (int st = 0; st < NS; ++st)
@@ -330,6 +354,7 @@ __device__ __forceinline__ void tile_mainloop(const T* __restrict__ Ablk, long l
     lstore(u & 1, ra[(u + 2) % P], rb[(u + 2) % P]);
     gload(koff(i + 2 + P), ra[(u + 2) % P], rb[(u + 2) % P]);
     mma(fr[u & 1]);
+    interleave_slice<NS * FB * FB, 2 * NV, 2 * NV, NR>();
     __syncthreads();
   };
   int it = 0;

@@ -62,10 +62,11 @@ struct ExactModel : gprx_hip_model {
   DevBuf<int> info;
   HostBuf<double> hscal, hacc;
   HostBuf<int> hinfo;
-  // online insertion: host copies of the data (column-major n x d, n), and
-  // device work vectors
+  // online insertion: host copies of the data (point-major: point i at
+  // hx[i d .. i d + d), so an insertion appends d values), and device work
+  // vectors
   std::vector<double> hx, hy;
-  DevBuf<double> zv, kcol, lv, wv, xnew, gpart1;
+  DevBuf<double> zv, kcol, lv, wv, xnew;
   bool z_valid = false;  // zv = X y for the factor in X64
 
   DevPoints pts() const { return DevPoints{x.p, npad, n}; }
@@ -398,6 +399,17 @@ void upload_points(DevBuf<double>& dst, const double* src, int n, int d, long ro
   GPRX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// The same from point-major host data (point i at src[i d .. i d + d)).
+void upload_point_rows(DevBuf<double>& dst, const double* src, int n, int d, long rows_pad, int dp,
+                       hipStream_t s) {
+  std::vector<double> h((size_t)rows_pad * dp, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < d; ++k) h[(size_t)k * rows_pad + i] = src[(size_t)i * d + k];
+  dst.alloc(h.size());
+  GPRX_HIP_CHECK(hipMemcpyAsync(dst.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+  GPRX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 namespace gprx {
@@ -547,6 +559,23 @@ __global__ __launch_bounds__(256) void k_transpose_lower(const double* __restric
   }
 }
 
+// The inserted point by value (kernel argument): no host staging.
+struct NewPoint {
+  double x[kMaxDim];
+  double y;
+};
+
+// xnew (kTile x dp, the one-point set) = the point in row 0, zeros below;
+// row n of the model's x and y = the point and its target.
+__global__ void k_insert_point(NewPoint pt, double* __restrict__ xnew, double* __restrict__ x,
+                               long ldx, double* __restrict__ y, int n, int d) {
+  const int q = blockIdx.x, i = threadIdx.x;
+  const double v = q < d ? pt.x[q] : 0.0;
+  xnew[(long)q * kTile + i] = i == 0 ? v : 0.0;
+  if (i == 0 && q < d) x[(long)q * ldx + n] = v;
+  if (i == 0 && q == 0) y[n] = pt.y;
+}
+
 __global__ void k_insert_scalars(double* sc, double noise, double y_new, double* z, double* logdiag,
                                  int n) {
   const double piv2 = sc[12] + noise - sc[8];
@@ -685,7 +714,7 @@ void regrow(ExactModel* m, long npad_new) {
   for (DevBuf<double>* b : {&m->A64, &m->X64, &m->T64, &m->K64}) b->release();
   for (DevBuf<float>* b : {&m->A32, &m->X32, &m->T32}) b->release();
   size_buffers(m);
-  upload_points(m->x, m->hx.data(), m->n, m->d, m->npad, m->dp, s);
+  upload_point_rows(m->x, m->hx.data(), m->n, m->d, m->npad, m->dp, s);
   std::vector<double> yh(m->npad, 0.0);
   std::memcpy(yh.data(), m->hy.data(), sizeof(double) * m->n);
   m->y.alloc(m->npad);
@@ -701,7 +730,8 @@ void regrow(ExactModel* m, long npad_new) {
 // alpha = X^T z = alpha_old - (z_n / d) X^T l (plus alpha_n = z_n / d),
 // NLML = 1/2 z^T z + sum log L_ii + n/2 log 2 pi.  z is kept from one
 // insertion to the next (recomputed after a full evaluation), so an
-// insertion reads X's lower triangle twice (X k, X^T l).  When the padding
+// insertion reads X's lower triangle twice (X k, X^T l), one workgroup per
+// 128-tile.  When the padding
 // is full the model is re-padded by 128 rows and factored again.
 int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_error* err) {
   if (m->precision != GPRX_FP64) {
@@ -715,22 +745,15 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   hipStream_t s = m->stream;
   const int n = m->n, d = m->d;
   const double* th = m->theta.data();
-  // host copies with the new point appended (column-major n x d)
-  std::vector<double> hx_new((size_t)(n + 1) * d);
-  for (int q = 0; q < d; ++q) {
-    std::memcpy(hx_new.data() + (size_t)q * (n + 1), m->hx.data() + (size_t)q * n, sizeof(double) * n);
-    hx_new[(size_t)q * (n + 1) + n] = xn[q];
-  }
   if (n + 1 > m->npad) {
-    std::vector<double> hx_old = m->hx;
-    m->hx = hx_new;
+    m->hx.insert(m->hx.end(), xn, xn + d);
     m->hy.push_back(yn);
     m->n = n + 1;
     regrow(m, m->npad + kTile);
     double v = 0.0;
     int rc = m->evaluate(th, false, &v, nullptr, err);
     if (rc != GPRX_OK) {  // back to the model before the insert
-      m->hx = hx_old;
+      m->hx.resize((size_t)n * d);
       m->hy.pop_back();
       m->n = n;
       regrow(m, m->npad - kTile);
@@ -754,16 +777,23 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   m->lv.alloc(np);
   m->wv.alloc(np);
   m->xnew.alloc((size_t)kTile * m->dp);
-  m->gpart1.alloc((size_t)np * ((np + 255) / 256));
-  // the new point as a one-row point set, and z = X y of the current factor
-  GPRX_HIP_CHECK(hipMemsetAsync(m->xnew.p, 0, sizeof(double) * kTile * m->dp, s));
-  GPRX_HIP_CHECK(hipMemcpy2DAsync(m->xnew.p, sizeof(double) * kTile, xn, sizeof(double),
-                                  sizeof(double), d, hipMemcpyHostToDevice, s));
+  // the new point as a one-row point set (and its row of x and y, cleared
+  // again below if the pivot fails), and z = X y of the current factor
+  NewPoint pt{};
+  for (int q = 0; q < d; ++q) pt.x[q] = xn[q];
+  pt.y = yn;
+  hipLaunchKernelGGL(k_insert_point, dim3(m->dp), dim3(kTile), 0, s, pt, m->xnew.p, m->x.p, np,
+                     m->y.p, n, d);
+  GPRX_HIP_CHECK(hipGetLastError());
   const DevPoints pn{m->xnew.p, kTile, 1};
-  if (!m->z_valid) gemv_n<double>(m->X64.p, np, n, n, m->y.p, m->zv.p, m->gpart1.p, 1, 0, 0, s);
+  // Products with X run over its padded lower tiles: the padding block of X
+  // is the identity and the padded entries of y and of k are zero, so the
+  // first n entries are those of the n x n products.
+  if (!m->z_valid)
+    tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvRow, m->y.p, m->zv.p, nullptr, m->tpart.p, s);
   fill_cross<double>(m->dp, m->prog.p, m->pts(), pn, m->kcol.p, np, s);
   fill_diag(m->dp, m->prog.p, pn, m->scal.p + 12, s);
-  gemv_n<double>(m->X64.p, np, n, n, m->kcol.p, m->lv.p, m->gpart1.p, 1, 0, 0, s);
+  tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvRow, m->kcol.p, m->lv.p, nullptr, m->tpart.p, s);
   dot_and_sum(m->lv.p, m->lv.p, m->lv.p, n, m->scal.p + 8, s);
   dot_and_sum(m->lv.p, m->zv.p, m->lv.p, n, m->scal.p + 10, s);
   // the pivot, z_n and log d on the device; every update below is skipped
@@ -774,16 +804,13 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
   GPRX_HIP_CHECK(hipGetLastError());
   // X's new row, then alpha = X^T z: the old alpha (= X^T z of the old
   // rows) plus the new row's term, from w = X^T l; the NLML sums
-  gemv_t<double>(m->X64.p, np, n, n, m->lv.p, m->wv.p, 1, 0, 0, 0, s);
+  tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvCol, m->lv.p, m->wv.p, nullptr, m->tpart.p, s);
   hipLaunchKernelGGL(k_insert_row, dim3((n + 1 + 255) / 256), dim3(256), 0, s, m->X64.p, np, n,
                      m->wv.p, m->scal.p);
   GPRX_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_alpha_extend, dim3((n + 1 + 255) / 256), dim3(256), 0, s, m->alpha.p,
                      m->wv.p, n, m->scal.p);
   GPRX_HIP_CHECK(hipGetLastError());
-  GPRX_HIP_CHECK(hipMemcpy2DAsync(m->x.p + n, sizeof(double) * np, xn, sizeof(double),
-                                  sizeof(double), d, hipMemcpyHostToDevice, s));
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p + n, &yn, sizeof(double), hipMemcpyHostToDevice, s));
   dot_and_sum(m->zv.p, m->zv.p, m->logdiag.p, n + 1, m->scal.p, s);
   GPRX_HIP_CHECK(hipMemcpyAsync(m->hscal.p, m->scal.p, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
   GPRX_HIP_CHECK(hipStreamSynchronize(s));
@@ -800,7 +827,7 @@ int exact_insert(ExactModel* m, const double* xn, double yn, double* nlml, gprx_
     return GPRX_CHOLESKY_FAILED;
   }
   m->n = n + 1;
-  m->hx = hx_new;
+  m->hx.insert(m->hx.end(), xn, xn + d);
   m->hy.push_back(yn);
   m->z_valid = true;
   if (nlml) *nlml = 0.5 * m->hscal.p[0] + m->hscal.p[1] + 0.5 * (double)(n + 1) * kLog2Pi;
@@ -834,7 +861,9 @@ int gprx_hip_exact_create(const double* x, size_t n, size_t d, const double* y,
       return GPRX_UNSUPPORTED;
     }
     auto m = std::make_unique<ExactModel>();
-    m->hx.assign(x, x + n * d);
+    m->hx.resize(n * d);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t q = 0; q < d; ++q) m->hx[i * d + q] = x[q * n + i];
     m->hy.assign(y, y + n);
     std::string why;
     int rc = parse_program(kernel, (int)d, &m->hp, &why);
@@ -978,14 +1007,10 @@ int exact_delete_downdate(ExactModel* m, int p, double* nlml, gprx_error* err) {
   std::swap(m->X64.p, m->A64.p);
   std::swap(m->X64.n, m->A64.n);
   // the points and targets without row p (rows keep their order)
-  std::vector<double> hx_new((size_t)(n - 1) * d);
-  for (int q = 0; q < d; ++q)
-    for (int i = 0, k = 0; i < n; ++i)
-      if (i != p) hx_new[(size_t)q * (n - 1) + k++] = m->hx[(size_t)q * n + i];
-  m->hx = std::move(hx_new);
+  m->hx.erase(m->hx.begin() + (long)p * d, m->hx.begin() + (long)(p + 1) * d);
   m->hy.erase(m->hy.begin() + p);
   m->n = n - 1;
-  upload_points(m->x, m->hx.data(), m->n, d, np, m->dp, s);
+  upload_point_rows(m->x, m->hx.data(), m->n, d, np, m->dp, s);
   std::vector<double> yh((size_t)np, 0.0);
   std::memcpy(yh.data(), m->hy.data(), sizeof(double) * m->n);
   GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
@@ -1005,10 +1030,8 @@ int exact_delete(ExactModel* m, size_t index, double* nlml, gprx_error* err) {
   if (m->precision == GPRX_FP64) return exact_delete_downdate(m, (int)index, nlml, err);
   // the mixed form factors the remaining points again
   const int n = m->n, d = m->d;
-  std::vector<double> hx_new((size_t)(n - 1) * d);
-  for (int q = 0; q < d; ++q)
-    for (int i = 0, k = 0; i < n; ++i)
-      if ((size_t)i != index) hx_new[(size_t)q * (n - 1) + k++] = m->hx[(size_t)q * n + i];
+  std::vector<double> hx_new = m->hx;
+  hx_new.erase(hx_new.begin() + (long)index * d, hx_new.begin() + (long)(index + 1) * d);
   std::vector<double> hx_old = m->hx, hy_old = m->hy;
   const long npad_old = m->npad;
   m->hx = hx_new;

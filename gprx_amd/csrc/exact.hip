@@ -639,7 +639,9 @@ __global__ __launch_bounds__(256) void k_delete_rows(const double* __restrict__ 
   }
   const long j = jn < p ? jn : jn + 1;
   const double* xc = X + j * ld;
-  for (int i = tid; i < p; i += 256) yc[i] = (j < p) ? xc[i] : 0.0;
+  // rows above p: old column j < p copied from its diagonal down, zeros
+  // above it (X's upper triangle is zero; the export's X K + s X reads it)
+  for (int i = tid; i < p; i += 256) yc[i] = (j < p && i >= j) ? xc[i] : 0.0;
   if (tid == 0) yc[ld - 1] = 0.0;  // new padding row
   const int mt = (int)(ld - 1 - p);  // trailing rows: new p .. ld-2 from old p+1 .. ld-1
   if (mt <= 0) return;
@@ -701,6 +703,93 @@ __global__ void k_delete_logdiag(const double* __restrict__ ld_old, const double
     out[i] = ld_old[i + 1] + lg[i - p];
   else
     out[i] = 0.0;
+}
+
+// Coefficients of the downdate of point p (k_delete_rows): v_k =
+// -X[p+1+k][p] / X[p][p], beta_k = 1 + sum_{i<=k} v_i^2 by a fixed-order
+// scan, 1 / beta_{k-1}, 1 / G_kk and log G_kk = 1/2 log1p(v_k^2 / beta_{k-1})
+// for k < mt.  One workgroup.
+constexpr int kCoeffThreads = 1024;
+__global__ __launch_bounds__(kCoeffThreads) void k_delete_coeffs(const double* __restrict__ X,
+                                                                 long ld, int p, int mt,
+                                                                 double* __restrict__ v,
+                                                                 double* __restrict__ ibp,
+                                                                 double* __restrict__ ig,
+                                                                 double* __restrict__ lg) {
+  constexpr int W = kCoeffThreads / 64, E = 16;
+  __shared__ double wsum[W];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double* col = X + (long)p * ld + p;  // col[0] = X[p][p]
+  const double xpp = col[0];
+  double carry = 0.0;
+  for (int c0 = 0; c0 < mt; c0 += kCoeffThreads * E) {
+    double sq[E];
+    double tsum = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = c0 + tid * E + e;
+      double vk = 0.0;
+      if (k < mt) {
+        vk = -col[1 + k] / xpp;
+        v[k] = vk;
+      }
+      sq[e] = vk * vk;
+      tsum += sq[e];
+    }
+    double incl = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    double before = 0.0, chunk = 0.0;
+    for (int w = 0; w < W; ++w) {
+      if (w < wave) before += wsum[w];
+      chunk += wsum[w];
+    }
+    __syncthreads();
+    double run = carry + before + (incl - tsum);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = c0 + tid * E + e;
+      if (k < mt) {
+        const double bp = 1.0 + run, b = bp + sq[e];
+        ibp[k] = 1.0 / bp;
+        ig[k] = 1.0 / sqrt(b / bp);
+        lg[k] = 0.5 * log1p(sq[e] / bp);
+      }
+      run += sq[e];
+    }
+    carry += chunk;
+  }
+}
+
+// Rows p+1 .. n-1 of the points (dp columns of ld rows) and of y, out to
+// tmp; k_shift_rows_in writes them back one row up and clears row n-1, so
+// no thread reads a row another has moved.
+__global__ void k_shift_rows_out(const double* __restrict__ x, long ld, int dp,
+                                 const double* __restrict__ y, int n, int p, double* __restrict__ tmp) {
+  const long cnt = n - 1 - p, total = cnt * (dp + 1);
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long q = idx / cnt, k = idx % cnt;
+    tmp[idx] = q < dp ? x[q * ld + p + 1 + k] : y[p + 1 + k];
+  }
+}
+__global__ void k_shift_rows_in(double* __restrict__ x, long ld, int dp, double* __restrict__ y, int n,
+                                int p, const double* __restrict__ tmp) {
+  const long cnt = n - 1 - p, total = (cnt + 1) * (dp + 1);
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long q = idx / (cnt + 1), k = idx % (cnt + 1);
+    const double val = k < cnt ? tmp[q * cnt + k] : 0.0;
+    if (q < dp)
+      x[q * ld + p + k] = val;
+    else
+      y[p + k] = val;
+  }
 }
 
 // Re-pads the model for n_new points: every npad-sized buffer again, the
@@ -971,30 +1060,11 @@ int exact_delete_downdate(ExactModel* m, int p, double* nlml, gprx_error* err) {
   const int n = m->n, d = m->d;
   const long np = m->npad;
   const int mt = (int)(np - 1 - p);
-  // column p of X from the diagonal down
-  std::vector<double> col((size_t)(np - p));
-  GPRX_HIP_CHECK(hipMemcpyAsync(col.data(), m->X64.p + p + (long)p * np, sizeof(double) * (np - p),
-                                hipMemcpyDeviceToHost, s));
-  GPRX_HIP_CHECK(hipStreamSynchronize(s));
-  const double xpp = col[0];
-  std::vector<double> hv(std::max(mt, 1)), hibp(std::max(mt, 1)), hig(std::max(mt, 1)),
-      hlg(std::max(mt, 1));
-  double beta_prev = 1.0;
-  for (int k = 0; k < mt; ++k) {
-    const double vk = -col[(size_t)k + 1] / xpp;
-    const double t = vk * vk / beta_prev;
-    const double beta = beta_prev + vk * vk;
-    hv[k] = vk;
-    hibp[k] = 1.0 / beta_prev;
-    hig[k] = 1.0 / std::sqrt(beta / beta_prev);
-    hlg[k] = 0.5 * std::log1p(t);
-    beta_prev = beta;
-  }
+  // the downdate's coefficients from column p of X, on the device
   if (mt > 0) {
-    GPRX_HIP_CHECK(hipMemcpyAsync(m->r.p, hv.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
-    GPRX_HIP_CHECK(hipMemcpyAsync(m->dlt.p, hibp.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
-    GPRX_HIP_CHECK(hipMemcpyAsync(m->p1.p, hig.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
-    GPRX_HIP_CHECK(hipMemcpyAsync(m->q1.p, hlg.data(), sizeof(double) * mt, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_delete_coeffs, dim3(1), dim3(kCoeffThreads), 0, s, m->X64.p, np, p, mt,
+                       m->r.p, m->dlt.p, m->p1.p, m->q1.p);
+    GPRX_HIP_CHECK(hipGetLastError());
   }
   // the downdated X into A64 (free between evaluations), then the buffers swap
   hipLaunchKernelGGL(k_delete_rows, dim3((unsigned)np), dim3(256), 0, s, m->X64.p, m->A64.p, np, p,
@@ -1006,14 +1076,25 @@ int exact_delete_downdate(ExactModel* m, int p, double* nlml, gprx_error* err) {
   GPRX_HIP_CHECK(hipMemcpyAsync(m->logdiag.p, m->tv.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
   std::swap(m->X64.p, m->A64.p);
   std::swap(m->X64.n, m->A64.n);
-  // the points and targets without row p (rows keep their order)
+  // the points and targets without row p (rows keep their order), on the
+  // device and in the host copies
+  if (p < n - 1) {
+    m->kcol.alloc((size_t)np * kTile);  // scratch: (n - 1 - p) (dp + 1) <= np kTile
+    const long cnt = (long)(n - 1 - p) * (m->dp + 1);
+    const unsigned blocks = (unsigned)std::min<long>((cnt + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_shift_rows_out, dim3(blocks), dim3(256), 0, s, m->x.p, np, m->dp, m->y.p, n,
+                       p, m->kcol.p);
+    hipLaunchKernelGGL(k_shift_rows_in, dim3(blocks), dim3(256), 0, s, m->x.p, np, m->dp, m->y.p, n, p,
+                       m->kcol.p);
+    GPRX_HIP_CHECK(hipGetLastError());
+  } else {  // the last point: its row becomes padding
+    hipLaunchKernelGGL(k_shift_rows_in, dim3(1), dim3(256), 0, s, m->x.p, np, m->dp, m->y.p, n, p,
+                       m->kcol.p);
+    GPRX_HIP_CHECK(hipGetLastError());
+  }
   m->hx.erase(m->hx.begin() + (long)p * d, m->hx.begin() + (long)(p + 1) * d);
   m->hy.erase(m->hy.begin() + p);
   m->n = n - 1;
-  upload_point_rows(m->x, m->hx.data(), m->n, d, np, m->dp, s);
-  std::vector<double> yh((size_t)np, 0.0);
-  std::memcpy(yh.data(), m->hy.data(), sizeof(double) * m->n);
-  GPRX_HIP_CHECK(hipMemcpyAsync(m->y.p, yh.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
   // alpha = X^T (X y) and the NLML's sums
   tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvRow, m->y.p, m->tv.p, nullptr, m->tpart.p, s);
   tile_mv_lower<double>(m->X64.p, np, m->nt, kTmvCol, m->tv.p, m->alpha.p, nullptr, m->tpart.p, s);

@@ -98,7 +98,8 @@ def test_deletions_match_a_fresh_factor_of_the_remaining_points(gpu):
     keep = list(range(n))
     m = g.Gpr(spec, lik).factor(x, n, d, y)
     print("[online] deletions from 140 points (across the 128-row padding) and an insertion", flush=True)
-    for idx in (0, 70, 11, len(keep) - 4, 5, 3, 2, 1, 0, 100, 50, 20, 7):
+    for idx in (0, 70, 11, len(keep) - 4, 5, 3, 2, 1, 0, 100, 50, 20, 7, -1):
+        idx = idx if idx >= 0 else len(keep) - 1  # -1: the last point
         nlml = m.delete(idx)
         keep.pop(idx)
         k = len(keep)

@@ -245,6 +245,14 @@ static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream, boo
 // 63.5; below 256: 35.8 / 62.1).
 constexpr int kSmallTileBelow = 256;
 constexpr int kTinyTileBelow = 64;
+// A launch with a triangular operand gives its tiles K-ranges up to the
+// whole K apart, so below two 128-tiles per CU its longest tile sets the
+// time: 64-tiles there (a quarter of that tile's work, four times the
+// tiles to balance).  Measured on gfx950: the m = 2048 products X^T H X of
+// an Svgp step 265 -> about 150 us each (step 3.41 -> 3.17 ms mixed, 4.56
+// -> 4.30 ms FP64), the 2048-wide products of the n = 16384 factor (FP64
+// 54.8 -> 53.5 ms).
+constexpr int kTriSmallTileBelow = 512;
 
 template <typename T>
 static void gemm_tiles(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream, bool scheduled) {
@@ -253,7 +261,7 @@ static void gemm_tiles(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t strea
   if (total == 0) return;
   if (total < kTinyTileBelow)
     launch_gemm<T, 32>(ta, tb, g, stream, scheduled);
-  else if (total < kSmallTileBelow)
+  else if (total < kSmallTileBelow || (g.krange && total < kTriSmallTileBelow))
     launch_gemm<T, 64>(ta, tb, g, stream, scheduled);
   else
     launch_gemm<T, 128>(ta, tb, g, stream, scheduled);

@@ -239,6 +239,8 @@ struct SvgpModel : gprx_hip_model {
     if (stream) {
       for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+      for (auto& e : sev)
+        if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);
     }
   }
@@ -250,6 +252,10 @@ struct SvgpModel : gprx_hip_model {
   hipStream_t stream = nullptr;
   FactorStreams fstreams;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // sev[0]: the step's program and batch indices are on the device; sev[1]:
+  // the factor-independent start of the step (on aux) is done
+  hipEvent_t sev[2] = {nullptr, nullptr};
+  hipStream_t aux = nullptr;  // shared, not owned
   DevBuf<double> x, y, z;
   DevBuf<DevProgram> prog;
   HostBuf<DevProgram> hprog;
@@ -296,6 +302,7 @@ struct SvgpModel : gprx_hip_model {
   // Fill and factor K_mm at jitter j (info bits in info; no wait).
   void launch_factor(const double* th, double j) {
     upload_program(th, j);
+    GPRX_HIP_CHECK(hipEventRecord(sev[0], stream));
     GPRX_HIP_CHECK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
     fill_lower<double>(dp, prog.p, zp(), KA.p, mpad, nullptr, info.p, stream);
     factor_inverse<double>(KA.p, Xmm.p, Tm.p, mpad, ldt, mt, logd.p, info.p, fstreams);
@@ -356,20 +363,27 @@ struct SvgpModel : gprx_hip_model {
     GPRX_HIP_CHECK(hipEventRecord(ev[0], s));
     const long bp = (long)round_up((size_t)b, kTile);
     const int M = (int)mpad, B = (int)bp;
-    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, s, x.p, npad, dp, y.p, bidx.p,
-                       (int)b, B, xb.p, yb.p);
+    const double* pu = params_d.p;
     DevPoints xbp{xb.p, bp, (int)b};
-    // storage copies of X = L^-1 and Lq
+    // What does not depend on K_mm's factor runs on aux beside it, from the
+    // moment the program and the batch indices are on the device (sev[0],
+    // recorded by launch_factor): the batch's points, K(Z, X_b), Lq and
+    // k(x, x).
+    GPRX_HIP_CHECK(hipStreamWaitEvent(aux, sev[0], 0));
+    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, aux, x.p, npad, dp, y.p, bidx.p,
+                       (int)b, B, xb.p, yb.p);
+    hipLaunchKernelGGL((k_unpack_lq<T>), dim3(grid_for(mpad * mpad)), dim3(256), 0, aux,
+                       pu + n_theta + m, m, mpad, Lc, Ld.p);
+    fill_cross<T>(dp, prog.p, zp(), xbp, Kzx, mpad, aux);
+    fill_diag(dp, prog.p, DevPoints{z.p, mpad, 1}, scal.p + 7, aux);
+    GPRX_HIP_CHECK(hipEventRecord(sev[1], aux));
+    // storage copy of X = L^-1
     if (std::is_same<T, double>::value) {
       GPRX_HIP_CHECK(hipMemcpyAsync(Xc, Xmm.p, sizeof(double) * mpad * mpad, hipMemcpyDeviceToDevice, s));
     } else {
       convert<float, double>(reinterpret_cast<float*>(Xc), Xmm.p, (size_t)mpad * mpad, s);
     }
-    const double* pu = params_d.p;
-    hipLaunchKernelGGL((k_unpack_lq<T>), dim3(grid_for(mpad * mpad)), dim3(256), 0, s,
-                       pu + n_theta + m, m, mpad, Lc, Ld.p);
-    fill_cross<T>(dp, prog.p, zp(), xbp, Kzx, mpad, s);
-    fill_diag(dp, prog.p, DevPoints{z.p, mpad, 1}, scal.p + 7, s);
+    GPRX_HIP_CHECK(hipStreamWaitEvent(s, sev[1], 0));
     // A = X Kzx; U = Lq^T A
     gemm_call<T>(false, false, Xc, mpad, Kzx, mpad, A, mpad, M, B, M, 1.0, 0.0, kKhiRow, 0,
                  kOrderColFastRev, s);
@@ -498,6 +512,8 @@ int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, c
     GPRX_HIP_CHECK(hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking));
     M.fstreams.init(M.stream);
     for (auto& e : M.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
+    for (auto& e : M.sev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    M.aux = aux_stream();
     hipStream_t s = M.stream;
     exact_upload_points(M.x, x, (int)n, (int)d, M.npad, M.dp, s);
     exact_upload_points(M.z, z, (int)m, (int)d, M.mpad, M.dp, s);

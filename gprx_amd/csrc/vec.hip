@@ -169,6 +169,7 @@ static hipStream_t masked_stream(int slot) {
 
 hipStream_t bulk_stream() { return masked_stream(0); }
 hipStream_t ahead_stream() { return masked_stream(1); }
+hipStream_t aux_stream() { return masked_stream(2); }
 
 // Lookahead.  A block N of s >= kAheadMin tiles at recursion depth <=
 // kAheadMaxDepth splits L21 by columns at q = s1 / 2: L21's first q columns

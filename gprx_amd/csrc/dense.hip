@@ -246,13 +246,13 @@ static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream, boo
 constexpr int kSmallTileBelow = 256;
 constexpr int kTinyTileBelow = 64;
 // A launch with a triangular operand gives its tiles K-ranges up to the
-// whole K apart, so below two 128-tiles per CU its longest tile sets the
+// whole K apart, so with few 128-tiles per CU its longest tile sets the
 // time: 64-tiles there (a quarter of that tile's work, four times the
 // tiles to balance).  Measured on gfx950: the m = 2048 products X^T H X of
-// an Svgp step 265 -> about 150 us each (step 3.41 -> 3.17 ms mixed, 4.56
-// -> 4.30 ms FP64), the 2048-wide products of the n = 16384 factor (FP64
-// 54.8 -> 53.5 ms).
-constexpr int kTriSmallTileBelow = 512;
+// an Svgp step 265 -> about 150 us each; the step 3.41 -> 3.07 ms mixed and
+// 4.56 -> 4.03 ms FP64 at this bound (3.17 / 4.30 at 512); the n = 16384
+// FP64 factor 54.8 -> 53.5 ms; a bound of 2048 lost 1 ms on that factor.
+constexpr int kTriSmallTileBelow = 1024;
 
 template <typename T>
 static void gemm_tiles(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t stream, bool scheduled) {

@@ -130,7 +130,7 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 // sitting at the top of the recursion.
 //
 // Overlap: in a block of at least kOverlapMin tiles, Tmp (read only by the
-// block's last step) is formed on the bulk stream, which leaves two CUs of
+// block's last step) is formed on the bulk stream, which leaves four CUs of
 // every XCD free, while main applies the update and runs rec(A22), a chain
 // of launches too small to fill the chip.  Measured on gfx950 at n = 16384
 // (factor + inverse, f32 / f64): 38.3 / 66.0 ms, against 41.6 / 71.2 ms on
@@ -153,12 +153,12 @@ static hipStream_t masked_stream(int slot) {
   GPRX_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
   const int ncu = prop.multiProcessorCount;
   hipStream_t st = nullptr;
-  // CU mask bit b selects a CU on XCD b mod 8, so bits 0..15 are two CUs of
-  // every XCD.  Other CU counts get an unrestricted stream.
+  // CU mask bit b selects a CU on XCD b mod 8, so bits 0..31 are four CUs
+  // of every XCD.  Other CU counts get an unrestricted stream.
   if (ncu == 256) {
     uint32_t mask[8];
     for (auto& w : mask) w = 0xffffffffu;
-    mask[0] &= ~((1u << kReservedCus) - 1u);
+    for (int b = 0; b < kReservedCus; ++b) mask[b / 32] &= ~(1u << (b % 32));
     GPRX_HIP_CHECK(hipExtStreamCreateWithCUMask(&st, 8, mask));
   } else {
     GPRX_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));

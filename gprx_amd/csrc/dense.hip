@@ -236,22 +236,23 @@ static void launch_gemm(bool ta, bool tb, GemmArgs<T> g, hipStream_t stream, boo
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
-// 128-tiles while they give every CU at least one workgroup, 64-tiles (four
-// times the workgroups, a quarter of the MFMA work each) below that, and
-// 32-tiles for a launch of only a few 128-tiles: a small product is bound
-// by how many CUs share it, not by per-CU reuse.  Measured on gfx950 at
-// n = 16384 (factor + inverse, f32 / f64): 38.4 / 66.0 ms with 64-tiles
-// alone, 35.6 / 62.9 ms with 32-tiles below 64 128-tiles (below 16: 36.4 /
-// 63.5; below 256: 35.8 / 62.1).
-constexpr int kSmallTileBelow = 256;
-constexpr int kTinyTileBelow = 64;
-// A launch with a triangular operand gives its tiles K-ranges up to the
-// whole K apart, so with few 128-tiles per CU its longest tile sets the
-// time: 64-tiles there (a quarter of that tile's work, four times the
-// tiles to balance).  Measured on gfx950: the m = 2048 products X^T H X of
-// an Svgp step 265 -> about 150 us each; the step 3.41 -> 3.07 ms mixed and
-// 4.56 -> 4.03 ms FP64 at this bound (3.17 / 4.30 at 512); the n = 16384
-// FP64 factor 54.8 -> 53.5 ms; a bound of 2048 lost 1 ms on that factor.
+// Tile size of a launch, by its number of 128-tiles:
+//   - below 256 (fewer than one per CU): 32-tiles, sixteen times the
+//     workgroups, so the product spreads over the chip instead of a few CUs;
+//   - below 1024 with a triangular operand: 64-tiles.  Its tiles' K-ranges
+//     differ by up to the whole K, so with few tiles per CU the longest sets
+//     the time; a 64-tile holds a quarter of that work and four times the
+//     tiles balance;
+//   - otherwise 128-tiles.
+// Measured on gfx950 (A/B runs on one box): 32-tiles up to 256 128-tiles
+// against 64 took the Svgp step (m = 2048, b = 4096) from 3.06 to 2.98 ms
+// mixed and the n = 16384 mixed evaluation from 42.4 to 42.0 ms; the
+// triangular rule took the Svgp step from 3.41 to 3.07 ms (mixed) and 4.56
+// to 4.03 ms (FP64), whose m = 2048 products X^T H X had run 265 us each on
+// 256 128-tiles, and the n = 16384 FP64 factor from 54.8 to 53.5 ms; its
+// bound at 512 gave 3.17 / 4.30 ms on the Svgp step, at 2048 it lost 1 ms
+// on the exact factor.
+constexpr int kTinyTileBelow = 256;
 constexpr int kTriSmallTileBelow = 1024;
 
 template <typename T>
@@ -261,7 +262,7 @@ static void gemm_tiles(bool ta, bool tb, const GemmArgs<T>& g, hipStream_t strea
   if (total == 0) return;
   if (total < kTinyTileBelow)
     launch_gemm<T, 32>(ta, tb, g, stream, scheduled);
-  else if (total < kSmallTileBelow || (g.krange && total < kTriSmallTileBelow))
+  else if (g.krange && total < kTriSmallTileBelow)
     launch_gemm<T, 64>(ta, tb, g, stream, scheduled);
   else
     launch_gemm<T, 128>(ta, tb, g, stream, scheduled);

@@ -100,9 +100,13 @@ __global__ __launch_bounds__(256, kTileOccupancy) void k_gemm(GemmArgs<T> g) {
 // that differ by up to the whole K, so a fixed blockIdx -> tile map that
 // keeps neighbours together (for L2 reuse) puts most of the work on a few
 // XCDs.  The schedule built here:
-//   - groups tiles into S x S super-blocks (S = 8: one XCD's 64 slots at two
-//     workgroups per CU; smaller when the launch has few tiles), so tiles
-//     running together on an XCD share 2S operand panels;
+//   - groups tiles into S x S super-blocks (S = 2; 1 when the launch has
+//     few tiles), so tiles running together on an XCD share 2S operand
+//     panels.  Balance counts for more than that sharing: measured on gfx950
+//     (A/B runs on one box), S = 2 against 8 took the n = 16384 FP64
+//     evaluation from 75.7 to 75.2 ms and the Sgpr FP64 evaluation (n =
+//     2^20, m = 1024) from 106.5 to 102.5 ms; S = 16 lost 1.5 ms on the
+//     exact evaluation, S = 1 tied S = 2 there and lost 1 ms on Sgpr;
 //   - deals super-blocks, heaviest first, each to the least-loaded XCD
 //     (longest-processing-time), which balances the XCDs' total work and
 //     starts the longest tiles first;
@@ -165,7 +169,7 @@ static std::vector<unsigned> build_tile_schedule(int mt, int nt, int lower, int 
       }
     return blocks;
   };
-  int S = 8;
+  int S = 2;
   std::vector<Block> blocks = make_blocks(S);
   while (S > 1 && (int)blocks.size() < 4 * kXcds) {
     S /= 2;

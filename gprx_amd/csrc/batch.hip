@@ -186,8 +186,8 @@ __global__ __launch_bounds__(256) void k_fill_tasks(const FillTask* __restrict__
 // Fused gradient contraction over every problem's lower tiles; W_ij =
 // alpha_i alpha_j - Ainv_ij.  Partials per task (tasks of a problem are
 // consecutive).
-template <int DP, typename T>
-__global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restrict__ tasks,
+template <int DP, typename T, int NS>
+__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_tasks(const FillTask* __restrict__ tasks,
                                                         const ProbDesc* __restrict__ probs,
                                                         const DevProgram* __restrict__ progs,
                                                         const double* __restrict__ xar,
@@ -215,10 +215,10 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
   double xi[DP];
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
-  double acc[kMaxSlots][kAccPerLeaf];
+  double acc[NS][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < NS; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   const T* M = Ainv + pd.moff;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void k_contract_tasks(const FillTask* __restri
       R dx[DP];
 #pragma unroll
       for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
-      LeavesT<R> L;
+      LeavesT<R, NS> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }
@@ -481,6 +481,7 @@ struct BatchModel : gprx_hip_model {
   DevBuf<int> d_fbeg, d_fend;
   struct DimGroup {
     int dp, off, count;
+    int ns;  // most leaf slots of the group's programs
   };
   std::vector<DimGroup> dp_groups;
   DevBuf<double> mvpart;        // tile matrix-vector partials, 4 * 128 per lower tile
@@ -595,7 +596,8 @@ struct BatchModel : gprx_hip_model {
     dp_groups.clear();
     for (int p : order) {
       const int pd = padded_dim(hp[p].d);
-      if (dp_groups.empty() || dp_groups.back().dp != pd) dp_groups.push_back(DimGroup{pd, (int)ft.size(), 0});
+      if (dp_groups.empty() || dp_groups.back().dp != pd) dp_groups.push_back(DimGroup{pd, (int)ft.size(), 0, 0});
+      dp_groups.back().ns = std::max(dp_groups.back().ns, hp[p].hp.ns);
       fbeg[p] = (int)ft.size();
       for (int i = 0; i < hp[p].nt; ++i)
         for (int j = 0; j <= i; ++j) ft.push_back(FillTask{p, i, j});
@@ -826,10 +828,17 @@ struct BatchModel : gprx_hip_model {
   template <typename T>
   void contract(const T* Ainv) {
     for (const DimGroup& g : dp_groups) {
-#define GPRX_CON(DPV)                                                                              \
-  hipLaunchKernelGGL((k_contract_tasks<DPV, T>), dim3(g.count), dim3(256), 0, stream,               \
+#define GPRX_CON_NS(DPV, NSV)                                                                      \
+  hipLaunchKernelGGL((k_contract_tasks<DPV, T, NSV>), dim3(g.count), dim3(256), 0, stream,          \
                      fill_tasks.p + g.off, probs.p, progs.p, xar.p, Ainv, alpha.p,                 \
                      partial.p + (long)g.off * kMaxAcc)
+#define GPRX_CON(DPV)                                                                              \
+  if (g.ns <= 2)                                                                                   \
+    GPRX_CON_NS(DPV, 2);                                                                           \
+  else if (g.ns <= 4)                                                                              \
+    GPRX_CON_NS(DPV, 4);                                                                           \
+  else                                                                                             \
+    GPRX_CON_NS(DPV, kMaxSlots)
       switch (g.dp) {
         case 1: GPRX_CON(1); break;
         case 2: GPRX_CON(2); break;
@@ -839,6 +848,7 @@ struct BatchModel : gprx_hip_model {
         default: GPRX_CON(32); break;
       }
 #undef GPRX_CON
+#undef GPRX_CON_NS
     }
     hipLaunchKernelGGL(k_reduce_by_problem, dim3(K * kMaxAcc), dim3(256), 0, stream, partial.p,
                        d_fbeg.p, d_fend.p, accs.p);

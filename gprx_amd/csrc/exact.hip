@@ -122,7 +122,7 @@ struct ExactModel : gprx_hip_model {
       gemm_call<double>(true, false, X64.p, npad, X64.p, npad, A64.p, npad, (int)npad, (int)npad,
                         (int)npad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
       GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
-      contract_exact<double>(dp, prog.p, pts(), A64.p, npad, alpha.p, partials.p, s);
+      contract_exact<double>(dp, hp.ns, prog.p, pts(), A64.p, npad, alpha.p, partials.p, s);
       reduce_partials(partials.p, lower_tiles(npad), accs.p, s);
       noise_trace<double>(alpha.p, A64.p, npad, n, scal.p + 2, s);
     } else {
@@ -186,7 +186,7 @@ struct ExactModel : gprx_hip_model {
       gemm_call<float>(true, false, X32.p, npad, X32.p, npad, A32.p, npad, (int)npad, (int)npad,
                        (int)npad, 1.0, 0.0, kKloRow, 1, kOrderRowFast, s);
       GPRX_HIP_CHECK(hipEventRecord(ev[4], s));
-      contract_exact<float>(dp, prog.p, pts(), A32.p, npad, alpha.p, partials.p, s);
+      contract_exact<float>(dp, hp.ns, prog.p, pts(), A32.p, npad, alpha.p, partials.p, s);
       reduce_partials(partials.p, lower_tiles(npad), accs.p, s);
       noise_trace<float>(alpha.p, A32.p, npad, n, scal.p + 2, s);
     } else {

@@ -297,8 +297,8 @@ __global__ void k_fill_diag(const DevProgram* __restrict__ Pg, int na, double* _
   out[i] = eval_entry<DP, false>(P, dx, L);
 }
 
-template <int DP, typename T>
-__global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __restrict__ Pg,
+template <int DP, typename T, int NS>
+__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_exact(const DevProgram* __restrict__ Pg,
                                                         const double* __restrict__ x, long ldx,
                                                         int n, const T* __restrict__ Ainv, long lda,
                                                         const double* __restrict__ alpha,
@@ -325,10 +325,10 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
   const double ai = ar[i];
-  double acc[kMaxSlots][kAccPerLeaf];
+  double acc[NS][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < NS; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   if (gi < n) {
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
       R dx[DP];
 #pragma unroll
       for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
-      LeavesT<R> L;
+      LeavesT<R, NS> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }
@@ -349,8 +349,8 @@ __global__ __launch_bounds__(256) void k_contract_exact(const DevProgram* __rest
   store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);
 }
 
-template <int DP, typename T>
-__global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __restrict__ Pg,
+template <int DP, typename T, int NS>
+__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_cross(const DevProgram* __restrict__ Pg,
                                                         const double* __restrict__ a, long lda,
                                                         int na, const double* __restrict__ b,
                                                         long ldb, int nb, const T* __restrict__ W,
@@ -373,10 +373,10 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
   double xi[DP];
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
-  double acc[kMaxSlots][kAccPerLeaf];
+  double acc[NS][kAccPerLeaf];
   double accd[DP];
 #pragma unroll
-  for (int l = 0; l < kMaxSlots; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < NS; ++l) acc[l][0] = acc[l][1] = 0.0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) accd[q] = 0.0;
   if (gi < na) {
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_contract_cross(const DevProgram* __rest
       R dx[DP];
 #pragma unroll
       for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
-      LeavesT<R> L;
+      LeavesT<R, NS> L;
       eval_entry<DP, true>(P, dx, L);
       accumulate_entry<DP>(P, dx, L, w, acc, accd);
     }
@@ -435,6 +435,20 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
 // ---------------------------------------------------------------------------
 // Host launchers: dispatch on the padded dimension.
 // ---------------------------------------------------------------------------
+// The slot count a kernel is compiled for: the program's ns rounded up to
+// 2, 4 or 8.
+#define GPRX_DISPATCH_NS(ns, CALL)                      \
+  if ((ns) <= 2) {                                      \
+    constexpr int NS = 2;                               \
+    CALL;                                               \
+  } else if ((ns) <= 4) {                               \
+    constexpr int NS = 4;                               \
+    CALL;                                               \
+  } else {                                              \
+    constexpr int NS = kMaxSlots;                       \
+    CALL;                                               \
+  }
+
 #define GPRX_DISPATCH_DIM(dp, CALL) \
   switch (dp) {                     \
     case 1: {                       \
@@ -497,24 +511,27 @@ void fill_diag(int dp, const DevProgram* prog, DevPoints a, double* out, hipStre
 }
 
 template <typename T>
-void contract_exact(int dp, const DevProgram* prog, DevPoints x, const T* Ainv, long lda,
+void contract_exact(int dp, int ns, const DevProgram* prog, DevPoints x, const T* Ainv, long lda,
                     const double* alpha, double* partial, hipStream_t stream) {
   const int tiles = lower_tiles(x.ld);
-  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_exact<DP, T>), dim3(tiles), dim3(256), 0,
-                                           stream, prog, x.p, x.ld, x.rows, Ainv, lda, alpha,
-                                           partial));
+  GPRX_DISPATCH_DIM(dp, GPRX_DISPATCH_NS(ns, hipLaunchKernelGGL((k_contract_exact<DP, T, NS>),
+                                                                dim3(tiles), dim3(256), 0, stream,
+                                                                prog, x.p, x.ld, x.rows, Ainv, lda,
+                                                                alpha, partial)));
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void contract_cross(int dp, const DevProgram* prog, DevPoints a, DevPoints b, const T* W, long ldw,
-                    double* partial, hipStream_t stream, const double* cv, const double* rv,
+void contract_cross(int dp, int ns, const DevProgram* prog, DevPoints a, DevPoints b, const T* W,
+                    long ldw, double* partial, hipStream_t stream, const double* cv, const double* rv,
                     double crs) {
   const int mt = (int)(a.ld / kTile);
   const int tiles = cross_tiles(a.ld, b.ld);
-  GPRX_DISPATCH_DIM(dp, hipLaunchKernelGGL((k_contract_cross<DP, T>), dim3(tiles), dim3(256), 0,
-                                           stream, prog, a.p, a.ld, a.rows, b.p, b.ld, b.rows, W,
-                                           ldw, mt, partial, cv, rv, crs));
+  GPRX_DISPATCH_DIM(dp, GPRX_DISPATCH_NS(ns, hipLaunchKernelGGL((k_contract_cross<DP, T, NS>),
+                                                                dim3(tiles), dim3(256), 0, stream,
+                                                                prog, a.p, a.ld, a.rows, b.p, b.ld,
+                                                                b.rows, W, ldw, mt, partial, cv, rv,
+                                                                crs)));
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
@@ -538,14 +555,14 @@ template void fill_cross<double>(int, const DevProgram*, DevPoints, DevPoints, d
                                  hipStream_t);
 template void fill_cross<float>(int, const DevProgram*, DevPoints, DevPoints, float*, long,
                                 hipStream_t);
-template void contract_exact<double>(int, const DevProgram*, DevPoints, const double*, long,
+template void contract_exact<double>(int, int, const DevProgram*, DevPoints, const double*, long,
                                      const double*, double*, hipStream_t);
-template void contract_exact<float>(int, const DevProgram*, DevPoints, const float*, long,
+template void contract_exact<float>(int, int, const DevProgram*, DevPoints, const float*, long,
                                     const double*, double*, hipStream_t);
-template void contract_cross<double>(int, const DevProgram*, DevPoints, DevPoints, const double*,
+template void contract_cross<double>(int, int, const DevProgram*, DevPoints, DevPoints, const double*,
                                      long, double*, hipStream_t, const double*, const double*,
                                      double);
-template void contract_cross<float>(int, const DevProgram*, DevPoints, DevPoints, const float*,
+template void contract_cross<float>(int, int, const DevProgram*, DevPoints, DevPoints, const float*,
                                     long, double*, hipStream_t, const double*, const double*,
                                     double);
 

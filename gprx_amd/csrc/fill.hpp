@@ -33,17 +33,18 @@ void fill_diag(int dim_pad, const DevProgram* prog, DevPoints a, double* out, hi
 // W_ij = alpha_i alpha_j - Ainv_ij (counted twice off the diagonal), each
 // tile writes its kMaxAcc accumulators (leaf slots, then ARD dimensions) to
 // partial[tile * kMaxAcc + k].
+// ns: the program's slot count (kernels are built for 2, 4 or 8 slots).
 template <typename T>
-void contract_exact(int dim_pad, const DevProgram* prog, DevPoints x, const T* Ainv, long lda,
-                    const double* alpha, double* partial, hipStream_t stream);
+void contract_exact(int dim_pad, int ns, const DevProgram* prog, DevPoints x, const T* Ainv,
+                    long lda, const double* alpha, double* partial, hipStream_t stream);
 
 // Contraction of a dense weight matrix W (a.rows x b.rows, f64 or f32, ld
 // ldw) against dK(a, b): partial per tile as above.  Tiles are full
 // (a.ld / 128) x (b.ld / 128).  With cv, the weight is W_ij - crs cv_i rv_j.
 template <typename T>
-void contract_cross(int dim_pad, const DevProgram* prog, DevPoints a, DevPoints b, const T* W,
-                    long ldw, double* partial, hipStream_t stream, const double* cv = nullptr,
-                    const double* rv = nullptr, double crs = 0.0);
+void contract_cross(int dim_pad, int ns, const DevProgram* prog, DevPoints a, DevPoints b,
+                    const T* W, long ldw, double* partial, hipStream_t stream,
+                    const double* cv = nullptr, const double* rv = nullptr, double crs = 0.0);
 
 // Contraction of a constant weight w against dk(a_i, a_i): one partial row.
 void contract_diag(int dim_pad, const DevProgram* prog, DevPoints a, double w, double* partial,

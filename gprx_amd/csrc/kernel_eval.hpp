@@ -12,11 +12,12 @@ namespace gprx {
 // ---------------------------------------------------------------------------
 // Per-entry arithmetic in R (double, or float for the mixed form's f32
 // matrices): leaf features and term products; accumulation stays f64.
-template <typename R>
+// NS: slots held (at least the program's ns); fewer slots, fewer registers.
+template <typename R, int NS = kMaxSlots>
 struct LeavesT {
-  R f0[kMaxSlots];  // d v_s / d log theta = v_s * (feature) per slot
-  R f1[kMaxSlots];
-  R tp[kMaxSlots];  // product of the slot's term up to and including it
+  R f0[NS];  // d v_s / d log theta = v_s * (feature) per slot
+  R f1[NS];
+  R tp[NS];  // product of the slot's term up to and including it
 };
 using Leaves = LeavesT<double>;
 
@@ -48,8 +49,8 @@ __device__ __forceinline__ float prog_w<float>(const DevProgram& P, int q) { ret
 
 // K at one entry (dx = a - b over the padded dimensions): leaf values slot by
 // slot, the running product of the open term, and the sum over closed terms.
-template <int DP, bool GRAD, typename R>
-__device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], LeavesT<R>& L) {
+template <int DP, bool GRAD, typename R, int NS>
+__device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], LeavesT<R, NS>& L) {
   R s = 0, r2 = 0;
 #pragma unroll
   for (int q = 0; q < DP; ++q) {
@@ -59,7 +60,7 @@ __device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], 
   }
   R k = 0, rp = 1;
 #pragma unroll
-  for (int sl = 0; sl < kMaxSlots; ++sl) {
+  for (int sl = 0; sl < NS; ++sl) {
     if (sl >= P.ns) break;
     const int kd = P.kind[sl];
     R v, f0 = 0, f1 = 0;
@@ -94,14 +95,14 @@ __device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], 
 // Adds w * dK/d(slot features) of one entry to the (f64) accumulators:
 // sweeping the slots backwards, each term's full product (held by its last
 // slot) is carried to the slots before it.
-template <int DP, typename R>
+template <int DP, typename R, int NS>
 __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&dx)[DP],
-                                                 const LeavesT<R>& L, double w,
-                                                 double (&acc)[kMaxSlots][kAccPerLeaf],
+                                                 const LeavesT<R, NS>& L, double w,
+                                                 double (&acc)[NS][kAccPerLeaf],
                                                  double (&accd)[DP]) {
   double gard = 0.0, term = 0.0;
 #pragma unroll
-  for (int sl = kMaxSlots - 1; sl >= 0; --sl) {
+  for (int sl = NS - 1; sl >= 0; --sl) {
     if (sl < P.ns) {
       if ((P.end >> sl) & 1u) term = (double)L.tp[sl];
       const double g = w * term;
@@ -116,9 +117,17 @@ __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&
   }
 }
 
-// Block reduction of the accumulators into partial[tile * kMaxAcc + k].
-template <int DP>
-__device__ __forceinline__ void store_partials(double (&acc)[kMaxSlots][kAccPerLeaf],
+// Workgroups per CU the contraction kernels are compiled for: three (at
+// most 168 registers) up to 8 padded dimensions, where it took the Sgpr
+// FP64 evaluation (n = 2^20, m = 1024, three leaf slots) from 101.8 to 96.9
+// ms; at 16 dimensions (the Svgp step) the bound cost 4%, and four
+// workgroups spilled.
+constexpr int contract_occupancy(int dp) { return dp <= 8 ? 3 : 1; }
+
+// Block reduction of the accumulators into partial[tile * kMaxAcc + k]
+// (slots from NS on hold zeros).
+template <int DP, int NS>
+__device__ __forceinline__ void store_partials(double (&acc)[NS][kAccPerLeaf],
                                                double (&accd)[DP], double* red /* [4][kMaxAcc] */,
                                                double* out) {
   const int lane = threadIdx.x & 63, wave = wave_index();
@@ -126,7 +135,7 @@ __device__ __forceinline__ void store_partials(double (&acc)[kMaxSlots][kAccPerL
   for (int l = 0; l < kMaxSlots; ++l)
 #pragma unroll
     for (int c = 0; c < kAccPerLeaf; ++c) {
-      const double v = wave_sum(acc[l][c]);
+      const double v = l < NS ? wave_sum(acc[l < NS ? l : 0][c]) : 0.0;
       if (lane == 0) red[wave * kMaxAcc + l * kAccPerLeaf + c] = v;
     }
 #pragma unroll

@@ -216,11 +216,11 @@ struct SgprModel : gprx_hip_model {
         convert<float, double>(Q32.p, Q64.p, (size_t)mpad * mpad, s);
         gemm_call<float>(false, false, Q32.p, mpad, A32.p, mpad, U32.p, mpad, M, N, M, 1.0, 0.0, 0,
                          0, kOrderRowFast, s);
-        contract_cross<float>(dp, prog.p, zp(), xp(), U32.p, mpad, partials.p, s, v.p, r.p, 1.0 / s2);
+        contract_cross<float>(dp, hp.ns, prog.p, zp(), xp(), U32.p, mpad, partials.p, s, v.p, r.p, 1.0 / s2);
       } else {
         gemm_call<double>(false, false, Q64.p, mpad, A64.p, mpad, U64.p, mpad, M, N, M, 1.0, 0.0, 0,
                           0, kOrderRowFast, s);
-        contract_cross<double>(dp, prog.p, zp(), xp(), U64.p, mpad, partials.p, s, v.p, r.p,
+        contract_cross<double>(dp, hp.ns, prog.p, zp(), xp(), U64.p, mpad, partials.p, s, v.p, r.p,
                                1.0 / s2);
       }
       reduce_partials(partials.p, tiles1, accs.p, s);
@@ -234,7 +234,7 @@ struct SgprModel : gprx_hip_model {
       gemm_call<double>(false, false, T1.p, mpad, Xmm.p, mpad, T2.p, mpad, M, M, M, 1.0, 0.0,
                         kKloCol, 0, kOrderRowFast, s);
       neg_sym(T2.p, mpad, M, Wmm.p, s);
-      contract_cross<double>(dp, prog.p, zp(), zp(), Wmm.p, mpad, partials2.p, s);
+      contract_cross<double>(dp, hp.ns, prog.p, zp(), zp(), Wmm.p, mpad, partials2.p, s);
       reduce_partials(partials2.p, cross_tiles(mpad, mpad), accs.p + kMaxAcc, s);
       contract_diag(dp, prog.p, xp(), 0.5 / s2, accs.p + 2 * kMaxAcc, s);
     }

@@ -418,9 +418,9 @@ struct SvgpModel : gprx_hip_model {
                       0, kOrderRowFast, s);
     hipLaunchKernelGGL(k_neg_sym, dim3(grid_for(mpad * mpad)), dim3(256), 0, s, T2.p, mpad, (int)mpad,
                        Wmm.p);
-    contract_cross<double>(dp, prog.p, zp(), zp(), Wmm.p, mpad, partials.p, s);
+    contract_cross<double>(dp, hp.ns, prog.p, zp(), zp(), Wmm.p, mpad, partials.p, s);
     reduce_partials(partials.p, cross_tiles(mpad, mpad), accs.p, s);
-    contract_cross<T>(dp, prog.p, zp(), xbp, Wmn, mpad, partials2.p, s);
+    contract_cross<T>(dp, hp.ns, prog.p, zp(), xbp, Wmn, mpad, partials2.p, s);
     reduce_partials(partials2.p, cross_tiles(mpad, bp), accs.p + kMaxAcc, s);
     contract_diag(dp, prog.p, xbp, -0.5 / s2, accs.p + 2 * kMaxAcc, s);
     GPRX_HIP_CHECK(hipEventRecord(ev[1], s));

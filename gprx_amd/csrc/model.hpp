@@ -85,9 +85,6 @@ constexpr int kReservedCus = 32;
 hipStream_t bulk_stream();
 // Shared lookahead stream of the current device (created on first use).
 hipStream_t ahead_stream();
-// Shared stream, masked like bulk, for work that runs beside a factor and
-// does not depend on it (created on first use).
-hipStream_t aux_stream();
 struct FactorStreams {
   hipStream_t main = nullptr, bulk = nullptr, ahead = nullptr;  // bulk, ahead shared
   hipEvent_t ev[2 * kMaxDepth] = {};

@@ -255,7 +255,10 @@ struct SvgpModel : gprx_hip_model {
   // sev[0]: the step's program and batch indices are on the device; sev[1]:
   // the factor-independent start of the step (on aux) is done
   hipEvent_t sev[2] = {nullptr, nullptr};
-  hipStream_t aux = nullptr;  // shared, not owned
+  // the factor's lookahead stream (shared, not owned): the step's
+  // factor-independent start runs there, so the process holds no further
+  // CU-masked queue
+  hipStream_t aux = nullptr;
   DevBuf<double> x, y, z;
   DevBuf<DevProgram> prog;
   HostBuf<DevProgram> hprog;
@@ -513,7 +516,7 @@ int gprx_hip_svgp_create(const double* x, size_t n, size_t d, const double* y, c
     M.fstreams.init(M.stream);
     for (auto& e : M.ev) GPRX_HIP_CHECK(hipEventCreate(&e));
     for (auto& e : M.sev) GPRX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    M.aux = aux_stream();
+    M.aux = ahead_stream();
     hipStream_t s = M.stream;
     exact_upload_points(M.x, x, (int)n, (int)d, M.npad, M.dp, s);
     exact_upload_points(M.z, z, (int)m, (int)d, M.mpad, M.dp, s);

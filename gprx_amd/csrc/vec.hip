@@ -140,7 +140,11 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 // ---------------------------------------------------------------------------
 constexpr int kOverlapMin = 8;
 
-// A stream per (device, slot) restricted to all CUs but kReservedCus.
+// A stream per (device, slot) restricted to all CUs but kReservedCus.  A
+// process keeps few of these: each holds a hardware queue, and with the
+// models' own streams beside them more queues than the device maps at once
+// are time-sliced (a third masked stream slowed every kernel of a second
+// model in the same process by 5-150%).
 static hipStream_t masked_stream(int slot) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, hipStream_t> streams;
@@ -169,7 +173,6 @@ static hipStream_t masked_stream(int slot) {
 
 hipStream_t bulk_stream() { return masked_stream(0); }
 hipStream_t ahead_stream() { return masked_stream(1); }
-hipStream_t aux_stream() { return masked_stream(2); }
 
 // Lookahead.  A block N of s >= kAheadMin tiles at recursion depth <=
 // kAheadMaxDepth splits L21 by columns at q = s1 / 2: L21's first q columns

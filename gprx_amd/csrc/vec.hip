@@ -141,10 +141,10 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 constexpr int kOverlapMin = 8;
 
 // A stream per (device, slot) restricted to all CUs but kReservedCus.  A
-// process keeps few of these: each holds a hardware queue, and with the
-// models' own streams beside them more queues than the device maps at once
-// are time-sliced (a third masked stream slowed every kernel of a second
-// model in the same process by 5-150%).
+// process keeps two of these: each holds a hardware queue, and with a third
+// one beside two models' own streams every kernel of the second model ran
+// 5-150% slower (measured; most likely the queues beyond those the device
+// maps at once being time-sliced).
 static hipStream_t masked_stream(int slot) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, hipStream_t> streams;

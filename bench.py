@@ -45,10 +45,11 @@ PEAK_HBM = 8000.0  # GB/s
 # HBM-side bytes per X^T X launch at n = 16384: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on that launch, each scaled by what the counter reads for a 1 GiB
 # device copy (x2.0 and x1.0); the run is summarised under profiles/.
-XTX_TRAFFIC_N16384 = {"mixed": 2.957e+10, "fp64": 6.705e+10}
+XTX_TRAFFIC_N16384 = {"mixed": 3.167e+10, "fp64": 6.639e+10}
 # HBM-side bytes of one whole evaluation at n = 16384: FETCH_SIZE + WRITE_SIZE
-# summed over the evaluation's dispatches, scaled as above.
-EVAL_TRAFFIC_N16384 = {"fp64": 1.745e11}
+# summed over the evaluation's dispatches, scaled as above (the model's
+# one-time buffer clears left out).
+EVAL_TRAFFIC_N16384 = {"fp64": 1.557e11}
 
 
 def exact_flops(n: int, d: int) -> float:
@@ -210,8 +211,8 @@ def bench_exact(args, gx):
         "algorithmic_flop_per_eval": F,
         "roofline": {"bound": "mfma",
                      "kernel": "whole FP64 evaluation, HIP events on its stream (by rocprof total time "
-                               "its dominant kernel is k_gemm<double,N,N,128>, the factor's "
-                               "trailing and inverse products)",
+                               "its dominant kernel is k_gemm<double,N,T,128>, the factor's "
+                               "L21 and trailing-update products, on three streams)",
                      "achieved": eval64, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": eval64 / PEAK_FP64,
                      "traffic": EVAL_TRAFFIC_N16384.get("fp64") if wl.n == 16384 else None},
         "roofline_xtx": {"bound": "mfma", "kernel": "k_gemm<double,T,N,128> (X^T X, one launch per eval)",

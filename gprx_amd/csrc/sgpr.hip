@@ -378,10 +378,12 @@ int gprx_hip_sgpr_create(const double* x, size_t n, size_t d, const double* y, c
     M.mt = (int)(M.mpad / kTile);
     M.ldt = (long)((M.mt + 1) / 2) * kTile;
     // Split the n-long reduction of U U^T into chunks so the launch holds
-    // about 2048 workgroups; pad n to a whole number of chunks.
+    // at most 2048 workgroups (four per slot at two per CU: a launch just
+    // above a multiple of 512 ends on a wave of a few); pad n to a whole
+    // number of chunks.
     const long ntiles = (long)round_up(n, kTile) / kTile;
     const long lower = (long)M.mt * (M.mt + 1) / 2;
-    long want = std::max<long>(1, std::min<long>(ntiles, (2048 + lower - 1) / lower));
+    long want = std::max<long>(1, std::min<long>(ntiles, 2048 / lower));
     M.kc = ((ntiles + want - 1) / want) * kTile;
     M.npad = (long)round_up(n, (size_t)M.kc);
     M.nchunks = (int)(M.npad / M.kc);

@@ -384,7 +384,12 @@ __device__ __forceinline__ void tile_krange(int K, int krange, TileCoord& t, int
 // dense.hip); false for a padding slot.
 __device__ __forceinline__ bool tile_from_schedule(const unsigned* sched, int K, int krange,
                                                    TileCoord& t, int tt = kTile) {
-  const unsigned v = sched[blockIdx.x];
+  // Slot 8k + x holds XCD x's k-th tile.  In a batched launch (blockIdx.y =
+  // problem) problem z takes XCD (x + z) mod 8's list, so the XCDs whose
+  // lists hold a tile more take turns across the batch.
+  const unsigned slot = gridDim.y > 1 ? ((blockIdx.x & ~7u) | ((blockIdx.x + blockIdx.y) & 7u))
+                                      : blockIdx.x;
+  const unsigned v = sched[slot];
   if (v == 0xffffffffu) return false;
   t.ti = (int)(v >> 16);
   t.tj = (int)(v & 0xffffu);

@@ -64,9 +64,13 @@ def _problem(n, d, seed, kname):
     return x, y, specs[kname]
 
 
+# n = 4700 pads to 37 tiles of 128: the recursive factor splits it 18 + 19 and
+# the 19 again 9 + 10, so the column-split lookahead (blocks of 16 tiles and
+# more, two recursion levels) runs on uneven halves at both levels (the oracle
+# takes about 15 s there).
 CASES = [(1, 1, "rbf"), (7, 2, "const_ard"), (128, 3, "const_ard"), (129, 4, "const_ard_plus_periodic"),
          (300, 8, "const_ard"), (257, 1, "sum_times"), (1000, 8, "const_ard"), (700, 16, "const_ard"),
-         (2500, 8, "const_ard")]
+         (2500, 8, "const_ard"), (4700, 8, "const_ard")]
 
 
 @pytest.mark.parametrize("n,d,kname", CASES)

@@ -313,16 +313,22 @@ __global__ __launch_bounds__(256) void k_potrf_diag_list(T* A, T* X, const long*
 template <typename T>
 constexpr size_t kDiagLds = diag::kLdsBytes;
 
+// The dynamic-LDS limit is raised once per device (the kernels need more
+// than the default 64 KiB); callers on several host threads share the flags.
 template <typename T>
 static void set_diag_lds_once() {
-  static bool done = false;
-  if (!done) {
-    GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
-    GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag_list<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
-    done = true;
-  }
+  static std::mutex mu;
+  static std::vector<char> done;
+  int dev = 0;
+  GPRX_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)done.size() <= dev) done.resize(dev + 1, 0);
+  if (done[dev]) return;
+  GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
+  GPRX_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_potrf_diag_list<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDiagLds<T>));
+  done[dev] = 1;
 }
 
 void prepare_dense_kernels() {

@@ -40,6 +40,32 @@ def test_batch_vs_oracle(gpu, prec, tol):
             assert_close(grads[i], og, 1e-2 if prec else 1e-9, f"grad {i} rep {rep}")
 
 
+EDGE = [(1, 1), (2, 3), (127, 5), (128, 8), (129, 16), (300, 2), (256, 1)]
+
+
+@pytest.mark.parametrize("prec,tol", [(g.Precision.FP64, 1e-10), (g.Precision.MIXED, 1e-4)])
+def test_batch_edge_sizes_vs_oracle(gpu, prec, tol):
+    """One and two points, a tile less one, exactly one and two tiles, one
+    tile and a point, beside a larger problem; every padded dimension group
+    (d = 1, 2, 3, 5, 8, 16)."""
+    rng = np.random.default_rng(21)
+    probs = []
+    for n, d in EDGE:
+        x = rng.random(n * d)
+        y = np.sin(4 * x[:n]) + 0.1 * rng.standard_normal(n)
+        ls = list(0.4 + 0.6 * rng.random(d))
+        probs.append((g.ConstantKernel(1.2) * g.RbfArdKernel(ls), g.GaussianLikelihood(0.02), x, n, d, y))
+    print(f"[batch edge] sizes={EDGE} prec={prec}", flush=True)
+    b = g.GprBatch(probs, precision=prec)
+    thetas = [np.concatenate([p[0].get_params(), [p[1].log_param()]]) for p in probs]
+    nlml, grads, errs = b.value_and_gradient(thetas)
+    for i, (spec, lik, x, n, d, y) in enumerate(probs):
+        assert errs[i] is None, errs[i]
+        ov, og, _, _ = oracle.exact_value_grad(spec.program(), thetas[i], x, n, d, y)
+        assert_close(nlml[i], ov, tol, f"nlml n={n} d={d}")
+        assert_close(grads[i], og, 1e-2 if prec else 1e-9, f"grad n={n} d={d}")
+
+
 def test_batch_matches_single_path(gpu):
     probs = _problems(k=3, seed=12)
     b = g.GprBatch(probs)

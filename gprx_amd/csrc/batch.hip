@@ -215,12 +215,15 @@ __global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_tasks(
   double xi[DP];
 #pragma unroll
   for (int q = 0; q < DP; ++q) xi[q] = xr[q][i];
-  double acc[NS][kAccPerLeaf];
-  double accd[DP];
+  // per-thread sums in the matrices' type (f32 sums of a thread's at most
+  // 64 f32 entries, widened to f64 at the block reduction)
+  using A = T;
+  A acc[NS][kAccPerLeaf];
+  A accd[DP];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) acc[l][0] = acc[l][1] = 0.0;
+  for (int l = 0; l < NS; ++l) acc[l][0] = acc[l][1] = 0;
 #pragma unroll
-  for (int q = 0; q < DP; ++q) accd[q] = 0.0;
+  for (int q = 0; q < DP; ++q) accd[q] = 0;
   const T* M = Ainv + pd.moff;
   if (gi < pd.n) {
     for (int jj = 0; jj < 64; ++jj) {
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_tasks(
       for (int q = 0; q < DP; ++q) dx[q] = (R)(xi[q] - xc[q][j]);
       LeavesT<R, NS> L;
       eval_entry<DP, true>(P, dx, L);
-      accumulate_entry<DP>(P, dx, L, w, acc, accd);
+      accumulate_entry<DP>(P, dx, L, (A)w, acc, accd);
     }
   }
   store_partials<DP>(acc, accd, red, partial + (long)blockIdx.x * kMaxAcc);

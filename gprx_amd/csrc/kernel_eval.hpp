@@ -92,28 +92,28 @@ __device__ __forceinline__ R eval_entry(const DevProgram& P, const R (&dx)[DP], 
   return k;
 }
 
-// Adds w * dK/d(slot features) of one entry to the (f64) accumulators:
-// sweeping the slots backwards, each term's full product (held by its last
-// slot) is carried to the slots before it.
-template <int DP, typename R, int NS>
+// Adds w * dK/d(slot features) of one entry to the accumulators (type A:
+// f64, or f32 for a thread's own short run of f32 entries): sweeping the
+// slots backwards, each term's full product (held by its last slot) is
+// carried to the slots before it.
+template <int DP, typename R, int NS, typename A>
 __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&dx)[DP],
-                                                 const LeavesT<R, NS>& L, double w,
-                                                 double (&acc)[NS][kAccPerLeaf],
-                                                 double (&accd)[DP]) {
-  double gard = 0.0, term = 0.0;
+                                                 const LeavesT<R, NS>& L, A w,
+                                                 A (&acc)[NS][kAccPerLeaf], A (&accd)[DP]) {
+  A gard = 0, term = 0;
 #pragma unroll
   for (int sl = NS - 1; sl >= 0; --sl) {
     if (sl < P.ns) {
-      if ((P.end >> sl) & 1u) term = (double)L.tp[sl];
-      const double g = w * term;
-      acc[sl][0] += g * (double)L.f0[sl];
-      acc[sl][1] += g * (double)L.f1[sl];
+      if ((P.end >> sl) & 1u) term = (A)L.tp[sl];
+      const A g = w * term;
+      acc[sl][0] += g * (A)L.f0[sl];
+      acc[sl][1] += g * (A)L.f1[sl];
       if ((P.ard >> sl) & 1u) gard += g;
     }
   }
   if (P.ard) {
 #pragma unroll
-    for (int q = 0; q < DP; ++q) accd[q] += gard * (double)(dx[q] * dx[q]);
+    for (int q = 0; q < DP; ++q) accd[q] += gard * (A)(dx[q] * dx[q]);
   }
 }
 
@@ -124,24 +124,23 @@ __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&
 // workgroups spilled.
 constexpr int contract_occupancy(int dp) { return dp <= 8 ? 3 : 1; }
 
-// Block reduction of the accumulators into partial[tile * kMaxAcc + k]
-// (slots from NS on hold zeros).
-template <int DP, int NS>
-__device__ __forceinline__ void store_partials(double (&acc)[NS][kAccPerLeaf],
-                                               double (&accd)[DP], double* red /* [4][kMaxAcc] */,
-                                               double* out) {
+// Block reduction of the accumulators (widened to f64) into
+// partial[tile * kMaxAcc + k] (slots from NS on hold zeros).
+template <int DP, int NS, typename A>
+__device__ __forceinline__ void store_partials(A (&acc)[NS][kAccPerLeaf], A (&accd)[DP],
+                                               double* red /* [4][kMaxAcc] */, double* out) {
   const int lane = threadIdx.x & 63, wave = wave_index();
 #pragma unroll
   for (int l = 0; l < kMaxSlots; ++l)
 #pragma unroll
     for (int c = 0; c < kAccPerLeaf; ++c) {
-      const double v = l < NS ? wave_sum(acc[l < NS ? l : 0][c]) : 0.0;
+      const double v = l < NS ? wave_sum((double)acc[l < NS ? l : 0][c]) : 0.0;
       if (lane == 0) red[wave * kMaxAcc + l * kAccPerLeaf + c] = v;
     }
 #pragma unroll
   for (int q = 0; q < kMaxDim; ++q) {
     double v = 0.0;
-    if (q < DP) v = wave_sum(accd[q < DP ? q : 0]);
+    if (q < DP) v = wave_sum((double)accd[q < DP ? q : 0]);
     if (lane == 0) red[wave * kMaxAcc + kMaxSlots * kAccPerLeaf + q] = v;
   }
   __syncthreads();

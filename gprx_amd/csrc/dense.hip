@@ -377,9 +377,34 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_n_part(const T* __restrict__ M, long ld, int rows,
                                                      int cols, int chunk, const double* __restrict__ v,
                                                      double* __restrict__ part, int tri, int strict,
-                                                     int abs_mode) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+                                                     int abs_mode, int vec) {
   const int c0 = blockIdx.y * chunk;
+  if (vec) {
+    // RV consecutive rows per thread in one 16-byte load per column (f32:
+    // four rows; 4-byte loads had left the f32 product at 3.5 TB/s); each
+    // row's sum runs over the same columns in the same order
+    constexpr int RV = 16 / (int)sizeof(T);
+    using V = typename Vec<T>::V;
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * RV;
+    if (i0 >= rows) return;
+    const int c1 = min(cols, c0 + chunk);
+    double acc[RV];
+#pragma unroll
+    for (int e = 0; e < RV; ++e) acc[e] = 0.0;
+    for (int c = c0; c < c1; ++c) {
+      const V mv = *reinterpret_cast<const V*>(M + i0 + (long)c * ld);
+      const double vc = v[c];
+#pragma unroll
+      for (int e = 0; e < RV; ++e) {
+        const double m = (double)mv[e];
+        acc[e] += (abs_mode ? fabs(m) : m) * vc;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < RV; ++e) part[(long)blockIdx.y * rows + i0 + e] = acc[e];
+    return;
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= rows) return;
   int c1 = min(cols, c0 + chunk);
   if (tri) c1 = min(c1, strict ? i : i + 1);
@@ -420,64 +445,99 @@ void gemv_n(const T* M, long ld, int rows, int cols, const double* v, double* ou
             int tri, int strict, int abs_mode, hipStream_t stream) {
   const int chunk = kGemvChunk;
   const int nparts = (cols + chunk - 1) / chunk;
-  dim3 grid((rows + 255) / 256, nparts);
+  constexpr int RV = 16 / (int)sizeof(T);
+  // the wide rows only where the launch keeps at least 2048 workgroups (the
+  // Svgp step's 2048 x 4096 products have 16 chunks and ran slower wide)
+  const long wide_blocks = (long)((rows + 256 * RV - 1) / (256 * RV)) * nparts;
+  const int vec = !tri && rows % RV == 0 && ld % RV == 0 &&
+                  reinterpret_cast<uintptr_t>(M) % 16 == 0 && wide_blocks >= 2048;
+  const int per_block = vec ? 256 * RV : 256;
+  dim3 grid((rows + per_block - 1) / per_block, nparts);
   hipLaunchKernelGGL((k_gemv_n_part<T>), grid, dim3(256), 0, stream, M, ld, rows, cols, chunk, v, part,
-                     tri, strict, abs_mode);
+                     tri, strict, abs_mode, vec);
   hipLaunchKernelGGL(k_sum_parts, dim3((rows + 63) / 64), dim3(256), 0, stream, part, rows, nparts, out);
   GPRX_HIP_CHECK(hipGetLastError());
 }
+
+// Columns per wave of k_gemv_t (at most): in the 16-byte path they share
+// each lane's loads of v (8 bytes of v per 4 bytes of an f32 column had held
+// the f32 product near 3.3 TB/s).  Launches that would then have fewer than
+// 2048 workgroups keep one column per wave.  Each column's sum keeps its own
+// order.
+constexpr int kGemvTCols = 4;
+
+static int gemv_t_cols_per_wave(int cols) { return cols >= 4 * kGemvTCols * 2048 ? kGemvTCols : 1; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_t(const T* __restrict__ M, long ld, int rows,
                                                 int cols, const double* __restrict__ v,
                                                 double* __restrict__ out, int tri, int strict,
-                                                int abs_mode, int accumulate, int sumsq) {
+                                                int abs_mode, int accumulate, int sumsq, int cpw) {
   const int lane = threadIdx.x & 63;
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= cols) return;
-  const int r0 = tri ? (strict ? j + 1 : j) : 0;
-  const T* col = M + (long)j * ld;
-  double acc = 0.0;
+  const int j0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * cpw;
+  if (j0 >= cols) return;
+  const int nc = min(cpw, cols - j0);
   constexpr int VEC = 16 / sizeof(T);
   using V = typename Vec<T>::V;
-  if (r0 == 0 && rows % (64 * VEC) == 0 && ld % VEC == 0) {
-    // whole 16-byte-aligned columns: one 16-byte load per lane and step
-    for (int i = lane * VEC; i < rows; i += 64 * VEC) {
-      const V mv = *reinterpret_cast<const V*>(col + i);
+  double acc[kGemvTCols];
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        const double m = (double)mv[e];
-        if (sumsq)
-          acc += m * m;
-        else
-          acc += (abs_mode ? fabs(m) : m) * v[i + e];
+  for (int c = 0; c < kGemvTCols; ++c) acc[c] = 0.0;
+  if (!tri && rows % (64 * VEC) == 0 && ld % VEC == 0 && reinterpret_cast<uintptr_t>(M) % 16 == 0) {
+    // whole 16-byte-aligned columns: one 16-byte load per lane, column and step
+    for (int i = lane * VEC; i < rows; i += 64 * VEC) {
+      double vi[VEC];
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) vi[e] = sumsq ? 0.0 : v[i + e];
+#pragma unroll
+      for (int c = 0; c < kGemvTCols; ++c) {
+        if (c < nc) {
+          const V mv = *reinterpret_cast<const V*>(M + (long)(j0 + c) * ld + i);
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const double m = (double)mv[e];
+            if (sumsq)
+              acc[c] += m * m;
+            else
+              acc[c] += (abs_mode ? fabs(m) : m) * vi[e];
+          }
+        }
       }
     }
   } else {
-    for (int i = r0 + lane; i < rows; i += 64) {
-      const double m = (double)col[i];
-      if (sumsq)
-        acc += m * m;
-      else
-        acc += (abs_mode ? fabs(m) : m) * v[i];
+    for (int c = 0; c < nc; ++c) {
+      const int j = j0 + c;
+      const int r0 = tri ? (strict ? j + 1 : j) : 0;
+      const T* col = M + (long)j * ld;
+      for (int i = r0 + lane; i < rows; i += 64) {
+        const double m = (double)col[i];
+        if (sumsq)
+          acc[c] += m * m;
+        else
+          acc[c] += (abs_mode ? fabs(m) : m) * v[i];
+      }
     }
   }
-  acc = wave_sum(acc);
-  if (lane == 0) out[j] = accumulate ? out[j] + acc : acc;
+#pragma unroll
+  for (int c = 0; c < kGemvTCols; ++c) {
+    const double a = wave_sum(acc[c]);
+    if (lane == 0 && c < nc) out[j0 + c] = accumulate ? out[j0 + c] + a : a;
+  }
 }
 
 template <typename T>
 void gemv_t(const T* M, long ld, int rows, int cols, const double* v, double* out, int tri,
             int strict, int abs_mode, int accumulate, hipStream_t stream) {
-  hipLaunchKernelGGL((k_gemv_t<T>), dim3((cols + 3) / 4), dim3(256), 0, stream, M, ld, rows, cols,
-                     v, out, tri, strict, abs_mode, accumulate, 0);
+  const int cpw = gemv_t_cols_per_wave(cols), per_block = 4 * cpw;
+  hipLaunchKernelGGL((k_gemv_t<T>), dim3((cols + per_block - 1) / per_block), dim3(256), 0, stream, M,
+                     ld, rows, cols, v, out, tri, strict, abs_mode, accumulate, 0, cpw);
   GPRX_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void col_sumsq(const T* M, long ld, int rows, int cols, double* out, hipStream_t stream) {
-  hipLaunchKernelGGL((k_gemv_t<T>), dim3((cols + 3) / 4), dim3(256), 0, stream, M, ld, rows, cols,
-                     (const double*)nullptr, out, 0, 0, 0, 0, 1);
+  const int cpw = gemv_t_cols_per_wave(cols), per_block = 4 * cpw;
+  hipLaunchKernelGGL((k_gemv_t<T>), dim3((cols + per_block - 1) / per_block), dim3(256), 0, stream, M,
+                     ld, rows, cols, (const double*)nullptr, out, 0, 0, 0, 0, 1, cpw);
   GPRX_HIP_CHECK(hipGetLastError());
 }
 

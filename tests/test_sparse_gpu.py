@@ -85,6 +85,20 @@ def test_sgpr_composite_vs_oracle(gpu, n, m, ls, seed):
         assert_close(grad, og, tol if prec == g.Precision.FP64 else 1e-2, "grad")
 
 
+@pytest.mark.parametrize("n,m,ls,seed", [(50, 1, 0.3, 3), (400, 128, 0.05, 7), (400, 129, 0.05, 8)])
+def test_sgpr_edge_inducing_counts_vs_oracle(gpu, n, m, ls, seed):
+    """One inducing point, exactly one tile of them, and one tile and one."""
+    print(f"[sgpr edge] n={n} m={m}", flush=True)
+    x, y, z, spec, th, n, m, d = well_conditioned(n, m, ls, seed)
+    for prec, tol in ((g.Precision.FP64, 1e-9), (g.Precision.MIXED, 1e-4)):
+        f = g.Sgpr(spec, g.GaussianLikelihood(0.02)).with_precision(prec).prepare(x, n, d, y, z, m)
+        grad = np.empty(th.size)
+        val = f.value_and_gradient_into(th, grad)
+        ov, og = oracle.vfe_value_grad(spec.program(), th, x, n, d, y, z, m)
+        assert_close(val, ov, tol, f"bound prec={prec}")
+        assert_close(grad, og, tol if prec == g.Precision.FP64 else 1e-2, f"grad prec={prec}")
+
+
 @pytest.mark.parametrize("name", ["svgp_forrester", "svgp_sphere"])
 def test_svgp_prior_goldens(gpu, name):
     gd = load_golden(name)
@@ -124,6 +138,23 @@ def test_svgp_random_q_vs_oracle(gpu, n, m, ls, seed, b):
         print(f"  prec={prec} value err {rel_err(val, ov):.2e} grad err {rel_err(grad, og):.2e}")
         assert_close(val, ov, tol, "-elbo")
         assert_close(grad, og, tol if prec == g.Precision.FP64 else 1e-2, "grad")
+
+
+@pytest.mark.parametrize("b", [1, 40])
+def test_svgp_one_point_and_full_batch_vs_oracle(gpu, b):
+    """A batch of one point and a batch of every point (n = 40, m = 6)."""
+    print(f"[svgp edge] b={b}", flush=True)
+    x, y, z, spec, th, n, m, d = well_conditioned(40, 6, 0.3, 4)
+    rng = np.random.default_rng(11)
+    Lq = np.tril(0.1 * rng.standard_normal((m, m))) + np.diag(0.5 + rng.random(m))
+    params = np.concatenate([th, 0.4 * rng.standard_normal(m), pack_lower(Lq)])
+    batch = rng.permutation(n)[:b]
+    ov, og = oracle.svgp_value_grad(spec.program(), params, x, n, d, y, z, m, batch)
+    f = g.Svgp(spec, g.GaussianLikelihood(0.02)).prepare(x, n, d, y, z, m)
+    grad = np.empty(params.size)
+    val = f.value_and_gradient_into(params, grad, batch)
+    assert_close(val, ov, 1e-9, "-elbo")
+    assert_close(grad, og, 1e-9, "grad")
 
 
 def test_svgp_adam_matches_oracle_loop(gpu):

@@ -93,6 +93,24 @@ def test_random_vs_oracle_fp64(gpu, n, d, kname):
     assert_close(p.variance, oo, TOL64, "var")
 
 
+@pytest.mark.parametrize("q", [1, 128, 129])
+def test_predict_query_counts_vs_oracle(gpu, q):
+    """One query point, exactly one tile of them, and one tile and one; both
+    variance kinds."""
+    print(f"[exact predict] q={q}", flush=True)
+    n, d = 300, 3
+    x, y, spec = _problem(n, d, 31, "const_ard_plus_periodic")
+    th = np.concatenate([spec.get_params(), [math.log(0.05)]])
+    m = _fit(spec, 0.05, x, n, d, y)
+    xs = np.random.default_rng(q).random(q * d)
+    p = m.predict(xs, q, d)
+    lat = m.predict_with(xs, q, d, g.VarianceKind.LATENT)
+    om, ol, oo = oracle.exact_predict(spec.program(), th, x, n, d, y, xs, q)
+    assert_close(p.mean, om, TOL64, "mean")
+    assert_close(p.variance, oo, TOL64, "obs var")
+    assert_close(lat.variance, ol, TOL64, "latent var")
+
+
 @pytest.mark.parametrize("n,d,kname,tol", [(300, 8, "const_ard", 1e-4), (1000, 8, "const_ard", 1e-4),
                                            (257, 1, "sum_times", 1e-4), (2500, 8, "const_ard", 1e-3)])
 def test_random_vs_oracle_mixed(gpu, n, d, kname, tol):

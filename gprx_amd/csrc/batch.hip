@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_fill_tasks(const FillTask* __restrict__
 // alpha_i alpha_j - Ainv_ij.  Partials per task (tasks of a problem are
 // consecutive).
 template <int DP, typename T, int NS>
-__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_tasks(const FillTask* __restrict__ tasks,
+__global__ __launch_bounds__(256, contract_occupancy(DP, NS)) void k_contract_tasks(const FillTask* __restrict__ tasks,
                                                         const ProbDesc* __restrict__ probs,
                                                         const DevProgram* __restrict__ progs,
                                                         const double* __restrict__ xar,

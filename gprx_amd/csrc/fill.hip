@@ -298,7 +298,7 @@ __global__ void k_fill_diag(const DevProgram* __restrict__ Pg, int na, double* _
 }
 
 template <int DP, typename T, int NS>
-__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_exact(const DevProgram* __restrict__ Pg,
+__global__ __launch_bounds__(256, contract_occupancy(DP, NS)) void k_contract_exact(const DevProgram* __restrict__ Pg,
                                                         const double* __restrict__ x, long ldx,
                                                         int n, const T* __restrict__ Ainv, long lda,
                                                         const double* __restrict__ alpha,
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_exact(
 }
 
 template <int DP, typename T, int NS>
-__global__ __launch_bounds__(256, contract_occupancy(DP)) void k_contract_cross(const DevProgram* __restrict__ Pg,
+__global__ __launch_bounds__(256, contract_occupancy(DP, NS)) void k_contract_cross(const DevProgram* __restrict__ Pg,
                                                         const double* __restrict__ a, long lda,
                                                         int na, const double* __restrict__ b,
                                                         long ldb, int nb, const T* __restrict__ W,

@@ -122,7 +122,9 @@ __device__ __forceinline__ void accumulate_entry(const DevProgram& P, const R (&
 // FP64 evaluation (n = 2^20, m = 1024, three leaf slots) from 101.8 to 96.9
 // ms; at 16 dimensions (the Svgp step) the bound cost 4%, and four
 // workgroups spilled.
-constexpr int contract_occupancy(int dp) { return dp <= 8 ? 3 : 1; }
+// Eight leaf slots get two (256 registers): under three their f64 kernels
+// spilled to scratch (up to 160 bytes per lane at 8 padded dimensions).
+constexpr int contract_occupancy(int dp, int ns) { return dp <= 8 ? (ns <= 4 ? 3 : 2) : 1; }
 
 // Block reduction of the accumulators (widened to f64) into
 // partial[tile * kMaxAcc + k] (slots from NS on hold zeros).

@@ -60,6 +60,10 @@ def _problem(n, d, seed, kname):
         "const_ard": g.ConstantKernel(1.3) * g.RbfArdKernel(ls),
         "const_ard_plus_periodic": g.ConstantKernel(0.8) * g.RbfArdKernel(ls) + g.PeriodicKernel(1.1, 2.3),
         "sum_times": (g.RbfKernel(0.5) + g.PeriodicKernel(1.0, 1.5)) * (g.ConstantKernel(2.0) + g.RbfKernel(1.5)),
+        # six leaf factors over three product terms: the contraction's
+        # eight-slot kernels below their full eight
+        "six_leaves": (g.ConstantKernel(0.8) * g.RbfArdKernel(ls) + g.RbfKernel(0.7) * g.PeriodicKernel(1.1, 2.3)
+                       + g.ConstantKernel(0.3) * g.RbfKernel(1.5)),
     }
     return x, y, specs[kname]
 
@@ -70,7 +74,7 @@ def _problem(n, d, seed, kname):
 # takes about 15 s there).
 CASES = [(1, 1, "rbf"), (7, 2, "const_ard"), (128, 3, "const_ard"), (129, 4, "const_ard_plus_periodic"),
          (300, 8, "const_ard"), (257, 1, "sum_times"), (1000, 8, "const_ard"), (700, 16, "const_ard"),
-         (2500, 8, "const_ard"), (4700, 8, "const_ard")]
+         (2500, 8, "const_ard"), (4700, 8, "const_ard"), (300, 1, "six_leaves"), (200, 2, "six_leaves")]
 
 
 @pytest.mark.parametrize("n,d,kname", CASES)

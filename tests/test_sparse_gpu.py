@@ -85,6 +85,22 @@ def test_sgpr_composite_vs_oracle(gpu, n, m, ls, seed):
         assert_close(grad, og, tol if prec == g.Precision.FP64 else 1e-2, "grad")
 
 
+def test_sgpr_six_leaf_factors_vs_oracle(gpu):
+    """Six leaf factors over three product terms (the cross contraction's
+    eight-slot kernel)."""
+    x, y, z, _, _, n, m, d = well_conditioned(600, 40, 0.1, 12)
+    spec = (g.ConstantKernel(0.8) * g.RbfArdKernel([0.1, 0.12]) + g.RbfKernel(0.15) * g.PeriodicKernel(0.3, 1.5)
+            + g.ConstantKernel(0.3) * g.RbfKernel(0.1))
+    th = np.concatenate([spec.get_params(), [math.log(0.02)]])
+    for prec, tol in ((g.Precision.FP64, 1e-9), (g.Precision.MIXED, 1e-4)):
+        f = g.Sgpr(spec, g.GaussianLikelihood(0.02)).with_precision(prec).prepare(x, n, d, y, z, m)
+        grad = np.empty(th.size)
+        val = f.value_and_gradient_into(th, grad)
+        ov, og = oracle.vfe_value_grad(spec.program(), th, x, n, d, y, z, m)
+        assert_close(val, ov, tol, f"bound prec={prec}")
+        assert_close(grad, og, tol if prec == g.Precision.FP64 else 1e-2, f"grad prec={prec}")
+
+
 @pytest.mark.parametrize("n,m,ls,seed", [(50, 1, 0.3, 3), (400, 128, 0.05, 7), (400, 129, 0.05, 8)])
 def test_sgpr_edge_inducing_counts_vs_oracle(gpu, n, m, ls, seed):
     """One inducing point, exactly one tile of them, and one tile and one."""
